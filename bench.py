@@ -1,0 +1,165 @@
+#!/usr/bin/env python3
+"""SpMM throughput on one MI355X: Y = A*X with A the R-MAT scale-22,
+edge-factor-16 binary adjacency (duplicates and self edges removed) in the
+reference's 2D tile format, X a 4,194,304 x 16 FP32 operand U[0,1).
+
+One step = one fmx_spmm over the whole matrix with A, X and Y resident in
+HBM.  value = algorithmic bytes / step time, where algorithmic bytes =
+tile bytes + 8*(block rows + 1) + X read once + Y written once.  The CPU
+baseline is the oracle's multi-threaded restatement of the reference engine
+on the same image and X, on this host's cores.
+
+python bench.py [--gpus 1] [--steps 20] [--warmup 5] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("SpMM HBM GB/s (% peak); eigensolve sec to 1e-6 residual, learned vs plain")
+PEAK_HBM_GBS = 8000.0
+SCALE, EDGE_FACTOR, K, SEED = 22, 16, 16, 22
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def load_traffic(workload, tile):
+    """HBM bytes per launch measured with rocprofv3 --pmc (committed summary)."""
+    p = os.path.join(ROOT, "profiles", "spmm_traffic.json")
+    try:
+        d = json.load(open(p))
+        e = d.get("%s/%s" % (workload, tile))
+        return e["hbm_bytes_per_launch"] if e else None
+    except Exception:
+        return None
+
+
+def cpu_baseline(data, offs, n, X_host, threads, reps):
+    from oracle import oracle as O
+    img = O.Image(data, offs, n)
+    O.spmm(img, X_host, threads=threads)  # warm-up
+    times = []
+    Y = None
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        Y = O.spmm(img, X_host, threads=threads)
+        times.append(time.perf_counter() - t0)
+    return statistics.median(times), Y
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--log2-h", type=int, default=int(os.environ.get("FMX_BENCH_LOG2H", "8")))
+    ap.add_argument("--log2-w", type=int, default=15)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-reps", type=int, default=5)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(--gpus %d requested)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    import flashx_amd.fmx as F
+
+    ctx = F.default_context(0)
+    n = 1 << SCALE
+    m = EDGE_FACTOR * n
+    t0 = time.perf_counter()
+    src, dst = F.rmat_edges(SCALE, m, SEED)
+    torch.cuda.synchronize()
+    spm = F.SparseMatrix.build(src, dst, n=n, log2_h=args.log2_h, log2_w=args.log2_w,
+                               flags=F.BUILD_DEDUP)
+    t_build = time.perf_counter() - t0
+    del src, dst
+    info = spm.info
+    log("built: n=%d nnz=%d tiles=%d image=%d B (%.2f s incl. host validation)"
+        % (info.nrows, info.nnz, info.ntiles, info.image_bytes, t_build))
+    g = torch.Generator(device="cuda")
+    g.manual_seed(SEED)
+    X = torch.rand((n, K), dtype=torch.float32, device="cuda", generator=g)
+    Y = torch.empty((n, K), dtype=torch.float32, device="cuda")
+    alg_bytes = spm.alg_bytes(torch.float32, K)
+
+    stream = torch.cuda.current_stream()
+    for _ in range(args.warmup):
+        spm.multiply_async(X, Y)
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(args.steps)]
+    t_start = torch.cuda.Event(enable_timing=True)
+    t_end = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t_start.record(stream)
+    for i in range(args.steps):
+        ev[i][0].record(stream)
+        spm.multiply_async(X, Y)
+        ev[i][1].record(stream)
+    t_end.record(stream)
+    torch.cuda.synchronize()
+    total_ms = t_start.elapsed_time(t_end)
+    per = [a.elapsed_time(b) for a, b in ev]
+    ms_per_step = total_ms / args.steps
+    avg_launch_ms = sum(per) / len(per)
+    value = alg_bytes / (ms_per_step * 1e-3) / 1e9
+    achieved = alg_bytes / (avg_launch_ms * 1e-3) / 1e9
+    log("spmm: %.1f us/step (events per launch avg %.1f us, min %.1f)" %
+        (ms_per_step * 1e3, avg_launch_ms * 1e3, min(per) * 1e3))
+
+    workload = "rmat%d_ef%d_f32_k%d" % (SCALE, EDGE_FACTOR, K)
+    tile = "%dx%d" % (1 << args.log2_h, 1 << args.log2_w)
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: seeded R-MAT (a,b,c,d)=(.57,.19,.19,.05), X U[0,1) seeded",
+        "config": {"workload": "spmm " + workload, "scale": SCALE, "edge_factor": EDGE_FACTOR,
+                   "k": K, "tile": tile, "nnz": int(info.nnz),
+                   "image_bytes": int(info.image_bytes), "alg_bytes_per_step": int(alg_bytes),
+                   "pct_hbm_peak": round(100.0 * value / PEAK_HBM_GBS, 2)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS,
+                     "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
+                     "traffic": load_traffic(workload, tile)},
+        "cpu_baseline": None,
+    }
+
+    if not args.no_cpu:
+        threads = len(os.sched_getaffinity(0))
+        data, offs = spm.download()
+        Xh = X.cpu().numpy()
+        t_cpu, Ycpu = cpu_baseline(data, offs, n, Xh, threads, args.cpu_reps)
+        Yg = Y.cpu().numpy()
+        ref = Ycpu.astype(np.float64)
+        nz = ref != 0
+        rel = float(np.max(np.abs(Yg.astype(np.float64) - ref)[nz] / np.abs(ref[nz])))
+        zeros_ok = bool(np.all(Yg[~nz] == 0))
+        out["cpu_baseline"] = {
+            "value": round(alg_bytes / t_cpu / 1e9, 3), "unit": "GB/s", "cores": threads,
+            "kind": "port",
+            "sample": "full image and X, 1 warm-up + median of %d multiply() calls, "
+                      "seconds per SpMM %.4f" % (args.cpu_reps, t_cpu)}
+        out["check"] = {"max_rel_err_vs_cpu": rel, "zero_rows_exact": zeros_ok}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,466 @@
+// build_tiles.hip -- on-device construction of the reference's 2D tile image
+// from an edge list (fm::create_2d_matrix, matrix/fm_utils.cpp:617-715,
+// 726-841), bit-exact with the reference writer.
+//
+// The reference sorts the edge list by source, groups rows into block rows,
+// sorts each row's neighbours, optionally de-duplicates, and then for every
+// block row sizes (collect_block_info, fm_utils.cpp:267-327) and serialises
+// (run_blocks/add_nz, fm_utils.cpp:228-265,329-433) its tiles.  Here the same
+// image comes out of data-parallel steps:
+//   1. keys = (block row, block col, row in tile, col in tile) packed in a u64,
+//      self edges dropped unless kept (fm_utils.cpp:450-452), both directions
+//      when symmetrising;
+//   2. stable LSD radix sort of the keys (tile order = image order), then
+//      std::unique-style compaction when de-duplicating (fm_utils.cpp:460-464);
+//   3. runs = maximal groups of one row inside one tile; a run of one non-zero
+//      is a COO entry, a longer run an SCSR row part (add_nz, fm_utils.cpp:
+//      235-264);
+//   4. per tile: nnz, nrow, num_coo, SCSR words -> tile size
+//      (sparse_block_2d::get_size, sparse_matrix_format.h:294-299); per block
+//      row: size or a 16-byte empty tile (fm_utils.cpp:341-351); offsets by
+//      exclusive scans (fm_utils.cpp:701-712);
+//   5. every key writes its own u16 words; every tile writes its header and
+//      end marker (sparse_block_2d::finalize, sparse_matrix_format.cpp:119-126).
+#include <cstring>
+
+#include "devprim.h"
+
+namespace fmx {
+namespace {
+
+constexpr int kB = 256;
+
+struct KeyFmt {
+    int lh, lw, bbc;   // log2 H, log2 W, bits of the block-column field
+    uint32_t H, W;
+    __host__ __device__ uint64_t key(uint32_t s, uint32_t d) const
+    {
+        return ((uint64_t)(s >> lh) << (bbc + lh + lw)) | ((uint64_t)(d >> lw) << (lh + lw)) |
+               ((uint64_t)(s & (H - 1)) << lw) | (uint64_t)(d & (W - 1));
+    }
+    __host__ __device__ uint32_t br(uint64_t k) const { return (uint32_t)(k >> (bbc + lh + lw)); }
+    __host__ __device__ uint32_t bc(uint64_t k) const
+    {
+        return (uint32_t)((k >> (lh + lw)) & ((1ull << bbc) - 1));
+    }
+    __host__ __device__ uint32_t rrel(uint64_t k) const { return (uint32_t)((k >> lw) & (H - 1)); }
+    __host__ __device__ uint32_t crel(uint64_t k) const { return (uint32_t)(k & (W - 1)); }
+};
+
+__device__ __forceinline__ uint64_t gtid() { return blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; }
+__device__ __forceinline__ uint64_t gstride() { return (uint64_t)gridDim.x * blockDim.x; }
+
+__global__ void __launch_bounds__(kB) make_keys(const uint32_t *src, const uint32_t *dst, uint64_t m,
+                                                 KeyFmt f, int keep_self, int sym, uint64_t *keys,
+                                                 unsigned long long *counter)
+{
+    const int lane = threadIdx.x & 63;
+    for (uint64_t base = blockIdx.x * (uint64_t)kB; base < m; base += gstride()) {
+        const uint64_t i = base + threadIdx.x;
+        uint32_t s = 0, d = 0;
+        bool ok = false;
+        if (i < m) {
+            s = src[i];
+            d = dst[i];
+            ok = keep_self || s != d;
+        }
+        const bool two = ok && sym && s != d;
+        const uint32_t cnt = ok ? (two ? 2u : 1u) : 0u;
+        // wave-aggregated reservation
+        uint32_t inc = cnt;
+#pragma unroll
+        for (int dd = 1; dd < 64; dd <<= 1) {
+            uint32_t o = __shfl_up(inc, dd, 64);
+            if (lane >= dd) inc += o;
+        }
+        const uint32_t wtot = __shfl(inc, 63, 64);
+        unsigned long long wbase = 0;
+        if (lane == 0 && wtot) wbase = atomicAdd(counter, (unsigned long long)wtot);
+        wbase = __shfl(wbase, 0, 64);
+        uint64_t pos = wbase + inc - cnt;
+        if (ok) keys[pos] = f.key(s, d);
+        if (two) keys[pos + 1] = f.key(d, s);
+    }
+}
+
+// flag[i] = 1 when keys[i] starts a new group under `shift` (keys >> shift)
+__global__ void __launch_bounds__(kB) group_flags(const uint64_t *k, uint64_t n, int shift, uint32_t *flag)
+{
+    for (uint64_t i = gtid(); i < n; i += gstride())
+        flag[i] = (i == 0 || (k[i] >> shift) != (k[i - 1] >> shift)) ? 1u : 0u;
+}
+
+__global__ void __launch_bounds__(kB) compact_unique(const uint64_t *k, uint64_t n, const uint32_t *ex,
+                                                     uint64_t *out)
+{
+    for (uint64_t i = gtid(); i < n; i += gstride())
+        if (i == 0 || k[i] != k[i - 1]) out[ex[i]] = k[i];
+}
+
+// run_start[r] = first key of run r; run_start[R] = nk
+__global__ void __launch_bounds__(kB) fill_run_start(const uint64_t *k, uint64_t nk, int lw,
+                                                     const uint32_t *ex, uint32_t *run_start,
+                                                     uint32_t R)
+{
+    for (uint64_t i = gtid(); i < nk; i += gstride())
+        if (i == 0 || (k[i] >> lw) != (k[i - 1] >> lw)) run_start[ex[i]] = (uint32_t)i;
+    if (gtid() == 0) run_start[R] = (uint32_t)nk;
+}
+
+// per run: tile-start flag, SCSR words, COO flag
+__global__ void __launch_bounds__(kB) run_props(const uint64_t *k, const uint32_t *run_start, uint32_t R,
+                                                int tshift, uint32_t *tflag, uint64_t *scsr_w,
+                                                uint32_t *coo_f)
+{
+    for (uint64_t r = gtid(); r < R; r += gstride()) {
+        const uint32_t s0 = run_start[r], len = run_start[r + 1] - s0;
+        tflag[r] = (r == 0 || (k[s0] >> tshift) != (k[run_start[r - 1]] >> tshift)) ? 1u : 0u;
+        scsr_w[r] = len >= 2 ? (uint64_t)len + 1 : 0;
+        coo_f[r] = len == 1 ? 1u : 0u;
+    }
+    if (gtid() == 0) {
+        scsr_w[R] = 0;
+        coo_f[R] = 0;
+    }
+}
+
+// tile_first_run[t], tile_of_run[r]; tile_first_run[T] = R
+__global__ void __launch_bounds__(kB) fill_tiles(const uint64_t *k, const uint32_t *run_start, uint32_t R,
+                                                 int tshift, const uint32_t *tex,
+                                                 uint32_t *tile_first_run, uint32_t *tile_of_run,
+                                                 uint32_t T)
+{
+    for (uint64_t r = gtid(); r < R; r += gstride()) {
+        const bool first = r == 0 || (k[run_start[r]] >> tshift) != (k[run_start[r - 1]] >> tshift);
+        const uint32_t t = tex[r] + (first ? 1u : 0u) - 1u;
+        tile_of_run[r] = t;
+        if (first) tile_first_run[t] = (uint32_t)r;
+    }
+    if (gtid() == 0) tile_first_run[T] = R;
+}
+
+__global__ void __launch_bounds__(kB) tile_props(const uint64_t *k, KeyFmt f, const uint32_t *run_start,
+                                                 const uint32_t *tile_first_run, uint32_t T,
+                                                 const uint64_t *scsr_ex, const uint32_t *coo_ex,
+                                                 uint64_t *tsize, unsigned long long *brsize)
+{
+    for (uint64_t t = gtid(); t < T; t += gstride()) {
+        const uint32_t fr = tile_first_run[t], lr = tile_first_run[t + 1];
+        const uint64_t nnz = run_start[lr] - run_start[fr];
+        const uint64_t nrow = lr - fr;
+        const uint64_t size = 16 + 2 * (nrow + nnz + 1);
+        tsize[t] = size;
+        atomicAdd(&brsize[f.br(k[run_start[fr]])], (unsigned long long)size);
+    }
+    if (gtid() == 0) tsize[T] = 0;
+}
+
+// empty block rows take a 16-byte empty tile; keep a copy of "had tiles"
+__global__ void __launch_bounds__(kB) fix_empty_rows(unsigned long long *brsize, uint64_t nbr,
+                                                     uint8_t *had)
+{
+    for (uint64_t b = gtid(); b < nbr; b += gstride()) {
+        had[b] = brsize[b] != 0;
+        if (!brsize[b]) brsize[b] = 16;
+    }
+    if (gtid() == 0) brsize[nbr] = 0;
+}
+
+__global__ void __launch_bounds__(kB) first_tile_of_row(const uint64_t *k, KeyFmt f,
+                                                        const uint32_t *run_start,
+                                                        const uint32_t *tile_first_run, uint32_t T,
+                                                        uint32_t *br_first)
+{
+    for (uint64_t t = gtid(); t < T; t += gstride()) {
+        const uint32_t b = f.br(k[run_start[tile_first_run[t]]]);
+        if (t == 0 || f.br(k[run_start[tile_first_run[t - 1]]]) != b) br_first[b] = (uint32_t)t;
+    }
+}
+
+__device__ __forceinline__ void put16(uint8_t *img, uint64_t off, uint32_t v)
+{
+    *(uint16_t *)(img + off) = (uint16_t)v;
+}
+
+__global__ void __launch_bounds__(kB) write_tiles(const uint64_t *k, KeyFmt f, const uint32_t *run_start,
+                                                  const uint32_t *tile_first_run, uint32_t T,
+                                                  const uint64_t *scsr_ex, const uint32_t *coo_ex,
+                                                  const uint64_t *offs_ex, const uint64_t *size_ex,
+                                                  const uint32_t *br_first, uint64_t *tile_off,
+                                                  uint8_t *img)
+{
+    for (uint64_t t = gtid(); t < T; t += gstride()) {
+        const uint32_t fr = tile_first_run[t], lr = tile_first_run[t + 1];
+        const uint64_t key0 = k[run_start[fr]];
+        const uint32_t b = f.br(key0), c = f.bc(key0);
+        const uint64_t off = 4096 + offs_ex[b] + size_ex[t] - size_ex[br_first[b]];
+        tile_off[t] = off;
+        const uint32_t nnz = run_start[lr] - run_start[fr];
+        const uint32_t nrow = lr - fr;
+        const uint32_t ncoo = coo_ex[lr] - coo_ex[fr];
+        const uint64_t sw = scsr_ex[lr] - scsr_ex[fr];
+        put16(img, off + 0, b & 0xFFFF);
+        put16(img, off + 2, b >> 16);
+        put16(img, off + 4, c & 0xFFFF);
+        put16(img, off + 6, c >> 16);
+        put16(img, off + 8, nnz & 0xFFFF);
+        put16(img, off + 10, nnz >> 16);
+        put16(img, off + 12, nrow);
+        put16(img, off + 14, ncoo);
+        put16(img, off + 16 + 2 * sw, 0xFFFF);
+    }
+}
+
+__global__ void __launch_bounds__(kB) write_stubs(const uint8_t *had, uint64_t nbr, const uint64_t *offs_ex,
+                                                  uint8_t *img)
+{
+    for (uint64_t b = gtid(); b < nbr; b += gstride()) {
+        if (had[b]) continue;
+        const uint64_t off = 4096 + offs_ex[b];
+        put16(img, off + 0, (uint32_t)b & 0xFFFF);
+        put16(img, off + 2, (uint32_t)(b >> 16));
+        for (int i = 4; i < 16; i += 2) put16(img, off + i, 0);
+    }
+}
+
+__global__ void __launch_bounds__(kB) write_nz(const uint64_t *k, uint64_t nk, KeyFmt f,
+                                               const uint32_t *rex, const uint32_t *run_start,
+                                               const uint32_t *tile_of_run,
+                                               const uint32_t *tile_first_run,
+                                               const uint64_t *scsr_ex, const uint32_t *coo_ex,
+                                               const uint64_t *tile_off, uint8_t *img)
+{
+    for (uint64_t i = gtid(); i < nk; i += gstride()) {
+        const uint64_t key = k[i];
+        const bool first = i == 0 || (key >> f.lw) != (k[i - 1] >> f.lw);
+        const uint32_t r = rex[i] + (first ? 1u : 0u) - 1u;
+        const uint32_t t = tile_of_run[r];
+        const uint32_t fr = tile_first_run[t], lr = tile_first_run[t + 1];
+        const uint32_t s0 = run_start[r], len = run_start[r + 1] - s0;
+        const uint64_t off = tile_off[t];
+        if (len == 1) {
+            const uint64_t sw = scsr_ex[lr] - scsr_ex[fr];
+            const uint64_t pos = off + 16 + 2 * (sw + 1) + 4ull * (coo_ex[r] - coo_ex[fr]);
+            put16(img, pos, 0x8000u | f.rrel(key));
+            put16(img, pos + 2, f.crel(key));
+        } else {
+            const uint64_t sb = off + 16 + 2 * (scsr_ex[r] - scsr_ex[fr]);
+            if (i == s0) put16(img, sb, 0x8000u | f.rrel(key));
+            put16(img, sb + 2 * (1 + (i - s0)), f.crel(key));
+        }
+    }
+}
+
+int bits_for(uint64_t n)
+{
+    int b = 0;
+    while (b < 64 && (1ull << b) < n) b++;
+    return b;
+}
+
+uint32_t grid_for(fmx_ctx *ctx, uint64_t n)
+{
+    uint64_t b = (n + kB - 1) / kB;
+    uint64_t cap = (uint64_t)ctx->num_cus * 16;
+    if (b > cap) b = cap;
+    return (uint32_t)(b ? b : 1);
+}
+
+}  // namespace
+
+fmx_status finish_device_image(fmx_ctx *ctx, fmx_spm *s);
+
+}  // namespace fmx
+
+using namespace fmx;
+
+extern "C" fmx_status fmx_spm_build(fmx_ctx *ctx, const uint32_t *src, const uint32_t *dst,
+                                    size_t m, uint64_t n, uint32_t log2_h, uint32_t log2_w,
+                                    uint32_t flags, fmx_spm **out)
+{
+    if (!ctx || !out) return fail(ctx, FMX_EINVAL, "null argument");
+    *out = nullptr;
+    if (m && (!src || !dst)) return fail(ctx, FMX_EINVAL, "null edge list");
+    if (log2_h > 15 || log2_w > 15) return fail(ctx, FMX_EINVAL, "tile dimensions must be <= 32768");
+    if (flags & ~7u) return fail(ctx, FMX_EINVAL, "unknown build flags");
+    FMX_HIP(ctx, hipSetDevice(ctx->device));
+    // the edge list may come from any stream: construction is synchronous
+    FMX_HIP(ctx, hipDeviceSynchronize());
+    hipStream_t s = ctx->stream;
+
+    // n = max id + 1 (fm_utils.cpp:730-739), or a caller-given n above every id
+    if (m) {
+        uint32_t mx = 0;
+        fmx_status st = max_u32_pair(ctx, src, dst, m, &mx, s);
+        if (st) return st;
+        if (n == 0) n = (uint64_t)mx + 1;
+        else if ((uint64_t)mx >= n) return fail(ctx, FMX_EDIM, "vertex id >= n");
+    } else if (n == 0) {
+        return fail(ctx, FMX_EINVAL, "empty edge list needs an explicit n");
+    }
+    if (n > (1ull << 32)) return fail(ctx, FMX_EINVAL, "n exceeds 2^32");
+    const uint64_t H = 1ull << log2_h, W = 1ull << log2_w;
+    const uint64_t nbr = (n + H - 1) / H, nbc = (n + W - 1) / W;
+    KeyFmt f;
+    f.lh = (int)log2_h;
+    f.lw = (int)log2_w;
+    f.bbc = bits_for(nbc);
+    f.H = (uint32_t)H;
+    f.W = (uint32_t)W;
+    const int keybits = bits_for(nbr) + f.bbc + f.lh + f.lw;
+    const int sym = (flags & FMX_BUILD_SYMMETRIZE) != 0;
+    const uint64_t cap = sym ? 2 * (uint64_t)m : (uint64_t)m;
+
+    // 1. keys
+    DevBuf<uint64_t> keys, tmp;
+    DevBuf<unsigned long long> counter;
+    FMX_HIP(ctx, keys.alloc(cap));
+    FMX_HIP(ctx, tmp.alloc(cap));
+    FMX_HIP(ctx, counter.alloc(1));
+    FMX_HIP(ctx, hipMemsetAsync(counter.p, 0, sizeof(unsigned long long), s));
+    if (m) {
+        hipLaunchKernelGGL(make_keys, dim3(grid_for(ctx, m)), dim3(kB), 0, s, src, dst, (uint64_t)m, f,
+                           (flags & FMX_BUILD_KEEP_SELF) ? 1 : 0, sym, keys.p, counter.p);
+        FMX_HIP(ctx, hipGetLastError());
+    }
+    unsigned long long nk64 = 0;
+    FMX_HIP(ctx, hipMemcpyAsync(&nk64, counter.p, sizeof(nk64), hipMemcpyDeviceToHost, s));
+    FMX_HIP(ctx, hipStreamSynchronize(s));
+    if (nk64 >= (1ull << 32)) return fail(ctx, FMX_EUNSUPPORTED, "more than 2^32 - 1 non-zeros");
+
+    // 2. sort + unique
+    uint64_t *K = keys.p;
+    fmx_status st = radix_sort_u64(ctx, keys.p, tmp.p, nk64, keybits, &K, s);
+    if (st) return st;
+    uint64_t *other = (K == keys.p) ? tmp.p : keys.p;
+    uint64_t nk = nk64;
+    DevBuf<uint32_t> flag;
+    FMX_HIP(ctx, flag.alloc(nk + 1));
+    if ((flags & FMX_BUILD_DEDUP) && nk) {
+        hipLaunchKernelGGL(group_flags, dim3(grid_for(ctx, nk)), dim3(kB), 0, s, K, nk, 0, flag.p);
+        FMX_HIP(ctx, hipGetLastError());
+        uint32_t nu = 0;
+        st = scan_exclusive<uint32_t>(ctx, flag.p, flag.p, nk, &nu, s);
+        if (st) return st;
+        hipLaunchKernelGGL(compact_unique, dim3(grid_for(ctx, nk)), dim3(kB), 0, s, K, nk, flag.p, other);
+        FMX_HIP(ctx, hipGetLastError());
+        K = other;
+        nk = nu;
+    }
+
+    // 3. runs
+    uint32_t R = 0;
+    if (nk) {
+        hipLaunchKernelGGL(group_flags, dim3(grid_for(ctx, nk)), dim3(kB), 0, s, K, nk, f.lw, flag.p);
+        FMX_HIP(ctx, hipGetLastError());
+        st = scan_exclusive<uint32_t>(ctx, flag.p, flag.p, nk, &R, s);
+        if (st) return st;
+    }
+    DevBuf<uint32_t> run_start, tflag, coo_ex, tile_of_run;
+    DevBuf<uint64_t> scsr_ex;
+    FMX_HIP(ctx, run_start.alloc((uint64_t)R + 1));
+    FMX_HIP(ctx, tflag.alloc((uint64_t)R + 1));
+    FMX_HIP(ctx, coo_ex.alloc((uint64_t)R + 1));
+    FMX_HIP(ctx, scsr_ex.alloc((uint64_t)R + 1));
+    FMX_HIP(ctx, tile_of_run.alloc((uint64_t)R + 1));
+    const int tshift = f.lh + f.lw;
+    uint32_t T = 0;
+    if (nk) {
+        hipLaunchKernelGGL(fill_run_start, dim3(grid_for(ctx, nk)), dim3(kB), 0, s, K, nk, f.lw, flag.p,
+                           run_start.p, R);
+        FMX_HIP(ctx, hipGetLastError());
+        hipLaunchKernelGGL(run_props, dim3(grid_for(ctx, R)), dim3(kB), 0, s, K, run_start.p, R, tshift,
+                           tflag.p, scsr_ex.p, coo_ex.p);
+        FMX_HIP(ctx, hipGetLastError());
+        st = scan_exclusive<uint32_t>(ctx, tflag.p, tflag.p, R, &T, s);
+        if (st) return st;
+        st = scan_exclusive<uint64_t>(ctx, scsr_ex.p, scsr_ex.p, (uint64_t)R + 1, nullptr, s);
+        if (st) return st;
+        st = scan_exclusive<uint32_t>(ctx, coo_ex.p, coo_ex.p, (uint64_t)R + 1, nullptr, s);
+        if (st) return st;
+    }
+
+    // 4. tiles and block rows
+    DevBuf<uint32_t> tile_first_run, br_first;
+    DevBuf<uint64_t> tsize, tile_off, offs_ex;
+    DevBuf<unsigned long long> brsize;
+    DevBuf<uint8_t> had;
+    FMX_HIP(ctx, tile_first_run.alloc((uint64_t)T + 1));
+    FMX_HIP(ctx, tsize.alloc((uint64_t)T + 1));
+    FMX_HIP(ctx, tile_off.alloc((uint64_t)T + 1));
+    FMX_HIP(ctx, brsize.alloc(nbr + 1));
+    FMX_HIP(ctx, offs_ex.alloc(nbr + 1));
+    FMX_HIP(ctx, br_first.alloc(nbr + 1));
+    FMX_HIP(ctx, had.alloc(nbr + 1));
+    FMX_HIP(ctx, hipMemsetAsync(brsize.p, 0, (nbr + 1) * sizeof(unsigned long long), s));
+    if (T) {
+        hipLaunchKernelGGL(fill_tiles, dim3(grid_for(ctx, R)), dim3(kB), 0, s, K, run_start.p, R, tshift,
+                           tflag.p, tile_first_run.p, tile_of_run.p, T);
+        FMX_HIP(ctx, hipGetLastError());
+        hipLaunchKernelGGL(tile_props, dim3(grid_for(ctx, T)), dim3(kB), 0, s, K, f, run_start.p,
+                           tile_first_run.p, T, scsr_ex.p, coo_ex.p, tsize.p, brsize.p);
+        FMX_HIP(ctx, hipGetLastError());
+        hipLaunchKernelGGL(first_tile_of_row, dim3(grid_for(ctx, T)), dim3(kB), 0, s, K, f, run_start.p,
+                           tile_first_run.p, T, br_first.p);
+        FMX_HIP(ctx, hipGetLastError());
+    }
+    hipLaunchKernelGGL(fix_empty_rows, dim3(grid_for(ctx, nbr)), dim3(kB), 0, s, brsize.p, nbr, had.p);
+    FMX_HIP(ctx, hipGetLastError());
+    uint64_t tiles_total = 0;
+    st = scan_exclusive<uint64_t>(ctx, (const uint64_t *)brsize.p, offs_ex.p, nbr + 1, &tiles_total, s);
+    if (st) return st;
+    if (T) {
+        st = scan_exclusive<uint64_t>(ctx, tsize.p, tsize.p, (uint64_t)T + 1, nullptr, s);
+        if (st) return st;
+    }
+
+    // 5. serialise
+    std::unique_ptr<fmx_spm> spm(new fmx_spm());
+    spm->ctx = ctx;
+    const uint64_t bytes = 4096 + tiles_total;
+    const uint64_t padded = (bytes + 4095) & ~4095ull;
+    FMX_HIP(ctx, hipMalloc(&spm->d_image, padded));
+    FMX_HIP(ctx, hipMemsetAsync(spm->d_image + bytes, 0, padded - bytes, s));
+    {
+        uint8_t hdr[4096];
+        memset(hdr, 0, sizeof(hdr));
+        const int64_t magic = 0x0123456789FFFFFFLL;
+        const int32_t ver = 2, type = 2, layout = 2, prim = 0;
+        const uint64_t esz = 0;
+        memcpy(hdr + 0, &magic, 8);
+        memcpy(hdr + 8, &ver, 4);
+        memcpy(hdr + 12, &type, 4);
+        memcpy(hdr + 16, &esz, 8);
+        memcpy(hdr + 24, &n, 8);
+        memcpy(hdr + 32, &n, 8);
+        memcpy(hdr + 40, &layout, 4);
+        memcpy(hdr + 44, &prim, 4);
+        memcpy(hdr + 48, &H, 8);
+        memcpy(hdr + 56, &W, 8);
+        FMX_HIP(ctx, hipMemcpyAsync(spm->d_image, hdr, 4096, hipMemcpyHostToDevice, s));
+        FMX_HIP(ctx, hipStreamSynchronize(s));
+    }
+    if (T) {
+        hipLaunchKernelGGL(write_tiles, dim3(grid_for(ctx, T)), dim3(kB), 0, s, K, f, run_start.p,
+                           tile_first_run.p, T, scsr_ex.p, coo_ex.p, offs_ex.p, tsize.p, br_first.p,
+                           tile_off.p, spm->d_image);
+        FMX_HIP(ctx, hipGetLastError());
+        hipLaunchKernelGGL(write_nz, dim3(grid_for(ctx, nk)), dim3(kB), 0, s, K, nk, f, flag.p,
+                           run_start.p, tile_of_run.p, tile_first_run.p, scsr_ex.p, coo_ex.p,
+                           tile_off.p, spm->d_image);
+        FMX_HIP(ctx, hipGetLastError());
+    }
+    hipLaunchKernelGGL(write_stubs, dim3(grid_for(ctx, nbr)), dim3(kB), 0, s, had.p, nbr, offs_ex.p,
+                       spm->d_image);
+    FMX_HIP(ctx, hipGetLastError());
+    std::vector<uint64_t> oex(nbr + 1);
+    FMX_HIP(ctx, hipMemcpyAsync(oex.data(), offs_ex.p, (nbr + 1) * 8, hipMemcpyDeviceToHost, s));
+    FMX_HIP(ctx, hipStreamSynchronize(s));
+    spm->offs.resize(nbr + 1);
+    for (uint64_t b = 0; b <= nbr; b++) spm->offs[b] = (int64_t)(4096 + oex[b]);
+    spm->bytes = bytes;
+    st = finish_device_image(ctx, spm.get());
+    if (st) return st;
+    *out = spm.release();
+    return FMX_OK;
+}

@@ -1,0 +1,19 @@
+// dense.h -- internal entry points of the tall-skinny dense operations.
+#pragma once
+
+#include "fmx_internal.h"
+
+namespace fmx {
+
+fmx_status ctx_scratch(fmx_ctx *ctx, size_t bytes, void **out);
+fmx_status gram(fmx_ctx *ctx, fmx_dtype dt, int mode, uint64_t n, int ka, int kb, const void *A,
+                int64_t lda, const void *B, int64_t ldb, double *G, hipStream_t s);
+fmx_status gemm_tall(fmx_ctx *ctx, fmx_dtype dt, uint64_t n, int ka, int kc, double alpha,
+                     const void *A, int64_t lda, const double *C, double beta, void *Y, int64_t ldy,
+                     hipStream_t s);
+fmx_status colnorm(fmx_ctx *ctx, fmx_dtype dt, uint64_t n, int k, const void *A, int64_t lda,
+                   double *norms, int squared, hipStream_t s);
+fmx_status row_scale(fmx_ctx *ctx, fmx_dtype dt, uint64_t n, int k, const double *d, const void *X,
+                     int64_t ldx, void *Y, int64_t ldy, hipStream_t s);
+
+}  // namespace fmx

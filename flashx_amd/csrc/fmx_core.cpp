@@ -1,0 +1,492 @@
+// fmx_core.cpp -- context, matrix upload/validation, work plans and the
+// fmx_spmm entry point of libfmx.
+//
+// Reference paths are relative to the FlashX source tree.
+#include <algorithm>
+#include <cstdlib>
+#include <cstring>
+#include <numeric>
+
+#include "fmx_internal.h"
+
+namespace fmx {
+
+Plan::~Plan()
+{
+    if (d_items) (void)hipFree(d_items);
+    if (d_splits) (void)hipFree(d_splits);
+}
+
+Scratch::~Scratch()
+{
+    if (ptr) (void)hipFree(ptr);
+}
+
+fmx_status fail(fmx_ctx *ctx, fmx_status st, const std::string &msg)
+{
+    if (ctx) ctx->last_error = msg;
+    return st;
+}
+
+fmx_status hip_fail(fmx_ctx *ctx, hipError_t e, const char *what)
+{
+    std::string msg = std::string(what) + ": " + hipGetErrorString(e);
+    return fail(ctx, e == hipErrorOutOfMemory ? FMX_ENOMEM : FMX_EHIP, msg);
+}
+
+int elem_size(fmx_dtype dt)
+{
+    switch (dt) {
+    case FMX_I32: case FMX_F32: return 4;
+    case FMX_I64: case FMX_F64: return 8;
+    case FMX_BF16: return 2;
+    }
+    return 0;
+}
+
+static uint16_t rd16(const uint8_t *p) { uint16_t v; memcpy(&v, p, 2); return v; }
+static uint32_t rd32(const uint8_t *p) { uint32_t v; memcpy(&v, p, 4); return v; }
+static uint64_t rd64(const uint8_t *p) { uint64_t v; memcpy(&v, p, 8); return v; }
+
+static int log2_exact(uint64_t v)
+{
+    for (int b = 0; b <= 15; b++)
+        if ((1ULL << b) == v) return b;
+    return -1;
+}
+
+// matrix_header::verify (matrix/matrix_header.cpp:52-60), block_2d_size
+// (matrix/matrix_header.h:42-91), SpM_2d_index::verify
+// (matrix/sparse_matrix_format.cpp:128-136) and sparse_block_2d::verify
+// (matrix/sparse_matrix_format.cpp:36-60), plus every structural fact the
+// reference writer guarantees, so that the device kernels can index without
+// further checks.
+fmx_status analyse_image(fmx_ctx *ctx, fmx_spm *s)
+{
+    const uint8_t *img = s->h_image.data();
+    const uint64_t bytes = s->h_image.size();
+    if (bytes < kHeaderBytes) return fail(ctx, FMX_EFORMAT, "image shorter than its 4096-byte header");
+    if (rd64(img) != 0x0123456789FFFFFFULL) return fail(ctx, FMX_EFORMAT, "wrong magic number");
+    if (rd32(img + 8) != 2) return fail(ctx, FMX_EFORMAT, "wrong version number");
+    if (rd32(img + 12) != 2) return fail(ctx, FMX_EFORMAT, "not a sparse matrix");
+    if (rd32(img + 40) != 2) return fail(ctx, FMX_EFORMAT, "layout is not L_ROW_2D");
+    s->esz = rd64(img + 16);
+    s->nrows = rd64(img + 24);
+    s->ncols = rd64(img + 32);
+    s->prim = (int)rd32(img + 44);
+    s->H = rd64(img + 48);
+    s->W = rd64(img + 56);
+    s->log2h = log2_exact(s->H);
+    s->log2w = log2_exact(s->W);
+    if (s->log2h < 0 || s->log2w < 0)
+        return fail(ctx, FMX_EFORMAT, "tile dimensions must be powers of two <= 32768");
+    if (s->nrows > (1ULL << 32) || s->ncols > (1ULL << 32))
+        return fail(ctx, FMX_EFORMAT, "more than 2^32 rows or columns");
+    if (s->esz != 0)
+        return fail(ctx, FMX_EUNSUPPORTED, "weighted tile images (entry_size > 0) are not supported by this build");
+    const uint64_t nbr = (s->nrows + s->H - 1) / s->H;
+    if (s->offs.size() != nbr + 1)
+        return fail(ctx, FMX_EFORMAT, "there are an incorrect number of offsets");
+    s->nbr = nbr;
+    s->nbc = (s->ncols + s->W - 1) / s->W;
+    if (s->offs[0] != (int64_t)kHeaderBytes) return fail(ctx, FMX_EFORMAT, "offs[0] must be 4096");
+    if ((uint64_t)s->offs[nbr] > bytes) return fail(ctx, FMX_EFORMAT, "offsets run past the image");
+    for (uint64_t i = 1; i <= nbr; i++)
+        if (s->offs[i] <= s->offs[i - 1] || (s->offs[i] & 1))
+            return fail(ctx, FMX_EFORMAT, "block-row offsets must be even and strictly increasing");
+
+    s->dir.clear();
+    s->brow_dir.assign(nbr + 1, 0);
+    uint64_t nnz = 0;
+    for (uint64_t br = 0; br < nbr; br++) {
+        s->brow_dir[br] = (uint32_t)s->dir.size();
+        const uint8_t *base = img + s->offs[br];
+        const uint8_t *p = base, *end = img + s->offs[br + 1];
+        if ((uint64_t)(end - base) / 2 >= (1ULL << 31))
+            return fail(ctx, FMX_EFORMAT, "block row larger than 2^31 words");
+        const uint64_t rows_here = std::min<uint64_t>(s->H, s->nrows - br * s->H);
+        long prev_bc = -1;
+        while (p < end) {
+            if (end - p < 16) return fail(ctx, FMX_EFORMAT, "truncated tile header");
+            uint32_t tbr = rd32(p), tbc = rd32(p + 4), tnnz = rd32(p + 8);
+            uint32_t tnrow = rd16(p + 12), tncoo = rd16(p + 14);
+            if (tbr != br) return fail(ctx, FMX_EFORMAT, "tile in the wrong block row");
+            uint64_t size = 16;
+            if (tnnz) {
+                if ((long)tbc <= prev_bc) return fail(ctx, FMX_EFORMAT, "tiles not in ascending block column");
+                if ((uint64_t)tbc >= s->nbc) return fail(ctx, FMX_EFORMAT, "block column out of range");
+                prev_bc = tbc;
+                if (tncoo > tnrow || tncoo > tnnz || tnrow > rows_here)
+                    return fail(ctx, FMX_EFORMAT, "inconsistent tile counts");
+                size = 16 + 2ULL * tnrow + 2ULL * tnnz + 2;
+                if ((uint64_t)(end - p) < size) return fail(ctx, FMX_EFORMAT, "tile runs past its block row");
+                const uint64_t cols_here = std::min<uint64_t>(s->W, s->ncols - (uint64_t)tbc * s->W);
+                const uint8_t *w = p + 16;
+                const uint8_t *rend = p + 16 + 2ULL * (tnrow - tncoo) + 2ULL * (tnnz - tncoo);
+                long prev_row = -1;
+                uint64_t nz = 0, nr = 0;
+                while (w < rend) {
+                    uint16_t h = rd16(w);
+                    // 0xFFFF is row 32767 of a 32768-row tile; the region
+                    // end is positional (sparse_matrix_format.h:339-347).
+                    if (!(h & 0x8000)) return fail(ctx, FMX_EFORMAT, "bad row-part header");
+                    long r = h & 0x7FFF;
+                    if (r <= prev_row || (uint64_t)r >= rows_here) return fail(ctx, FMX_EFORMAT, "row-part rows out of order or range");
+                    prev_row = r;
+                    w += 2;
+                    uint64_t len = 0;
+                    while (w < rend && !(rd16(w) & 0x8000)) {
+                        if (rd16(w) >= cols_here) return fail(ctx, FMX_EFORMAT, "column out of range");
+                        w += 2;
+                        len++;
+                    }
+                    if (len == 0) return fail(ctx, FMX_EFORMAT, "empty row part");
+                    nz += len;
+                    nr++;
+                }
+                if (rd16(rend) != 0xFFFF) return fail(ctx, FMX_EFORMAT, "missing end-of-row-parts marker");
+                const uint8_t *coo = rend + 2;
+                prev_row = -1;
+                for (uint32_t i = 0; i < tncoo; i++) {
+                    uint16_t h = rd16(coo + 4ULL * i), c = rd16(coo + 4ULL * i + 2);
+                    if (!(h & 0x8000) || (c & 0x8000)) return fail(ctx, FMX_EFORMAT, "bad COO entry");
+                    long r = h & 0x7FFF;
+                    if (r <= prev_row || (uint64_t)r >= rows_here || c >= cols_here)
+                        return fail(ctx, FMX_EFORMAT, "COO rows out of order or range");
+                    prev_row = r;
+                }
+                if (nz + tncoo != tnnz || nr + tncoo != tnrow)
+                    return fail(ctx, FMX_EFORMAT, "tile counts disagree with its contents");
+                nnz += tnnz;
+            }
+            s->dir.push_back(make_uint2((uint32_t)((p - base) / 2), tbc));
+            p += size;
+        }
+        if (p != end) return fail(ctx, FMX_EFORMAT, "block row size disagrees with its tiles");
+        if (s->dir.size() == s->brow_dir[br]) return fail(ctx, FMX_EFORMAT, "block row without tiles");
+    }
+    s->brow_dir[nbr] = (uint32_t)s->dir.size();
+    s->ntiles = s->dir.size();
+    s->nnz = nnz;
+    s->bytes = (uint64_t)s->offs[nbr];
+    return FMX_OK;
+}
+
+uint32_t band_rows(const fmx_spm *spm, int k, int s)
+{
+    static const uint64_t budget = [] {
+        const char *e = getenv("FMX_BAND_BYTES");
+        uint64_t v = e ? strtoull(e, nullptr, 10) : 16384;
+        return v < 256 ? 256 : v;
+    }();
+    uint64_t rows = budget / ((uint64_t)k * s);
+    uint32_t hs = 1;
+    while ((uint64_t)hs * 2 <= rows && hs < 32768) hs *= 2;
+    return (uint32_t)std::min<uint64_t>(hs, spm->H);
+}
+
+// Tile-relative row in force at word `p` of block row `br` (kNone if `p`
+// is a row-part or COO header); `p` is first moved out of a tile header.
+static uint32_t carry_at(const fmx_spm *s, uint64_t br, uint32_t &p, uint32_t &dir_idx)
+{
+    const uint16_t *w = reinterpret_cast<const uint16_t *>(s->h_image.data() + s->offs[br]);
+    uint32_t lo = s->brow_dir[br], hi = s->brow_dir[br + 1];
+    // last tile whose start <= p
+    uint32_t t = (uint32_t)(std::upper_bound(s->dir.begin() + lo, s->dir.begin() + hi, p,
+                                            [](uint32_t v, const uint2 &d) { return v < d.x; })
+                            - s->dir.begin()) - 1;
+    if (p < s->dir[t].x + 8) p = s->dir[t].x + 8;
+    dir_idx = t;
+    if (w[p] & 0x8000) return kNone;
+    uint32_t q = p;
+    while (!(w[q - 1] & 0x8000)) q--;
+    return w[q - 1] & 0x7FFF;
+}
+
+fmx_status get_plan(fmx_ctx *ctx, fmx_spm *s, uint32_t hs, Plan **out)
+{
+    auto it = s->plans.find(hs);
+    if (it != s->plans.end()) { *out = it->second.get(); return FMX_OK; }
+    std::unique_ptr<Plan> plan(new Plan());
+    plan->hs = hs;
+
+    struct Band { uint64_t br; uint64_t row0; uint32_t sb_base, rows; uint32_t words; };
+    std::vector<Band> bands;
+    uint64_t total = 0;
+    for (uint64_t br = 0; br < s->nbr; br++) {
+        uint64_t rows_here = std::min<uint64_t>(s->H, s->nrows - br * s->H);
+        uint32_t words = (uint32_t)((s->offs[br + 1] - s->offs[br]) / 2);
+        for (uint64_t sb = 0; sb * hs < rows_here; sb++) {
+            uint32_t rows = (uint32_t)std::min<uint64_t>(hs, rows_here - sb * hs);
+            bands.push_back({br, br * s->H + sb * hs, (uint32_t)(sb * hs), rows, words});
+            total += words;
+        }
+    }
+    // Work cap: a band whose stream exceeds it is split into near-equal word
+    // ranges reduced in order afterwards (power-law hub rows).
+    uint64_t avg = bands.empty() ? 0 : total / bands.size();
+    uint64_t cap = std::max<uint64_t>(8192, 4 * avg);
+    const char *e = getenv("FMX_ITEM_CAP");
+    if (e) cap = std::max<uint64_t>(64, strtoull(e, nullptr, 10));
+
+    std::vector<Split> splits;
+    uint32_t nslots = 0;
+    for (const Band &b : bands) {
+        uint32_t pieces = (uint32_t)((b.words + cap - 1) / cap);
+        if (pieces < 1) pieces = 1;
+        Item base{};
+        base.brow_word = (uint64_t)s->offs[b.br] / 2;
+        base.dir_end = s->brow_dir[b.br + 1];
+        base.row0_lo = (uint32_t)b.row0;
+        base.row0_hi = (uint32_t)(b.row0 >> 32);
+        base.sb_base = b.sb_base;
+        base.rows = b.rows;
+        if (pieces == 1) {
+            Item t = base;
+            t.begin = 0;
+            t.nwords = b.words;
+            t.dir_begin = s->brow_dir[b.br];
+            t.carry = kNone;
+            t.slot = kNone;
+            plan->items.push_back(t);
+            continue;
+        }
+        std::vector<uint32_t> cut(pieces + 1), dirs(pieces + 1), carries(pieces + 1);
+        cut[0] = 0;
+        dirs[0] = s->brow_dir[b.br];
+        carries[0] = kNone;
+        for (uint32_t i = 1; i < pieces; i++) {
+            uint32_t p = (uint32_t)((uint64_t)b.words * i / pieces);
+            if (p <= cut[i - 1]) p = cut[i - 1] + 1;
+            carries[i] = carry_at(s, b.br, p, dirs[i]);
+            cut[i] = p;
+        }
+        cut[pieces] = b.words;
+        // drop pieces emptied by header adjustment
+        std::vector<uint32_t> keep;
+        for (uint32_t i = 0; i < pieces; i++)
+            if (cut[i] < cut[i + 1]) keep.push_back(i);
+        Split sp{b.row0, b.rows, nslots, (uint32_t)keep.size(), 0};
+        for (uint32_t i : keep) {
+            Item t = base;
+            t.begin = cut[i];
+            t.nwords = cut[i + 1] - cut[i];
+            t.dir_begin = dirs[i];
+            t.carry = carries[i];
+            t.slot = nslots++;
+            plan->items.push_back(t);
+        }
+        splits.push_back(sp);
+    }
+    // Longest items first, so the hardware dispatcher approximates LPT.
+    std::stable_sort(plan->items.begin(), plan->items.end(),
+                     [](const Item &a, const Item &b) { return a.nwords > b.nwords; });
+    plan->nitems = (uint32_t)plan->items.size();
+    plan->nslots = nslots;
+    plan->nsplits = (uint32_t)splits.size();
+    if (plan->nitems) {
+        FMX_HIP(ctx, hipMalloc(&plan->d_items, plan->nitems * sizeof(Item)));
+        FMX_HIP(ctx, hipMemcpy(plan->d_items, plan->items.data(), plan->nitems * sizeof(Item), hipMemcpyHostToDevice));
+    }
+    if (plan->nsplits) {
+        FMX_HIP(ctx, hipMalloc(&plan->d_splits, plan->nsplits * sizeof(Split)));
+        FMX_HIP(ctx, hipMemcpy(plan->d_splits, splits.data(), plan->nsplits * sizeof(Split), hipMemcpyHostToDevice));
+    }
+    *out = plan.get();
+    s->plans[hs] = std::move(plan);
+    return FMX_OK;
+}
+
+// A device-built image: bring the host copy, validate it like an uploaded
+// one and publish the tile directory.
+fmx_status finish_device_image(fmx_ctx *ctx, fmx_spm *s)
+{
+    s->h_image.resize(s->bytes);
+    FMX_HIP(ctx, hipMemcpy(s->h_image.data(), s->d_image, s->bytes, hipMemcpyDeviceToHost));
+    fmx_status st = analyse_image(ctx, s);
+    if (st) return st;
+    FMX_HIP(ctx, hipMalloc(&s->d_dir, std::max<size_t>(1, s->dir.size()) * sizeof(uint2)));
+    FMX_HIP(ctx, hipMemcpy(s->d_dir, s->dir.data(), s->dir.size() * sizeof(uint2), hipMemcpyHostToDevice));
+    return FMX_OK;
+}
+
+}  // namespace fmx
+
+fmx_spm::~fmx_spm()
+{
+    if (d_image) (void)hipFree(d_image);
+    if (d_dir) (void)hipFree(d_dir);
+}
+
+using namespace fmx;
+
+extern "C" {
+
+const char *fmx_strerror(fmx_status st)
+{
+    switch (st) {
+    case FMX_OK: return "success";
+    case FMX_EINVAL: return "invalid argument";
+    case FMX_ETYPE: return "unsupported or mismatched element type";
+    case FMX_EDIM: return "dimension mismatch";
+    case FMX_EFORMAT: return "malformed tile image";
+    case FMX_ENOMEM: return "out of memory";
+    case FMX_EHIP: return "HIP runtime error";
+    case FMX_EUNSUPPORTED: return "not supported by this build";
+    }
+    return "unknown status";
+}
+
+fmx_status fmx_ctx_create(int device, fmx_ctx **out)
+{
+    if (!out) return FMX_EINVAL;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return FMX_EHIP;
+    if (device < 0 || device >= ndev) return FMX_EINVAL;
+    std::unique_ptr<fmx_ctx> c(new fmx_ctx());
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess) return FMX_EHIP;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return FMX_EHIP;
+    c->num_cus = prop.multiProcessorCount;
+    c->lds_per_block = prop.sharedMemPerBlock;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return FMX_EHIP;
+    *out = c.release();
+    return FMX_OK;
+}
+
+fmx_status fmx_ctx_destroy(fmx_ctx *ctx)
+{
+    if (!ctx) return FMX_EINVAL;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) {
+        (void)hipStreamSynchronize(ctx->stream);
+        (void)hipStreamDestroy(ctx->stream);
+    }
+    if (ctx->ws) (void)hipFree(ctx->ws);
+    delete ctx;
+    return FMX_OK;
+}
+
+fmx_status fmx_ctx_stream(fmx_ctx *ctx, void **stream)
+{
+    if (!ctx || !stream) return FMX_EINVAL;
+    *stream = ctx->stream;
+    return FMX_OK;
+}
+
+fmx_status fmx_ctx_sync(fmx_ctx *ctx)
+{
+    if (!ctx) return FMX_EINVAL;
+    FMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return FMX_OK;
+}
+
+const char *fmx_ctx_last_error(const fmx_ctx *ctx)
+{
+    return ctx ? ctx->last_error.c_str() : "";
+}
+
+fmx_status fmx_spm_upload(fmx_ctx *ctx, const void *image, size_t bytes,
+                          const int64_t *offs, size_t nbr, fmx_spm **out)
+{
+    if (!ctx || !image || !offs || !out) return fail(ctx, FMX_EINVAL, "null argument");
+    *out = nullptr;
+    FMX_HIP(ctx, hipSetDevice(ctx->device));
+    std::unique_ptr<fmx_spm> s(new fmx_spm());
+    s->ctx = ctx;
+    s->h_image.assign((const uint8_t *)image, (const uint8_t *)image + bytes);
+    s->offs.assign(offs, offs + nbr + 1);
+    fmx_status st = analyse_image(ctx, s.get());
+    if (st) return st;
+    size_t padded = (s->bytes + 4095) & ~size_t(4095);
+    FMX_HIP(ctx, hipMalloc(&s->d_image, padded));
+    FMX_HIP(ctx, hipMemcpy(s->d_image, s->h_image.data(), s->bytes, hipMemcpyHostToDevice));
+    if (padded > s->bytes) FMX_HIP(ctx, hipMemset(s->d_image + s->bytes, 0, padded - s->bytes));
+    FMX_HIP(ctx, hipMalloc(&s->d_dir, std::max<size_t>(1, s->dir.size()) * sizeof(uint2)));
+    FMX_HIP(ctx, hipMemcpy(s->d_dir, s->dir.data(), s->dir.size() * sizeof(uint2), hipMemcpyHostToDevice));
+    *out = s.release();
+    return FMX_OK;
+}
+
+fmx_status fmx_spm_info_get(const fmx_spm *spm, fmx_spm_info *info)
+{
+    if (!spm || !info) return FMX_EINVAL;
+    info->nrows = spm->nrows;
+    info->ncols = spm->ncols;
+    info->nnz = spm->nnz;
+    info->block_h = spm->H;
+    info->block_w = spm->W;
+    info->nbr = spm->nbr;
+    info->image_bytes = spm->bytes;
+    info->tile_bytes = spm->bytes - kHeaderBytes;
+    info->entry_size = spm->esz;
+    info->ntiles = spm->ntiles;
+    return FMX_OK;
+}
+
+fmx_status fmx_spm_download(const fmx_spm *spm, void *image, size_t cap,
+                            size_t *bytes, int64_t *offs)
+{
+    if (!spm) return FMX_EINVAL;
+    fmx_ctx *ctx = spm->ctx;
+    if (bytes) *bytes = spm->bytes;
+    if (image) {
+        if (cap < spm->bytes) return fail(ctx, FMX_EINVAL, "image buffer too small");
+        FMX_HIP(ctx, hipSetDevice(ctx->device));
+        FMX_HIP(ctx, hipMemcpy(image, spm->d_image, spm->bytes, hipMemcpyDeviceToHost));
+    }
+    if (offs) memcpy(offs, spm->offs.data(), spm->offs.size() * sizeof(int64_t));
+    return FMX_OK;
+}
+
+fmx_status fmx_spm_destroy(fmx_spm *spm)
+{
+    if (!spm) return FMX_EINVAL;
+    (void)hipSetDevice(spm->ctx->device);
+    delete spm;
+    return FMX_OK;
+}
+
+fmx_status fmx_spmm_alg_bytes(const fmx_spm *spm, fmx_dtype dtype, int k, uint64_t *bytes)
+{
+    if (!spm || !bytes || k <= 0) return FMX_EINVAL;
+    int es = elem_size(dtype);
+    if (!es || dtype == FMX_BF16) return FMX_ETYPE;
+    *bytes = (spm->bytes - kHeaderBytes) + 8 * (spm->nbr + 1) +
+             spm->ncols * (uint64_t)k * es + spm->nrows * (uint64_t)k * es;
+    return FMX_OK;
+}
+
+fmx_status fmx_spmm(fmx_ctx *ctx, const fmx_spm *cspm, fmx_dtype dtype, int k,
+                    const void *X, int64_t ldx, void *Y, int64_t ldy, void *stream)
+{
+    if (!ctx || !cspm) return fail(ctx, FMX_EINVAL, "null context or matrix");
+    fmx_spm *spm = const_cast<fmx_spm *>(cspm);
+    if (spm->ctx != ctx) return fail(ctx, FMX_EINVAL, "matrix belongs to another context");
+    if (dtype != FMX_I32 && dtype != FMX_I64 && dtype != FMX_F32 && dtype != FMX_F64)
+        return fail(ctx, FMX_ETYPE, "wrong matrix type in spmm");
+    if (k <= 0) return fail(ctx, FMX_EDIM, "k must be positive");
+    if (ldx < k || ldy < k) return fail(ctx, FMX_EDIM, "leading dimension smaller than k");
+    const int es = elem_size(dtype);
+    if ((uint64_t)k * es > 32768) return fail(ctx, FMX_EUNSUPPORTED, "dense rows wider than 32 KiB");
+    if ((!X && spm->ncols) || (!Y && spm->nrows)) return fail(ctx, FMX_EINVAL, "null dense operand");
+    if (spm->nrows == 0) return FMX_OK;
+    FMX_HIP(ctx, hipSetDevice(ctx->device));
+    Plan *plan = nullptr;
+    fmx_status st = get_plan(ctx, spm, band_rows(spm, k, es), &plan);
+    if (st) return st;
+    size_t need = (size_t)plan->nslots * plan->hs * k * es;
+    if (need > spm->partial.bytes) {
+        if (spm->partial.ptr) FMX_HIP(ctx, hipFree(spm->partial.ptr));
+        spm->partial.ptr = nullptr;
+        spm->partial.bytes = 0;
+        FMX_HIP(ctx, hipMalloc(&spm->partial.ptr, need));
+        spm->partial.bytes = need;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    return launch_spmm(ctx, spm, plan, dtype, k, X, ldx, Y, ldy, spm->partial.ptr, s);
+}
+
+}  // extern "C"

@@ -1,0 +1,127 @@
+// fmx_internal.h -- shared host/device declarations of libfmx.
+//
+// Layout of a matrix in HBM:
+//   image    : the reference-format tile image, byte for byte (4096-byte
+//              header + block rows; matrix/sparse_matrix_format.h:213-240).
+//   dir      : one {start, bc} pair per stored tile, in image order; `start`
+//              is the tile's first u16 word relative to its block row.
+//   plans    : per sub-band height Hs, a list of work items (contiguous word
+//              ranges of one block row) and the partial-band reduction list.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/fmx.h"
+
+namespace fmx {
+
+constexpr uint32_t kNone = 0xFFFFFFFFu;
+constexpr uint64_t kHeaderBytes = 4096;
+
+// One unit of SpMM work: words [begin, begin+nwords) of a block row, whose
+// non-zeros with tile-relative rows in [sb_base, sb_base + rows) accumulate
+// into one LDS band.  `carry` is the tile-relative row in force at `begin`
+// when the range starts inside a row part (kNone otherwise).
+struct Item {
+    uint64_t brow_word;   // absolute u16 index of the block row's first word
+    uint32_t begin;       // first word, relative to the block row
+    uint32_t nwords;
+    uint32_t dir_begin;   // directory index of the tile holding `begin`
+    uint32_t dir_end;     // one past the block row's last directory entry
+    uint32_t row0_lo;     // first output row of the band (low / high 32 bits)
+    uint32_t row0_hi;
+    uint32_t sb_base;     // tile-relative row of band row 0
+    uint32_t rows;        // rows in the band
+    uint32_t carry;
+    uint32_t slot;        // partial-band slot, kNone = write Y directly
+};
+static_assert(sizeof(Item) == 48, "Item layout");
+
+// A band split over several items: Y rows [row0, row0+rows) are the ordered
+// sum of partial slots [slot_begin, slot_begin+nslots).
+struct Split {
+    uint64_t row0;
+    uint32_t rows;
+    uint32_t slot_begin;
+    uint32_t nslots;
+    uint32_t pad;
+};
+static_assert(sizeof(Split) == 24, "Split layout");
+
+struct Plan {
+    uint32_t hs = 0;            // band height (rows)
+    uint32_t nitems = 0;
+    uint32_t nslots = 0;
+    uint32_t nsplits = 0;
+    Item *d_items = nullptr;
+    Split *d_splits = nullptr;
+    std::vector<Item> items;    // host copy (tests / diagnostics)
+    ~Plan();
+};
+
+struct Scratch {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    ~Scratch();
+};
+
+}  // namespace fmx
+
+struct fmx_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string last_error;
+    int num_cus = 0;
+    size_t lds_per_block = 0;
+    void *ws = nullptr;        // grow-only device workspace (dense-op partials)
+    size_t ws_bytes = 0;
+};
+
+struct fmx_spm {
+    fmx_ctx *ctx = nullptr;
+    uint64_t nrows = 0, ncols = 0, nnz = 0, H = 0, W = 0, nbr = 0, nbc = 0;
+    uint64_t bytes = 0, esz = 0, ntiles = 0;
+    int log2h = 0, log2w = 0, prim = 0;
+    uint8_t *d_image = nullptr;         // image in HBM (padded to 4 KiB)
+    uint2 *d_dir = nullptr;             // tile directory
+    std::vector<uint8_t> h_image;       // host copy used to build plans
+    std::vector<int64_t> offs;          // block-row byte offsets
+    std::vector<uint2> dir;             // host directory
+    std::vector<uint32_t> brow_dir;     // first directory index per block row (+1)
+    std::map<uint32_t, std::unique_ptr<fmx::Plan>> plans;
+    fmx::Scratch partial;               // partial bands of split items
+    ~fmx_spm();
+};
+
+namespace fmx {
+
+// Records `msg` on the context and returns `st`.
+fmx_status fail(fmx_ctx *ctx, fmx_status st, const std::string &msg);
+fmx_status hip_fail(fmx_ctx *ctx, hipError_t e, const char *what);
+int elem_size(fmx_dtype dt);
+
+// Validates a host image and fills the matrix fields + directory.
+fmx_status analyse_image(fmx_ctx *ctx, fmx_spm *spm);
+// Builds (or fetches) the work plan for band height hs.
+fmx_status get_plan(fmx_ctx *ctx, fmx_spm *spm, uint32_t hs, Plan **out);
+// Band height for element size s and width k.
+uint32_t band_rows(const fmx_spm *spm, int k, int s);
+
+fmx_status launch_spmm(fmx_ctx *ctx, const fmx_spm *spm, const Plan *plan,
+                       fmx_dtype dt, int k, const void *X, int64_t ldx,
+                       void *Y, int64_t ldy, void *partial,
+                       hipStream_t stream);
+
+}  // namespace fmx
+
+#define FMX_HIP(ctx, call)                                            \
+    do {                                                              \
+        hipError_t e_ = (call);                                       \
+        if (e_ != hipSuccess) return fmx::hip_fail((ctx), e_, #call); \
+    } while (0)

@@ -1,0 +1,351 @@
+"""ctypes binding of libfmx (include/fmx.h) and the Python mirror of the
+reference's sparse-matrix operator interface.
+
+The classes mirror fm::sparse_matrix (matrix/sparse_matrix.h:475-651):
+``SparseMatrix.multiply(X)`` returns a new Y = A*X like
+``sparse_matrix::multiply(dense_matrix::ptr)`` (sparse_matrix.cpp:773-831) and
+``SparseMatrix.multiply_into(X, Y)`` fills a caller-owned Y and returns a bool
+like ``sparse_matrix::multiply(in, out)`` (sparse_matrix.cpp:751-771).  Dense
+operands are torch tensors on the GPU (row-major, any leading dimension);
+torch only provides device memory and streams here.  There is no CPU path:
+every call goes to the HIP library and raises if it is not built.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libfmx.so")
+
+OK, EINVAL, ETYPE, EDIM, EFORMAT, ENOMEM, EHIP, EUNSUPPORTED = range(8)
+I32, I64, F32, F64, BF16 = range(5)
+BUILD_DEDUP, BUILD_KEEP_SELF, BUILD_SYMMETRIZE = 1, 2, 4
+
+_vp = ctypes.c_void_p
+_u64 = ctypes.c_uint64
+_i64 = ctypes.c_int64
+
+
+class FmxError(RuntimeError):
+    """A libfmx call failed; ``status`` is the fmx_status code."""
+
+    def __init__(self, status, msg):
+        super().__init__(msg)
+        self.status = status
+
+
+class fmx_spm_info(ctypes.Structure):
+    _fields_ = [("nrows", _u64), ("ncols", _u64), ("nnz", _u64), ("block_h", _u64),
+                ("block_w", _u64), ("nbr", _u64), ("image_bytes", _u64),
+                ("tile_bytes", _u64), ("entry_size", _u64), ("ntiles", _u64)]
+
+
+# name -> (restype, argtypes); every symbol include/fmx.h declares.
+SIGNATURES = {
+    "fmx_ctx_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp)]),
+    "fmx_ctx_destroy": (ctypes.c_int, [_vp]),
+    "fmx_ctx_stream": (ctypes.c_int, [_vp, ctypes.POINTER(_vp)]),
+    "fmx_ctx_sync": (ctypes.c_int, [_vp]),
+    "fmx_strerror": (ctypes.c_char_p, [ctypes.c_int]),
+    "fmx_ctx_last_error": (ctypes.c_char_p, [_vp]),
+    "fmx_spm_upload": (ctypes.c_int, [_vp, _vp, ctypes.c_size_t, _vp, ctypes.c_size_t,
+                                      ctypes.POINTER(_vp)]),
+    "fmx_spm_build": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_size_t, _u64, ctypes.c_uint32,
+                                     ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(_vp)]),
+    "fmx_spm_info_get": (ctypes.c_int, [_vp, ctypes.POINTER(fmx_spm_info)]),
+    "fmx_spm_download": (ctypes.c_int, [_vp, _vp, ctypes.c_size_t,
+                                        ctypes.POINTER(ctypes.c_size_t), _vp]),
+    "fmx_spm_destroy": (ctypes.c_int, [_vp]),
+    "fmx_spmm": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int, _vp, _i64, _vp, _i64, _vp]),
+    "fmx_spmm_alg_bytes": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int,
+                                          ctypes.POINTER(_u64)]),
+    "fmx_gram": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _u64, ctypes.c_int, ctypes.c_int,
+                                _vp, _i64, _vp, _i64, _vp, _vp]),
+    "fmx_gemm_tall": (ctypes.c_int, [_vp, ctypes.c_int, _u64, ctypes.c_int, ctypes.c_int,
+                                     ctypes.c_double, _vp, _i64, _vp, ctypes.c_double, _vp, _i64,
+                                     _vp]),
+    "fmx_colnorm": (ctypes.c_int, [_vp, ctypes.c_int, _u64, ctypes.c_int, _vp, _i64, _vp, _vp]),
+    "fmx_row_scale": (ctypes.c_int, [_vp, ctypes.c_int, _u64, ctypes.c_int, _vp, _vp, _i64, _vp,
+                                     _i64, _vp]),
+    "fmx_rmat_edges": (ctypes.c_int, [_vp, ctypes.c_int, _u64, _u64, ctypes.c_uint32,
+                                      ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, _vp,
+                                      _vp, _vp]),
+}
+
+_lib = None
+
+
+def lib():
+    """Loads libfmx.so; raises if it has not been built (no CPU fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("libfmx.so is not built (%s); run python -m flashx_amd.build"
+                              % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def dtype_code(t):
+    torch = _torch()
+    m = {torch.int32: I32, torch.int64: I64, torch.float32: F32, torch.float64: F64,
+         torch.bfloat16: BF16}
+    if t not in m:
+        raise FmxError(ETYPE, "unsupported dtype %s" % t)
+    return m[t]
+
+
+class Context:
+    """A device and its stream (replaces fm::init_flash_matrix)."""
+
+    def __init__(self, device=0):
+        self.handle = _vp()
+        st = lib().fmx_ctx_create(device, ctypes.byref(self.handle))
+        if st:
+            raise FmxError(st, "fmx_ctx_create(%d): %s" % (device, lib().fmx_strerror(st).decode()))
+        self.device = device
+
+    def check(self, st, what):
+        if st:
+            detail = lib().fmx_ctx_last_error(self.handle).decode()
+            raise FmxError(st, "%s: %s (%s)" % (what, lib().fmx_strerror(st).decode(), detail))
+
+    def stream_ptr(self):
+        torch = _torch()
+        return ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+
+    def sync(self):
+        self.check(lib().fmx_ctx_sync(self.handle), "fmx_ctx_sync")
+
+    def close(self):
+        if self.handle:
+            lib().fmx_ctx_destroy(self.handle)
+            self.handle = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default_ctx = {}
+
+
+def default_context(device=0):
+    if device not in _default_ctx:
+        _default_ctx[device] = Context(device)
+    return _default_ctx[device]
+
+
+class SparseMatrix:
+    """fm::sparse_matrix over a 2D tile image resident in HBM."""
+
+    def __init__(self, ctx, handle):
+        self.ctx = ctx
+        self.handle = handle
+        info = fmx_spm_info()
+        ctx.check(lib().fmx_spm_info_get(handle, ctypes.byref(info)), "fmx_spm_info_get")
+        self.info = info
+
+    # -- factories (sparse_matrix::create, fm::create_2d_matrix) -----------
+    @classmethod
+    def upload(cls, image, offs, ctx=None):
+        """From a reference-format image (bytes/uint8 array) and offsets."""
+        ctx = ctx or default_context()
+        img = np.ascontiguousarray(np.frombuffer(image, np.uint8) if isinstance(image, (bytes, bytearray))
+                                   else image, dtype=np.uint8)
+        offs = np.ascontiguousarray(offs, dtype=np.int64)
+        h = _vp()
+        ctx.check(lib().fmx_spm_upload(ctx.handle, img.ctypes.data, img.nbytes, offs.ctypes.data,
+                                       len(offs) - 1, ctypes.byref(h)), "fmx_spm_upload")
+        return cls(ctx, h)
+
+    @classmethod
+    def build(cls, src, dst, n=0, log2_h=15, log2_w=15, flags=BUILD_DEDUP, ctx=None):
+        """On-device tile construction from device tensors src, dst (int32/uint32)."""
+        torch = _torch()
+        ctx = ctx or default_context()
+        if src.device.type != "cuda" or dst.device.type != "cuda" or src.numel() != dst.numel():
+            raise FmxError(EINVAL, "src and dst must be device tensors of equal length")
+        if src.dtype not in (torch.int32, torch.uint32) or dst.dtype != src.dtype:
+            raise FmxError(ETYPE, "src and dst must be 32-bit vertex ids")
+        src = src.contiguous()
+        dst = dst.contiguous()
+        h = _vp()
+        ctx.check(lib().fmx_spm_build(ctx.handle, src.data_ptr(), dst.data_ptr(), src.numel(), n,
+                                      log2_h, log2_w, flags, ctypes.byref(h)), "fmx_spm_build")
+        return cls(ctx, h)
+
+    # -- reference accessors -------------------------------------------------
+    def get_num_rows(self):
+        return self.info.nrows
+
+    def get_num_cols(self):
+        return self.info.ncols
+
+    def get_nnz(self):
+        return self.info.nnz
+
+    def get_block_size(self):
+        return (self.info.block_h, self.info.block_w)
+
+    def get_entry_size(self):
+        return self.info.entry_size
+
+    def download(self):
+        """(image uint8 array, offsets int64 array) copied back to the host."""
+        img = np.empty(self.info.image_bytes, np.uint8)
+        offs = np.empty(self.info.nbr + 1, np.int64)
+        nb = ctypes.c_size_t()
+        self.ctx.check(lib().fmx_spm_download(self.handle, img.ctypes.data, img.nbytes,
+                                              ctypes.byref(nb), offs.ctypes.data),
+                       "fmx_spm_download")
+        return img, offs
+
+    def alg_bytes(self, dtype, k):
+        out = _u64()
+        self.ctx.check(lib().fmx_spmm_alg_bytes(self.handle, dtype_code(dtype), k,
+                                                ctypes.byref(out)), "fmx_spmm_alg_bytes")
+        return out.value
+
+    # -- SpMM -------------------------------------------------------------------
+    def multiply_async(self, X, Y, stream=None):
+        """Enqueues Y = A*X on `stream` (default: torch's current stream)."""
+        torch = _torch()
+        if X.dim() == 1:
+            X = X.view(-1, 1)
+        if Y.dim() == 1:
+            Y = Y.view(-1, 1)
+        if X.device.type != "cuda" or Y.device.type != "cuda":
+            raise FmxError(EINVAL, "dense operands must be device tensors")
+        if X.dtype != Y.dtype:
+            raise FmxError(ETYPE, "wrong matrix type in spmm creator")
+        if X.shape[0] != self.info.ncols or Y.shape[0] != self.info.nrows or X.shape[1] != Y.shape[1]:
+            raise FmxError(EDIM, "dimension mismatch: A is %dx%d, X %s, Y %s"
+                           % (self.info.nrows, self.info.ncols, tuple(X.shape), tuple(Y.shape)))
+        if X.stride(1) != 1 or Y.stride(1) != 1:
+            raise FmxError(EINVAL, "dense operands must be row-major")
+        k = X.shape[1]
+        sp = ctypes.c_void_p(stream) if stream is not None else self.ctx.stream_ptr()
+        self.ctx.check(lib().fmx_spmm(self.ctx.handle, self.handle, dtype_code(X.dtype), k,
+                                      X.data_ptr(), max(X.stride(0), k), Y.data_ptr(),
+                                      max(Y.stride(0), k), sp), "fmx_spmm")
+        return Y
+
+    def multiply_into(self, X, Y):
+        """sparse_matrix::multiply(in, out): True on success, False on a
+        type or dimension error (logged as an exception message)."""
+        try:
+            self.multiply_async(X, Y)
+        except FmxError:
+            return False
+        _torch().cuda.current_stream(self.ctx.device).synchronize()
+        return True
+
+    def multiply(self, X):
+        """sparse_matrix::multiply(dense_matrix): a new Y = A*X (synchronous)."""
+        torch = _torch()
+        squeeze = X.dim() == 1
+        X2 = X.view(-1, 1) if squeeze else X
+        Y = torch.empty((self.info.nrows, X2.shape[1]), dtype=X.dtype, device=X.device)
+        self.multiply_async(X2, Y)
+        torch.cuda.current_stream(self.ctx.device).synchronize()
+        return Y.view(-1) if squeeze else Y
+
+    def close(self):
+        if self.handle:
+            lib().fmx_spm_destroy(self.handle)
+            self.handle = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+GRAM_F64, GRAM_BF16 = 0, 1
+
+
+def _ld(t):
+    if t.dim() != 2 or t.stride(1) != 1:
+        raise FmxError(EINVAL, "dense operands must be 2-D row-major device tensors")
+    return max(t.stride(0), t.shape[1])
+
+
+def gram(A, B, mode=GRAM_F64, ctx=None):
+    """G = A^T B (FP64 result), MvTransMv."""
+    torch = _torch()
+    ctx = ctx or default_context()
+    if A.shape[0] != B.shape[0] or A.dtype != B.dtype:
+        raise FmxError(EDIM, "Gram operands must share rows and dtype")
+    G = torch.empty((A.shape[1], B.shape[1]), dtype=torch.float64, device=A.device)
+    ctx.check(lib().fmx_gram(ctx.handle, dtype_code(A.dtype), mode, A.shape[0], A.shape[1],
+                             B.shape[1], A.data_ptr(), _ld(A), B.data_ptr(), _ld(B), G.data_ptr(),
+                             ctx.stream_ptr()), "fmx_gram")
+    return G
+
+
+def gemm_tall(A, C, alpha=1.0, beta=0.0, Y=None, ctx=None):
+    """Y = alpha*A*C + beta*Y with C a small FP64 matrix (MvTimesMatAddMv)."""
+    torch = _torch()
+    ctx = ctx or default_context()
+    C = C.to(device=A.device, dtype=torch.float64).contiguous()
+    if Y is None:
+        Y = torch.empty((A.shape[0], C.shape[1]), dtype=A.dtype, device=A.device)
+        beta = 0.0
+    if C.shape[0] != A.shape[1] or Y.shape != (A.shape[0], C.shape[1]) or Y.dtype != A.dtype:
+        raise FmxError(EDIM, "gemm_tall shape mismatch")
+    ctx.check(lib().fmx_gemm_tall(ctx.handle, dtype_code(A.dtype), A.shape[0], A.shape[1],
+                                  C.shape[1], alpha, A.data_ptr(), _ld(A), C.data_ptr(), beta,
+                                  Y.data_ptr(), _ld(Y), ctx.stream_ptr()), "fmx_gemm_tall")
+    return Y
+
+
+def colnorm(A, ctx=None):
+    """Column 2-norms (FP64), MvNorm."""
+    torch = _torch()
+    ctx = ctx or default_context()
+    out = torch.empty(A.shape[1], dtype=torch.float64, device=A.device)
+    ctx.check(lib().fmx_colnorm(ctx.handle, dtype_code(A.dtype), A.shape[0], A.shape[1],
+                                A.data_ptr(), _ld(A), out.data_ptr(), ctx.stream_ptr()),
+              "fmx_colnorm")
+    return out
+
+
+def row_scale(d, X, Y=None, ctx=None):
+    """Y = diag(d) X with d FP64."""
+    torch = _torch()
+    ctx = ctx or default_context()
+    d = d.to(dtype=torch.float64).contiguous()
+    if Y is None:
+        Y = torch.empty_like(X)
+    ctx.check(lib().fmx_row_scale(ctx.handle, dtype_code(X.dtype), X.shape[0], X.shape[1],
+                                  d.data_ptr(), X.data_ptr(), _ld(X), Y.data_ptr(), _ld(Y),
+                                  ctx.stream_ptr()), "fmx_row_scale")
+    return Y
+
+
+def rmat_edges(scale, m, seed, a=0.57, b=0.19, c=0.19, permute=True, ctx=None):
+    """Seeded R-MAT edge list generated on the device (two int32 tensors)."""
+    torch = _torch()
+    ctx = ctx or default_context()
+    f = lambda x: min(int(np.floor(x * 4294967296.0)), 0xFFFFFFFF)
+    src = torch.empty(m, dtype=torch.int32, device="cuda:%d" % ctx.device)
+    dst = torch.empty(m, dtype=torch.int32, device="cuda:%d" % ctx.device)
+    ctx.check(lib().fmx_rmat_edges(ctx.handle, scale, m, seed, f(a), f(a + b), f(a + b + c),
+                                   int(permute), src.data_ptr(), dst.data_ptr(),
+                                   ctx.stream_ptr()), "fmx_rmat_edges")
+    return src, dst

@@ -1,0 +1,169 @@
+/*
+ * fmx.h -- C ABI of the MI355X-native FlashMatrix SpMM path.
+ *
+ * Plain C types only: device buffers are `void *` / typed pointers into HBM
+ * (allocated by the caller with hipMalloc or any allocator on the same
+ * device), host buffers are ordinary pointers, sizes are counts.  Streams are
+ * hipStream_t passed as `void *` and used as given (NULL = HIP's null
+ * stream); fmx_ctx_stream returns a stream the context owns.
+ *
+ * Every call returns an fmx_status.  Calls taking a stream are stream-ordered
+ * (asynchronous); the C++ layer (include/fm/sparse_matrix.h) synchronises
+ * like the reference's blocking sparse_matrix::multiply
+ * (matrix/sparse_matrix.cpp:465).  A context, and every matrix made from it,
+ * is used by one host thread at a time (the reference's multiply is not
+ * re-entrant either, matrix/sparse_matrix.cpp:436,471).
+ *
+ * Reference paths are relative to the FlashX source tree.
+ */
+#ifndef FMX_H
+#define FMX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    FMX_OK = 0,
+    FMX_EINVAL = 1,      /* bad argument                                     */
+    FMX_ETYPE = 2,       /* unsupported or mismatched element type           */
+    FMX_EDIM = 3,        /* dimension mismatch                               */
+    FMX_EFORMAT = 4,     /* malformed tile image / header                    */
+    FMX_ENOMEM = 5,      /* device or host allocation failed                 */
+    FMX_EHIP = 6,        /* HIP runtime error                                */
+    FMX_EUNSUPPORTED = 7 /* valid request this build does not implement      */
+} fmx_status;
+
+typedef enum {
+    FMX_I32 = 0,  /* int      (P_INTEGER)  */
+    FMX_I64 = 1,  /* long     (P_LONG)     */
+    FMX_F32 = 2,  /* float    (P_FLOAT)    */
+    FMX_F64 = 3,  /* double   (P_DOUBLE)   */
+    FMX_BF16 = 4  /* bfloat16 (dense-op inputs only) */
+} fmx_dtype;
+
+/* Edge normalisation flags for fmx_spm_build. */
+#define FMX_BUILD_DEDUP      1u /* std::unique per row, fm_utils.cpp:37,460-464 */
+#define FMX_BUILD_KEEP_SELF  2u /* keep self edges, fm_utils.cpp:39,450-452     */
+#define FMX_BUILD_SYMMETRIZE 4u /* add (dst,src) for every (src,dst)            */
+
+typedef struct fmx_ctx fmx_ctx;
+typedef struct fmx_spm fmx_spm;
+
+typedef struct {
+    uint64_t nrows, ncols;     /* matrix_header nrows / ncols                  */
+    uint64_t nnz;              /* non-zeros in the image                       */
+    uint64_t block_h, block_w; /* tile height / width (powers of two)          */
+    uint64_t nbr;              /* block rows                                   */
+    uint64_t image_bytes;      /* header + tiles (offs[nbr])                   */
+    uint64_t tile_bytes;       /* image_bytes - 4096                           */
+    uint64_t entry_size;       /* 0 = binary                                   */
+    uint64_t ntiles;           /* stored tiles (including empty stubs)         */
+} fmx_spm_info;
+
+/* ---- context -------------------------------------------------------------
+ * Replaces fm::init_flash_matrix / destroy_flash_matrix
+ * (matrix/sparse_matrix.cpp:835-872): binds a device and owns a stream. */
+fmx_status fmx_ctx_create(int device, fmx_ctx **out);
+fmx_status fmx_ctx_destroy(fmx_ctx *ctx);
+fmx_status fmx_ctx_stream(fmx_ctx *ctx, void **stream);
+fmx_status fmx_ctx_sync(fmx_ctx *ctx);
+const char *fmx_strerror(fmx_status st);
+/* Message of the last failing call on this context ("" if none). */
+const char *fmx_ctx_last_error(const fmx_ctx *ctx);
+
+/* ---- sparse matrix ---------------------------------------------------------
+ * fmx_spm_upload replaces sparse_matrix::create(SpM_2d_index::ptr,
+ * SpM_2d_storage::ptr) (matrix/sparse_matrix.h:522, .cpp:679-683): `image` is
+ * the host copy of a reference-format image (4096-byte header followed by the
+ * block rows, matrix/sparse_matrix_format.cpp:308-334) of `bytes` bytes and
+ * `offs` the nbr+1 block-row offsets of SpM_2d_index
+ * (matrix/sparse_matrix_format.h:441-485).  The image is validated
+ * (SpM_2d_index::verify, sparse_block_2d::verify,
+ * matrix/sparse_matrix_format.cpp:36-60,128-136) and copied into HBM. */
+fmx_status fmx_spm_upload(fmx_ctx *ctx, const void *image, size_t bytes,
+                          const int64_t *offs, size_t nbr, fmx_spm **out);
+
+/* fmx_spm_build replaces fm::create_2d_matrix(data_frame, block_2d_size, ...)
+ * (matrix/fm_utils.h:37-39, matrix/fm_utils.cpp:617-715,726-841): builds the
+ * tile image on the device from a DEVICE edge list src[m], dst[m] (u32 ids).
+ * n == 0 takes n = max id + 1 as the reference does; a non-zero n must exceed
+ * every id.  H = 2^log2_h, W = 2^log2_w, both <= 32768.  Bit-exact with the
+ * reference writer. */
+fmx_status fmx_spm_build(fmx_ctx *ctx, const uint32_t *src, const uint32_t *dst,
+                         size_t m, uint64_t n, uint32_t log2_h, uint32_t log2_w,
+                         uint32_t flags, fmx_spm **out);
+
+fmx_status fmx_spm_info_get(const fmx_spm *spm, fmx_spm_info *info);
+
+/* Copies the image (cap >= image_bytes) and the nbr+1 offsets to the host;
+ * the counterpart of SpM_2d_storage::dump / SpM_2d_index::dump
+ * (matrix/sparse_matrix_format.cpp:164-181,417-420). */
+fmx_status fmx_spm_download(const fmx_spm *spm, void *image, size_t cap,
+                            size_t *bytes, int64_t *offs);
+fmx_status fmx_spm_destroy(fmx_spm *spm);
+
+/* ---- SpMM ------------------------------------------------------------------
+ * Y = A * X, replacing sparse_matrix::multiply(in, out)
+ * (matrix/sparse_matrix.h:636-644, matrix/sparse_matrix.cpp:709-831) with
+ * spmm_creator<T,T> (matrix/sparse_matrix.h:377-436).  X is ncols x k and Y
+ * nrows x k, both row-major DEVICE buffers with leading dimensions ldx, ldy
+ * (elements).  Y is overwritten; rows without non-zeros become 0
+ * (matrix/sparse_matrix.cpp:339-345).  dtype is one of I32, I64, F32, F64;
+ * a weighted image must carry values of the same type (sparse_matrix.cpp:
+ * 754-766).  Integer results are exact; floating-point results are
+ * deterministic from call to call. */
+fmx_status fmx_spmm(fmx_ctx *ctx, const fmx_spm *spm, fmx_dtype dtype, int k,
+                    const void *X, int64_t ldx, void *Y, int64_t ldy,
+                    void *stream);
+
+/* The algorithmic bytes one fmx_spmm moves: tile bytes + 8*(nbr+1) +
+ * ncols*k*s (X read once) + nrows*k*s (Y written once). */
+fmx_status fmx_spmm_alg_bytes(const fmx_spm *spm, fmx_dtype dtype, int k,
+                              uint64_t *bytes);
+
+/* ---- tall-skinny dense operations (device operands, row-major) -------------- */
+#define FMX_GRAM_F64  0  /* FP64 accumulation                              */
+#define FMX_GRAM_BF16 1  /* BF16 MFMA products, FP32 accumulation (F32 in)  */
+
+/* G[i][j] = sum_r A[r][i] * B[r][j]: G is ka x kb FP64 on the device.
+ * Replaces MvTransMv / block_multi_vector::MvTransMv
+ * (matrix/eigensolver/FM_MultiVector.h:499-558,
+ * matrix/eigensolver/block_dense_matrix.cpp:1429-1513).  ka, kb <= 256. */
+fmx_status fmx_gram(fmx_ctx *ctx, fmx_dtype dtype, int mode, uint64_t n, int ka, int kb,
+                    const void *A, int64_t lda, const void *B, int64_t ldb, double *G,
+                    void *stream);
+
+/* Y = alpha * A * C + beta * Y with A n x ka, C ka x kc (FP64, device,
+ * row-major), Y n x kc; FP64 accumulation.  Replaces MvTimesMatAddMv /
+ * gemm_op::run (FM_MultiVector.h:389-433, block_dense_matrix.cpp:822-914).
+ * ka * kc <= 8192. */
+fmx_status fmx_gemm_tall(fmx_ctx *ctx, fmx_dtype dtype, uint64_t n, int ka, int kc,
+                         double alpha, const void *A, int64_t lda, const double *C,
+                         double beta, void *Y, int64_t ldy, void *stream);
+
+/* norms[j] = ||A[:, j]||_2 (FP64, device), MvNorm (FM_MultiVector.h:589-628). */
+fmx_status fmx_colnorm(fmx_ctx *ctx, fmx_dtype dtype, uint64_t n, int k, const void *A,
+                       int64_t lda, double *norms, void *stream);
+
+/* Y[i][:] = d[i] * X[i][:] (d FP64 on the device): the diagonal scaling of
+ * the normalised-adjacency operator (matrix/test/test-eigen.cpp:112-127). */
+fmx_status fmx_row_scale(fmx_ctx *ctx, fmx_dtype dtype, uint64_t n, int k, const double *d,
+                         const void *X, int64_t ldx, void *Y, int64_t ldy, void *stream);
+
+/* ---- seeded generators (device output) ------------------------------------
+ * R-MAT edges with the project's SplitMix64 specification (the reference's
+ * generator, flash-graph/tools/rmat-gen.cpp:43-48, is a time-seeded Boost
+ * rmat_iterator); thr_* = floor(cumulative probability * 2^32). */
+fmx_status fmx_rmat_edges(fmx_ctx *ctx, int scale, uint64_t m, uint64_t seed,
+                          uint32_t thr_a, uint32_t thr_ab, uint32_t thr_abc,
+                          int permute, uint32_t *src, uint32_t *dst,
+                          void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,233 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and
+by the cpu_baseline leg of bench.py as the checker / CPU baseline.  The
+product package (flashx_amd) never imports this module.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+I32, I64, F32, F64 = 0, 1, 2, 3
+DEDUP, KEEP_SELF, SYMMETRIZE = 1, 2, 4
+_NP2DT = {np.dtype(np.int32): I32, np.dtype(np.int64): I64,
+          np.dtype(np.float32): F32, np.dtype(np.float64): F64}
+
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_vp = ctypes.c_void_p
+
+
+def build():
+    """Compile liboracle.so with the committed Makefile."""
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        L.orc_splitmix64.restype = ctypes.c_uint64
+        L.orc_splitmix64.argtypes = [ctypes.c_uint64, ctypes.c_uint64]
+        L.orc_rmat_perm.restype = ctypes.c_uint32
+        L.orc_rmat_perm.argtypes = [ctypes.c_uint32, ctypes.c_int, ctypes.c_uint64]
+        L.orc_rmat_edges.restype = None
+        L.orc_rmat_edges.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64,
+                                     ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                     ctypes.c_int, _u32p, _u32p]
+        L.orc_build_2d.restype = ctypes.c_int
+        L.orc_build_2d.argtypes = [_u32p, _u32p, ctypes.c_uint64, ctypes.c_uint64,
+                                   ctypes.c_int, ctypes.c_int, ctypes.c_uint,
+                                   ctypes.POINTER(_u8p), _u64p, ctypes.POINTER(_i64p),
+                                   _u64p, _u64p, _u64p]
+        L.orc_normalize.restype = ctypes.c_int
+        L.orc_normalize.argtypes = [_u32p, _u32p, ctypes.c_uint64, ctypes.c_uint64,
+                                    ctypes.c_uint, ctypes.POINTER(_u64p), _u64p, _u64p]
+        L.orc_free.restype = None
+        L.orc_free.argtypes = [_vp]
+        L.orc_verify_image.restype = ctypes.c_int
+        L.orc_verify_image.argtypes = [_u8p, ctypes.c_uint64, _i64p, ctypes.c_uint64]
+        L.orc_image_nonzeros.restype = ctypes.c_int
+        L.orc_image_nonzeros.argtypes = [_u8p, ctypes.c_uint64, _i64p, ctypes.c_uint64,
+                                         _u64p, _u64p, ctypes.c_uint64, _u64p]
+        L.orc_spmm.restype = ctypes.c_int
+        L.orc_spmm.argtypes = [_u8p, ctypes.c_uint64, _i64p, ctypes.c_uint64,
+                               ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int64, _vp,
+                               ctypes.c_int64]
+        L.orc_spmm_mt.restype = ctypes.c_int
+        L.orc_spmm_mt.argtypes = [_u8p, ctypes.c_uint64, _i64p, ctypes.c_uint64,
+                                  ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int64, _vp,
+                                  ctypes.c_int64, ctypes.c_int, ctypes.c_int]
+        for nm in ("orc_gram_f64",):
+            getattr(L, nm).restype = None
+        L.orc_gram_f64.argtypes = [_vp, ctypes.c_int64, _vp, ctypes.c_int64,
+                                   ctypes.c_uint64, ctypes.c_int, ctypes.c_int, _vp]
+        L.orc_gemm_tall_f64.restype = None
+        L.orc_gemm_tall_f64.argtypes = [_vp, ctypes.c_int64, ctypes.c_uint64, ctypes.c_int,
+                                        _vp, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                        _vp, ctypes.c_int64]
+        L.orc_colnorm_f64.restype = None
+        L.orc_colnorm_f64.argtypes = [_vp, ctypes.c_int64, ctypes.c_uint64, ctypes.c_int, _vp]
+        _lib = L
+    return _lib
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+def rmat_thresholds(a=0.57, b=0.19, c=0.19):
+    """floor(cumulative probability * 2^32), clamped to u32."""
+    f = lambda x: min(int(np.floor(x * 4294967296.0)), 0xFFFFFFFF)
+    return f(a), f(a + b), f(a + b + c)
+
+
+def splitmix64(key, ctr):
+    return lib().orc_splitmix64(key, ctr)
+
+
+def rmat_perm(v, scale, seed):
+    return lib().orc_rmat_perm(v, scale, seed)
+
+
+def rmat_edges(scale, m, seed, a=0.57, b=0.19, c=0.19, permute=True):
+    ta, tab, tabc = rmat_thresholds(a, b, c)
+    src = np.empty(m, np.uint32)
+    dst = np.empty(m, np.uint32)
+    lib().orc_rmat_edges(scale, m, seed, ta, tab, tabc, int(permute),
+                         _p(src, _u32p), _p(dst, _u32p))
+    return src, dst
+
+
+class Image:
+    """A reference-format tile image plus its block-row offsets."""
+
+    def __init__(self, data, offs, n, nnz=None):
+        self.data = np.ascontiguousarray(data, dtype=np.uint8)
+        self.offs = np.ascontiguousarray(offs, dtype=np.int64)
+        self.nbr = len(self.offs) - 1
+        self.n = n
+        self.nnz = nnz
+
+    @property
+    def H(self):
+        return int(self.data[48:56].view(np.uint64)[0])
+
+    @property
+    def W(self):
+        return int(self.data[56:64].view(np.uint64)[0])
+
+
+def build_2d(src, dst, n=0, log2h=15, log2w=15, flags=DEDUP):
+    src = np.ascontiguousarray(src, dtype=np.uint32)
+    dst = np.ascontiguousarray(dst, dtype=np.uint32)
+    img = _u8p()
+    nbytes = ctypes.c_uint64()
+    offs = _i64p()
+    nbr = ctypes.c_uint64()
+    n_out = ctypes.c_uint64()
+    nnz = ctypes.c_uint64()
+    rc = lib().orc_build_2d(_p(src, _u32p), _p(dst, _u32p), len(src), n, log2h, log2w,
+                            flags, ctypes.byref(img), ctypes.byref(nbytes),
+                            ctypes.byref(offs), ctypes.byref(nbr), ctypes.byref(n_out),
+                            ctypes.byref(nnz))
+    if rc:
+        raise ValueError("orc_build_2d failed with status %d" % rc)
+    data = np.ctypeslib.as_array(img, shape=(nbytes.value,)).copy()
+    o = np.ctypeslib.as_array(offs, shape=(nbr.value + 1,)).copy()
+    lib().orc_free(ctypes.cast(img, _vp))
+    lib().orc_free(ctypes.cast(offs, _vp))
+    return Image(data, o, n_out.value, nnz.value)
+
+
+def normalize(src, dst, n=0, flags=DEDUP):
+    src = np.ascontiguousarray(src, dtype=np.uint32)
+    dst = np.ascontiguousarray(dst, dtype=np.uint32)
+    keys = _u64p()
+    nk = ctypes.c_uint64()
+    n_out = ctypes.c_uint64()
+    rc = lib().orc_normalize(_p(src, _u32p), _p(dst, _u32p), len(src), n, flags,
+                             ctypes.byref(keys), ctypes.byref(nk), ctypes.byref(n_out))
+    if rc:
+        raise ValueError("orc_normalize failed with status %d" % rc)
+    out = np.ctypeslib.as_array(keys, shape=(max(nk.value, 1),))[:nk.value].copy()
+    lib().orc_free(ctypes.cast(keys, _vp))
+    return out, n_out.value
+
+
+def verify(image):
+    return lib().orc_verify_image(_p(image.data, _u8p), len(image.data),
+                                  _p(image.offs, _i64p), image.nbr)
+
+
+def nonzeros(image, cap=None):
+    if cap is None:
+        cap = len(image.data)  # at most one non-zero per two image bytes
+    rows = np.empty(cap, np.uint64)
+    cols = np.empty(cap, np.uint64)
+    cnt = ctypes.c_uint64()
+    rc = lib().orc_image_nonzeros(_p(image.data, _u8p), len(image.data),
+                                  _p(image.offs, _i64p), image.nbr,
+                                  _p(rows, _u64p), _p(cols, _u64p), cap,
+                                  ctypes.byref(cnt))
+    if rc:
+        raise ValueError("orc_image_nonzeros failed with status %d" % rc)
+    return rows[:cnt.value], cols[:cnt.value]
+
+
+def spmm(image, X, threads=0, copy=True):
+    """Y = A*X in the reference order (threads=0) or with the multi-threaded
+    restatement of the reference engine (threads>=1)."""
+    X = np.ascontiguousarray(X)
+    if X.ndim == 1:
+        X = X.reshape(-1, 1)
+    dt = _NP2DT[X.dtype]
+    n_rows = int(image.data[24:32].view(np.uint64)[0])
+    k = X.shape[1]
+    Y = np.empty((n_rows, k), X.dtype)
+    if threads <= 0:
+        rc = lib().orc_spmm(_p(image.data, _u8p), len(image.data), _p(image.offs, _i64p),
+                            image.nbr, dt, k, X.ctypes.data, k, Y.ctypes.data, k)
+    else:
+        rc = lib().orc_spmm_mt(_p(image.data, _u8p), len(image.data),
+                               _p(image.offs, _i64p), image.nbr, dt, k, X.ctypes.data,
+                               k, Y.ctypes.data, k, threads, int(copy))
+    if rc:
+        raise ValueError("orc_spmm failed with status %d" % rc)
+    return Y
+
+
+def gram_f64(A, B):
+    A = np.ascontiguousarray(A, np.float64)
+    B = np.ascontiguousarray(B, np.float64)
+    G = np.empty((A.shape[1], B.shape[1]), np.float64)
+    lib().orc_gram_f64(A.ctypes.data, A.shape[1], B.ctypes.data, B.shape[1], A.shape[0],
+                       A.shape[1], B.shape[1], G.ctypes.data)
+    return G
+
+
+def gemm_tall_f64(A, C, alpha=1.0, beta=0.0, Y=None):
+    A = np.ascontiguousarray(A, np.float64)
+    C = np.ascontiguousarray(C, np.float64)
+    if Y is None:
+        Y = np.zeros((A.shape[0], C.shape[1]), np.float64)
+    lib().orc_gemm_tall_f64(A.ctypes.data, A.shape[1], A.shape[0], A.shape[1],
+                            C.ctypes.data, C.shape[1], alpha, beta, Y.ctypes.data, Y.shape[1])
+    return Y
+
+
+def colnorm_f64(A):
+    A = np.ascontiguousarray(A, np.float64)
+    out = np.empty(A.shape[1], np.float64)
+    lib().orc_colnorm_f64(A.ctypes.data, A.shape[1], A.shape[0], A.shape[1], out.ctypes.data)
+    return out

@@ -1,0 +1,72 @@
+"""GPU parity of the tall-skinny dense operations against the oracle
+(oracle/fmx_oracle.c: orc_gram_f64, orc_gemm_tall_f64, orc_colnorm_f64)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(x):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(x)).cuda()
+
+
+def bf16_round(x):
+    u = np.asarray(x, np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32)
+
+
+@pytest.mark.parametrize("n,ka,kb", [(100003, 128, 16), (5000, 16, 16), (777, 20, 3), (65536, 64, 64),
+                                     (1, 5, 7)])
+def test_gram_f64(fmx, orc, n, ka, kb):
+    rng = np.random.default_rng(n + ka)
+    for dt in (np.float32, np.float64):
+        A = rng.standard_normal((n, ka)).astype(dt)
+        B = rng.standard_normal((n, kb)).astype(dt)
+        G = fmx.gram(dev(A), dev(B)).cpu().numpy()
+        ref = orc.gram_f64(A.astype(np.float64), B.astype(np.float64))
+        scale = np.abs(A.astype(np.float64)).T @ np.abs(B.astype(np.float64))
+        assert np.all(np.abs(G - ref) <= 1e-13 * scale + 1e-300)
+
+
+@pytest.mark.parametrize("n,ka,kb", [(100003, 128, 16), (4096, 16, 16), (999, 20, 3), (65536, 64, 48)])
+def test_gram_bf16_mfma(fmx, orc, n, ka, kb):
+    rng = np.random.default_rng(n)
+    A = rng.standard_normal((n, ka)).astype(np.float32)
+    B = rng.standard_normal((n, kb)).astype(np.float32)
+    G = fmx.gram(dev(A), dev(B), mode=fmx.GRAM_BF16).cpu().numpy()
+    # products of BF16 values are exact in FP32; FP32 accumulation bounds the rest
+    Ab = bf16_round(A).astype(np.float64)
+    Bb = bf16_round(B).astype(np.float64)
+    ref = orc.gram_f64(Ab, Bb)
+    scale = np.abs(Ab).T @ np.abs(Bb)
+    assert np.all(np.abs(G - ref) <= 2e-6 * scale)
+    # and close to the FP64 Gram of the unrounded data (BF16 rounding ~ 2^-9)
+    exact = orc.gram_f64(A.astype(np.float64), B.astype(np.float64))
+    assert np.all(np.abs(G - exact) <= 1e-2 * scale)
+
+
+@pytest.mark.parametrize("n,ka,kc", [(100003, 128, 16), (3000, 16, 48), (10, 3, 5)])
+def test_gemm_tall(fmx, orc, n, ka, kc):
+    rng = np.random.default_rng(ka * kc)
+    C = rng.standard_normal((ka, kc))
+    for dt in (np.float32, np.float64):
+        A = rng.standard_normal((n, ka)).astype(dt)
+        Y0 = rng.standard_normal((n, kc)).astype(dt)
+        Y = fmx.gemm_tall(dev(A), dev(C), alpha=0.5, beta=-2.0, Y=dev(Y0)).cpu().numpy()
+        ref = orc.gemm_tall_f64(A.astype(np.float64), C, 0.5, -2.0, Y0.astype(np.float64).copy())
+        tol = 1e-6 if dt == np.float32 else 1e-12
+        scale = 0.5 * np.abs(A.astype(np.float64)) @ np.abs(C) + 2 * np.abs(Y0)
+        assert np.all(np.abs(Y - ref) <= tol * scale)
+
+
+def test_colnorm_and_row_scale(fmx, orc):
+    rng = np.random.default_rng(0)
+    for dt in (np.float32, np.float64):
+        A = rng.standard_normal((123457, 16)).astype(dt)
+        nrm = fmx.colnorm(dev(A)).cpu().numpy()
+        assert np.allclose(nrm, orc.colnorm_f64(A.astype(np.float64)), rtol=1e-13)
+        d = rng.standard_normal(123457)
+        Y = fmx.row_scale(dev(d), dev(A)).cpu().numpy()
+        assert np.array_equal(Y, (A.astype(np.float64) * d[:, None]).astype(dt))
