@@ -50,36 +50,59 @@ struct KeyFmt {
 __device__ __forceinline__ uint64_t gtid() { return blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; }
 __device__ __forceinline__ uint64_t gstride() { return (uint64_t)gridDim.x * blockDim.x; }
 
+// One reservation per block and 8 edges per thread keep the global counter
+// cold; the order of keys is irrelevant (they are sorted next).
+constexpr int kEPT = 8;
+
 __global__ void __launch_bounds__(kB) make_keys(const uint32_t *src, const uint32_t *dst, uint64_t m,
                                                  KeyFmt f, int keep_self, int sym, uint64_t *keys,
                                                  unsigned long long *counter)
 {
-    const int lane = threadIdx.x & 63;
-    for (uint64_t base = blockIdx.x * (uint64_t)kB; base < m; base += gstride()) {
-        const uint64_t i = base + threadIdx.x;
-        uint32_t s = 0, d = 0;
-        bool ok = false;
-        if (i < m) {
-            s = src[i];
-            d = dst[i];
-            ok = keep_self || s != d;
+    __shared__ uint32_t wsum[kB / 64];
+    __shared__ unsigned long long bbase;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (uint64_t base = blockIdx.x * (uint64_t)kB * kEPT; base < m;
+         base += (uint64_t)gridDim.x * kB * kEPT) {
+        uint32_t s[kEPT], d[kEPT];
+        bool ok[kEPT];
+        uint32_t cnt = 0;
+#pragma unroll
+        for (int t = 0; t < kEPT; t++) {
+            const uint64_t i = base + (uint64_t)t * kB + threadIdx.x;
+            s[t] = 0;
+            d[t] = 0;
+            ok[t] = false;
+            if (i < m) {
+                s[t] = src[i];
+                d[t] = dst[i];
+                ok[t] = keep_self || s[t] != d[t];
+            }
+            cnt += ok[t] ? ((sym && s[t] != d[t]) ? 2u : 1u) : 0u;
         }
-        const bool two = ok && sym && s != d;
-        const uint32_t cnt = ok ? (two ? 2u : 1u) : 0u;
-        // wave-aggregated reservation
         uint32_t inc = cnt;
 #pragma unroll
         for (int dd = 1; dd < 64; dd <<= 1) {
             uint32_t o = __shfl_up(inc, dd, 64);
             if (lane >= dd) inc += o;
         }
-        const uint32_t wtot = __shfl(inc, 63, 64);
-        unsigned long long wbase = 0;
-        if (lane == 0 && wtot) wbase = atomicAdd(counter, (unsigned long long)wtot);
-        wbase = __shfl(wbase, 0, 64);
-        uint64_t pos = wbase + inc - cnt;
-        if (ok) keys[pos] = f.key(s, d);
-        if (two) keys[pos + 1] = f.key(d, s);
+        if (lane == 63) wsum[w] = inc;
+        __syncthreads();
+        uint32_t woff = 0, btot = 0;
+#pragma unroll
+        for (int q = 0; q < kB / 64; q++) {
+            if (q < w) woff += wsum[q];
+            btot += wsum[q];
+        }
+        if (threadIdx.x == 0) bbase = btot ? atomicAdd(counter, (unsigned long long)btot) : 0ull;
+        __syncthreads();
+        uint64_t pos = bbase + woff + inc - cnt;
+#pragma unroll
+        for (int t = 0; t < kEPT; t++) {
+            if (!ok[t]) continue;
+            keys[pos++] = f.key(s[t], d[t]);
+            if (sym && s[t] != d[t]) keys[pos++] = f.key(d[t], s[t]);
+        }
+        __syncthreads();
     }
 }
 

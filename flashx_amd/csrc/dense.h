@@ -11,6 +11,9 @@ fmx_status gram(fmx_ctx *ctx, fmx_dtype dt, int mode, uint64_t n, int ka, int kb
 fmx_status gemm_tall(fmx_ctx *ctx, fmx_dtype dt, uint64_t n, int ka, int kc, double alpha,
                      const void *A, int64_t lda, const double *C, double beta, void *Y, int64_t ldy,
                      hipStream_t s);
+fmx_status gemm_tall_f32_f64(fmx_ctx *ctx, uint64_t n, int ka, int kc, double alpha, const float *A,
+                             int64_t lda, const double *C, double beta, double *Y, int64_t ldy,
+                             hipStream_t s);
 fmx_status colnorm(fmx_ctx *ctx, fmx_dtype dt, uint64_t n, int k, const void *A, int64_t lda,
                    double *norms, int squared, hipStream_t s);
 fmx_status row_scale(fmx_ctx *ctx, fmx_dtype dt, uint64_t n, int k, const double *d, const void *X,

@@ -168,10 +168,10 @@ __global__ void __launch_bounds__(kT) gram_bf16_kernel(const float *A, int64_t l
 // Y[r][j] = alpha * sum_i A[r][i] C[i][j] + beta * Y[r][j], FP64 accumulation.
 constexpr int kMRows = 32;
 
-template <typename T>
+template <typename T, typename TY = T>
 __global__ void __launch_bounds__(kT) gemm_tall_kernel(const T *A, int64_t lda, uint64_t n, int ka,
                                                        int kc, const double *C, double alpha,
-                                                       double beta, T *Y, int64_t ldy)
+                                                       double beta, TY *Y, int64_t ldy)
 {
     extern __shared__ double sg[];
     double *Cs = sg;                 // ka x kc
@@ -189,9 +189,9 @@ __global__ void __launch_bounds__(kT) gemm_tall_kernel(const T *A, int64_t lda, 
             const int r = o / kc, j = o - r * kc;
             double s = 0.0;
             for (int i = 0; i < ka; i++) s = fma(As[r * ka + i], Cs[i * kc + j], s);
-            T *y = Y + (rb + r) * ldy + j;
+            TY *y = Y + (rb + r) * ldy + j;
             const double prev = (beta == 0.0) ? 0.0 : beta * (double)*y;
-            *y = (T)(prev + alpha * s);
+            *y = (TY)(prev + alpha * s);
         }
     }
 }
@@ -306,6 +306,26 @@ fmx_status gram(fmx_ctx *ctx, fmx_dtype dt, int mode, uint64_t n, int ka, int kb
     FMX_HIP(ctx, hipGetLastError());
     hipLaunchKernelGGL((reduce_partials<double, double>), dim3((nout + kT - 1) / kT), dim3(kT), 0, s,
                        (const double *)ws, (int)nblk, nout, G, 0);
+    FMX_HIP(ctx, hipGetLastError());
+    return FMX_OK;
+}
+
+fmx_status gemm_tall_f32_f64(fmx_ctx *ctx, uint64_t n, int ka, int kc, double alpha, const float *A,
+                             int64_t lda, const double *C, double beta, double *Y, int64_t ldy,
+                             hipStream_t s)
+{
+    if (ka <= 0 || kc <= 0 || (size_t)ka * kc > 8192 || ka > 512)
+        return fail(ctx, FMX_EDIM, "gemm_tall needs ka*kc <= 8192");
+    const size_t lds = ((size_t)ka * kc + (size_t)kMRows * ka) * sizeof(double);
+    const uint32_t nblk = blocks_for(ctx, n, kMRows);
+    static bool attr = false;
+    if (!attr) {
+        FMX_HIP(ctx, hipFuncSetAttribute((const void *)gemm_tall_kernel<float, double>,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+        attr = true;
+    }
+    hipLaunchKernelGGL((gemm_tall_kernel<float, double>), dim3(nblk), dim3(kT), lds, s, A, lda, n, ka, kc,
+                       C, alpha, beta, Y, ldy);
     FMX_HIP(ctx, hipGetLastError());
     return FMX_OK;
 }

@@ -174,11 +174,9 @@ fmx_status analyse_image(fmx_ctx *ctx, fmx_spm *s)
 
 uint32_t band_rows(const fmx_spm *spm, int k, int s)
 {
-    static const uint64_t budget = [] {
-        const char *e = getenv("FMX_BAND_BYTES");
-        uint64_t v = e ? strtoull(e, nullptr, 10) : 16384;
-        return v < 256 ? 256 : v;
-    }();
+    const char *e = getenv("FMX_BAND_BYTES");
+    uint64_t budget = e ? strtoull(e, nullptr, 10) : 16384;
+    if (budget < 256) budget = 256;
     uint64_t rows = budget / ((uint64_t)k * s);
     uint32_t hs = 1;
     while ((uint64_t)hs * 2 <= rows && hs < 32768) hs *= 2;

@@ -1,0 +1,347 @@
+// small.hip -- small dense FP64 kernels of the block eigensolver, each run by
+// one workgroup: the symmetric Rayleigh-Ritz eigenproblem (parallel cyclic
+// Jacobi), Cholesky with triangular inverse for Cholesky-QR, the projected
+// matrix update, residual norms and precision casts.
+//
+// In the reference these steps run inside Trilinos/Anasazi on the CPU
+// (LAPACK through Teuchos, matrix/eigensolver/eigensolver.cpp:215-235).
+#include "small.h"
+
+namespace fmx {
+namespace {
+
+constexpr int kJT = 512;   // threads of the Jacobi workgroup
+
+// Cyclic Jacobi for a symmetric m x m matrix (m even, m <= 128): A is kept
+// in LDS, the eigenvector matrix V (row-major m x m) in global memory.  Each
+// round rotates m/2 disjoint pairs (round-robin ordering), rows first then
+// columns, so A' = P^T A P exactly as the sequential method would.
+__global__ void __launch_bounds__(kJT) jacobi_kernel(const double *Ain, int m, double *evals,
+                                                     double *V, int max_sweeps, double *info)
+{
+    extern __shared__ double A[];        // m * m
+    __shared__ double cs[64], sn[64];
+    __shared__ int pp[64], qq[64];
+    __shared__ double red[kJT];
+    const int tid = threadIdx.x;
+    for (int e = tid; e < m * m; e += kJT) {
+        A[e] = 0.5 * (Ain[e] + Ain[(e % m) * m + e / m]);  // symmetrise
+        V[e] = (e / m == e % m) ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    const int half = m / 2;
+    int sweep = 0;
+    double off = 0.0, tot = 0.0;
+    for (; sweep < max_sweeps; sweep++) {
+        // convergence test: off-diagonal Frobenius norm vs whole norm
+        double so = 0.0, st = 0.0;
+        for (int e = tid; e < m * m; e += kJT) {
+            double v = A[e] * A[e];
+            st += v;
+            if (e / m != e % m) so += v;
+        }
+        red[tid] = so;
+        __syncthreads();
+        for (int s = kJT / 2; s; s >>= 1) {
+            if (tid < s) red[tid] += red[tid + s];
+            __syncthreads();
+        }
+        off = red[0];
+        __syncthreads();
+        red[tid] = st;
+        __syncthreads();
+        for (int s = kJT / 2; s; s >>= 1) {
+            if (tid < s) red[tid] += red[tid + s];
+            __syncthreads();
+        }
+        tot = red[0];
+        __syncthreads();
+        if (off <= 1e-30 * tot || off == 0.0) break;
+        for (int r = 0; r < m - 1; r++) {
+            if (tid < half) {
+                const int k = tid;
+                int a = (k == 0) ? 0 : ((r + k - 1) % (m - 1)) + 1;
+                int b = ((r + (m - 1 - k) - 1) % (m - 1)) + 1;
+                if (a > b) { int t = a; a = b; b = t; }
+                pp[k] = a;
+                qq[k] = b;
+                const double apq = A[a * m + b];
+                double c = 1.0, s = 0.0;
+                if (apq != 0.0) {
+                    const double theta = (A[b * m + b] - A[a * m + a]) / (2.0 * apq);
+                    const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+                    c = 1.0 / sqrt(t * t + 1.0);
+                    s = t * c;
+                }
+                cs[k] = c;
+                sn[k] = s;
+            }
+            __syncthreads();
+            // rows p, q
+            for (int e = tid; e < half * m; e += kJT) {
+                const int k = e / m, col = e - k * m;
+                const int p = pp[k], q = qq[k];
+                const double c = cs[k], s = sn[k];
+                const double ap = A[p * m + col], aq = A[q * m + col];
+                A[p * m + col] = c * ap - s * aq;
+                A[q * m + col] = s * ap + c * aq;
+            }
+            __syncthreads();
+            // columns p, q of A and V
+            for (int e = tid; e < half * m; e += kJT) {
+                const int k = e / m, row = e - k * m;
+                const int p = pp[k], q = qq[k];
+                const double c = cs[k], s = sn[k];
+                const double ap = A[row * m + p], aq = A[row * m + q];
+                A[row * m + p] = c * ap - s * aq;
+                A[row * m + q] = s * ap + c * aq;
+                const double vp = V[row * m + p], vq = V[row * m + q];
+                V[row * m + p] = c * vp - s * vq;
+                V[row * m + q] = s * vp + c * vq;
+            }
+            __syncthreads();
+        }
+    }
+    for (int i = tid; i < m; i += kJT) evals[i] = A[i * m + i];
+    if (tid == 0 && info) {
+        info[0] = sweep;
+        info[1] = sqrt(off / (tot > 0 ? tot : 1.0));
+    }
+}
+
+// G = R^T R (R upper), Rinv = R^-1; info[0] = 0 on success, else 1-based
+// index of the first non-positive pivot (relative threshold eps).
+__global__ void __launch_bounds__(64) chol_inv_kernel(const double *G, int b, double *Rinv, double eps,
+                                                      double *info)
+{
+    __shared__ double R[32 * 32];
+    __shared__ double I[32 * 32];
+    __shared__ int bad;
+    const int tid = threadIdx.x;
+    if (tid == 0) bad = 0;
+    for (int e = tid; e < b * b; e += 64) {
+        R[e] = 0.5 * (G[e] + G[(e % b) * b + e / b]);
+        I[e] = 0.0;
+    }
+    __syncthreads();
+    double dmax = 0.0;
+    for (int i = 0; i < b; i++) dmax = fmax(dmax, R[i * b + i]);
+    // right-looking Cholesky on the upper triangle
+    for (int k = 0; k < b; k++) {
+        if (tid == 0) {
+            double d = R[k * b + k];
+            if (!(d > eps * dmax) && !bad) bad = k + 1;
+            R[k * b + k] = sqrt(d > eps * dmax ? d : eps * dmax + 1e-300);
+        }
+        __syncthreads();
+        const double rkk = R[k * b + k];
+        for (int j = k + 1 + tid; j < b; j += 64) R[k * b + j] /= rkk;
+        __syncthreads();
+        for (int e = tid; e < b * b; e += 64) {
+            const int i = e / b, j = e - i * b;
+            if (i > k && j >= i) R[i * b + j] -= R[k * b + i] * R[k * b + j];
+        }
+        __syncthreads();
+    }
+    // inverse of the upper-triangular R, column by column (back substitution)
+    for (int j = tid; j < b; j += 64) {
+        for (int i = j; i >= 0; i--) {
+            double s = (i == j) ? 1.0 : 0.0;
+            for (int l = i + 1; l <= j; l++) s -= R[i * b + l] * I[l * b + j];
+            I[i * b + j] = s / R[i * b + i];
+        }
+    }
+    __syncthreads();
+    for (int e = tid; e < b * b; e += 64) {
+        const int i = e / b, j = e - i * b;
+        Rinv[e] = (j >= i) ? I[e] : 0.0;
+    }
+    if (tid == 0) info[0] = bad;
+}
+
+// H[0:rows, c0:c0+b] = C1 + C2 and the mirrored block, H is ld x ld.
+__global__ void __launch_bounds__(256) h_update_kernel(double *H, int ld, int rows, int c0, int b,
+                                                       const double *C1, const double *C2)
+{
+    for (int e = threadIdx.x + blockIdx.x * 256; e < rows * b; e += gridDim.x * 256) {
+        const int i = e / b, j = e - i * b;
+        const double v = C1[e] + (C2 ? C2[e] : 0.0);
+        H[i * ld + c0 + j] = v;
+        H[(c0 + j) * ld + i] = v;
+    }
+}
+
+__global__ void __launch_bounds__(256) cast_kernel(const float *X, int64_t ldx, uint64_t n, int k,
+                                                   double *Y, int64_t ldy)
+{
+    const uint64_t tot = n * (uint64_t)k;
+    for (uint64_t e = blockIdx.x * 256ull + threadIdx.x; e < tot; e += (uint64_t)gridDim.x * 256) {
+        const uint64_t r = e / k, c = e - r * k;
+        Y[r * ldy + c] = (double)X[r * ldx + c];
+    }
+}
+
+__global__ void __launch_bounds__(256) cast_down_kernel(const double *X, int64_t ldx, uint64_t n, int k,
+                                                        float *Y, int64_t ldy)
+{
+    const uint64_t tot = n * (uint64_t)k;
+    for (uint64_t e = blockIdx.x * 256ull + threadIdx.x; e < tot; e += (uint64_t)gridDim.x * 256) {
+        const uint64_t r = e / k, c = e - r * k;
+        Y[r * ldy + c] = (float)X[r * ldx + c];
+    }
+}
+
+// partial[b][j] = sum over the block's rows of (Y[r][j] - theta[j] X[r][j])^2
+__global__ void __launch_bounds__(256) resid_kernel(const double *Y, int64_t ldy, const double *X,
+                                                    int64_t ldx, const double *theta, uint64_t n, int k,
+                                                    uint64_t rows_per_blk, double *partial)
+{
+    const int per = 256 / k;
+    const int c = threadIdx.x % k, rr = threadIdx.x / k;
+    double s = 0.0;
+    if (rr < per) {
+        const double th = theta[c];
+        const uint64_t r0 = blockIdx.x * rows_per_blk;
+        const uint64_t r1 = min(n, r0 + rows_per_blk);
+        for (uint64_t r = r0 + rr; r < r1; r += per) {
+            const double v = Y[r * ldy + c] - th * X[r * ldx + c];
+            s = fma(v, v, s);
+        }
+    }
+    __shared__ double red[256];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x < k) {
+        double t = 0.0;
+        for (int q = 0; q < per; q++) t += red[q * k + threadIdx.x];
+        partial[(uint64_t)blockIdx.x * k + threadIdx.x] = t;
+    }
+}
+
+__global__ void __launch_bounds__(256) sum_cols_kernel(const double *partial, int nblk, int k, double *out)
+{
+    for (int j = threadIdx.x; j < k; j += 256) {
+        double s = 0.0;
+        for (int b = 0; b < nblk; b++) s += partial[(uint64_t)b * k + j];
+        out[j] = sqrt(s);
+    }
+}
+
+// dinv[i] = deg[i] > 0 ? 1/sqrt(deg[i]) : 0  (deg as float from A*1)
+__global__ void __launch_bounds__(256) dinv_kernel(const float *deg, uint64_t n, double *dinv)
+{
+    for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) {
+        const double d = (double)deg[i];
+        dinv[i] = d > 0.0 ? 1.0 / sqrt(d) : 0.0;
+    }
+}
+
+// X[r][c] = uniform [-1, 1) from SplitMix64(seed, r * k + c) (the random
+// start of FM_MultiVector::Random, FM_MultiVector.h:164-168)
+template <typename T>
+__global__ void __launch_bounds__(256) rand_kernel(T *X, int64_t ldx, uint64_t n, int k, uint64_t seed)
+{
+    const uint64_t tot = n * (uint64_t)k;
+    for (uint64_t e = blockIdx.x * 256ull + threadIdx.x; e < tot; e += (uint64_t)gridDim.x * 256) {
+        uint64_t z = seed + (e + 1) * 0x9E3779B97F4A7C15ULL;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+        z ^= z >> 31;
+        const double u = (double)(z >> 11) * (1.0 / 9007199254740992.0);  // [0, 1)
+        const uint64_t r = e / k, c = e - r * k;
+        X[r * ldx + c] = (T)(2.0 * u - 1.0);
+    }
+}
+
+}  // namespace
+
+fmx_status rand_uniform(fmx_ctx *ctx, fmx_dtype dt, void *X, int64_t ldx, uint64_t n, int k, uint64_t seed,
+                        hipStream_t s)
+{
+    uint64_t blocks = std::min<uint64_t>((n * k + 255) / 256, (uint64_t)ctx->num_cus * 16);
+    if (!blocks) return FMX_OK;
+    if (dt == FMX_F32)
+        hipLaunchKernelGGL(rand_kernel<float>, dim3((uint32_t)blocks), dim3(256), 0, s, (float *)X, ldx, n, k,
+                           seed);
+    else
+        hipLaunchKernelGGL(rand_kernel<double>, dim3((uint32_t)blocks), dim3(256), 0, s, (double *)X, ldx, n,
+                           k, seed);
+    FMX_HIP(ctx, hipGetLastError());
+    return FMX_OK;
+}
+
+fmx_status small_sym_eig(fmx_ctx *ctx, const double *H, int m, double *evals, double *V, double *info,
+                         hipStream_t s)
+{
+    if (m <= 0 || m > 128 || (m & 1)) return fail(ctx, FMX_EDIM, "Rayleigh-Ritz size must be even and <= 128");
+    static bool attr = false;
+    if (!attr) {
+        FMX_HIP(ctx, hipFuncSetAttribute((const void *)jacobi_kernel,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
+        attr = true;
+    }
+    hipLaunchKernelGGL(jacobi_kernel, dim3(1), dim3(kJT), (size_t)m * m * sizeof(double), s, H, m, evals,
+                       V, 40, info);
+    FMX_HIP(ctx, hipGetLastError());
+    return FMX_OK;
+}
+
+fmx_status small_chol_inv(fmx_ctx *ctx, const double *G, int b, double *Rinv, double *info, hipStream_t s)
+{
+    if (b <= 0 || b > 32) return fail(ctx, FMX_EDIM, "Cholesky block must be <= 32");
+    hipLaunchKernelGGL(chol_inv_kernel, dim3(1), dim3(64), 0, s, G, b, Rinv, 1e-13, info);
+    FMX_HIP(ctx, hipGetLastError());
+    return FMX_OK;
+}
+
+fmx_status h_update(fmx_ctx *ctx, double *H, int ld, int rows, int c0, int b, const double *C1,
+                    const double *C2, hipStream_t s)
+{
+    hipLaunchKernelGGL(h_update_kernel, dim3(8), dim3(256), 0, s, H, ld, rows, c0, b, C1, C2);
+    FMX_HIP(ctx, hipGetLastError());
+    return FMX_OK;
+}
+
+fmx_status cast_f32_f64(fmx_ctx *ctx, const float *X, int64_t ldx, uint64_t n, int k, double *Y,
+                        int64_t ldy, hipStream_t s)
+{
+    uint64_t blocks = std::min<uint64_t>((n * k + 255) / 256, (uint64_t)ctx->num_cus * 16);
+    if (!blocks) return FMX_OK;
+    hipLaunchKernelGGL(cast_kernel, dim3((uint32_t)blocks), dim3(256), 0, s, X, ldx, n, k, Y, ldy);
+    FMX_HIP(ctx, hipGetLastError());
+    return FMX_OK;
+}
+
+fmx_status cast_f64_f32(fmx_ctx *ctx, const double *X, int64_t ldx, uint64_t n, int k, float *Y,
+                        int64_t ldy, hipStream_t s)
+{
+    uint64_t blocks = std::min<uint64_t>((n * k + 255) / 256, (uint64_t)ctx->num_cus * 16);
+    if (!blocks) return FMX_OK;
+    hipLaunchKernelGGL(cast_down_kernel, dim3((uint32_t)blocks), dim3(256), 0, s, X, ldx, n, k, Y, ldy);
+    FMX_HIP(ctx, hipGetLastError());
+    return FMX_OK;
+}
+
+fmx_status resid_norms(fmx_ctx *ctx, const double *Y, int64_t ldy, const double *X, int64_t ldx,
+                       const double *theta, uint64_t n, int k, double *out, double *partial,
+                       int nblk, hipStream_t s)
+{
+    if (k <= 0 || k > 256) return fail(ctx, FMX_EDIM, "residual width must be <= 256");
+    const uint64_t rpb = (n + nblk - 1) / nblk;
+    hipLaunchKernelGGL(resid_kernel, dim3(nblk), dim3(256), 0, s, Y, ldy, X, ldx, theta, n, k, rpb, partial);
+    FMX_HIP(ctx, hipGetLastError());
+    hipLaunchKernelGGL(sum_cols_kernel, dim3(1), dim3(256), 0, s, partial, nblk, k, out);
+    FMX_HIP(ctx, hipGetLastError());
+    return FMX_OK;
+}
+
+fmx_status degree_inv_sqrt(fmx_ctx *ctx, const float *deg, uint64_t n, double *dinv, hipStream_t s)
+{
+    uint64_t blocks = std::min<uint64_t>((n + 255) / 256, (uint64_t)ctx->num_cus * 16);
+    if (!blocks) return FMX_OK;
+    hipLaunchKernelGGL(dinv_kernel, dim3((uint32_t)blocks), dim3(256), 0, s, deg, n, dinv);
+    FMX_HIP(ctx, hipGetLastError());
+    return FMX_OK;
+}
+
+}  // namespace fmx

@@ -19,6 +19,8 @@
 //     partial slot when a power-law band was split over several items; split
 //     bands are summed slot by slot, in stream order, by spmm_reduce.
 // Each band is owned by exactly one wave, so results do not depend on timing.
+#include <cstdlib>
+
 #include "fmx_internal.h"
 
 namespace fmx {
@@ -39,6 +41,7 @@ struct SpmmArgs {
     uint32_t hs;
     uint32_t wave_bytes;
     uint32_t band_bytes;
+    uint32_t band_stride;
     int log2w;
     int k;
     uint64_t ncols;
@@ -77,32 +80,33 @@ struct Lanes {
     static constexpr int NZP = 64 / L;                     // non-zeros per pass
 };
 
-// Drains queue entries [0, qn) into the band.  K > 0: compile-time width.
+// Drains queue entries [0, qn) (qn >= 1) into the band.  K > 0: compile-time
+// width.  Loads are unconditional (indices clamped to the last entry) so the
+// P gathers of a batch are all in flight before the first LDS add.
 template <typename T, int K, int P>
-__device__ __forceinline__ void drain(const SpmmArgs &a, T *band, const uint32_t *qcol,
+__device__ __forceinline__ void drain(const SpmmArgs &a, T *band, uint32_t S, const uint32_t *qcol,
                                       const uint16_t *qrow, uint32_t qn, int lane)
 {
     using C = Lanes<T, K>;
     const T *X = (const T *)a.X;
     const int q = lane % C::L;
     const int g = lane / C::L;
+    const uint32_t last = qn - 1;
     for (uint32_t base = 0; base < qn; base += C::NZP * P) {
         Vec<T, C::E> v[P];
         uint32_t rr[P];
 #pragma unroll
         for (int pp = 0; pp < P; pp++) {
-            uint32_t e = base + pp * C::NZP + g;
-            rr[pp] = 0xFFFFFFFFu;
-            if (e < qn) {
-                uint32_t gc = qcol[e];
-                rr[pp] = qrow[e];
-                v[pp] = *(const Vec<T, C::E> *)(X + (uint64_t)gc * a.ldx + q * C::E);
-            }
+            const uint32_t e = base + pp * C::NZP + g;
+            const uint32_t ec = e < last ? e : last;
+            const uint32_t gc = qcol[ec];
+            rr[pp] = e <= last ? (uint32_t)qrow[ec] : 0xFFFFFFFFu;
+            v[pp] = *(const Vec<T, C::E> *)(X + (uint64_t)gc * a.ldx + q * C::E);
         }
 #pragma unroll
         for (int pp = 0; pp < P; pp++) {
             if (rr[pp] != 0xFFFFFFFFu) {
-                T *dst = band + rr[pp] * K + q * C::E;
+                T *dst = band + rr[pp] * S + q * C::E;
 #pragma unroll
                 for (int i = 0; i < C::E; i++) lds_add(dst + i, v[pp].v[i]);
             }
@@ -112,7 +116,7 @@ __device__ __forceinline__ void drain(const SpmmArgs &a, T *band, const uint32_t
 
 // Runtime width: flat (entry, element) pairs, one element per lane.
 template <typename T, int P>
-__device__ __forceinline__ void drain_generic(const SpmmArgs &a, T *band, const uint32_t *qcol,
+__device__ __forceinline__ void drain_generic(const SpmmArgs &a, T *band, uint32_t S, const uint32_t *qcol,
                                               const uint16_t *qrow, uint32_t qn, int lane)
 {
     const T *X = (const T *)a.X;
@@ -123,13 +127,11 @@ __device__ __forceinline__ void drain_generic(const SpmmArgs &a, T *band, const 
         uint32_t dst[P];
 #pragma unroll
         for (int pp = 0; pp < P; pp++) {
-            uint32_t f = base + pp * 64 + lane;
-            dst[pp] = 0xFFFFFFFFu;
-            if (f < tot) {
-                uint32_t e = f / k, j = f - e * k;
-                v[pp] = X[(uint64_t)qcol[e] * a.ldx + j];
-                dst[pp] = (uint32_t)qrow[e] * k + j;
-            }
+            const uint32_t f = base + pp * 64 + lane;
+            const uint32_t fc = f < tot ? f : tot - 1;
+            const uint32_t e = fc / k, j = fc - e * k;
+            v[pp] = X[(uint64_t)qcol[e] * a.ldx + j];
+            dst[pp] = f < tot ? (uint32_t)qrow[e] * S + j : 0xFFFFFFFFu;
         }
 #pragma unroll
         for (int pp = 0; pp < P; pp++)
@@ -163,26 +165,31 @@ __global__ void __launch_bounds__(64 * kWPB) spmm_stream(SpmmArgs a)
     int cur_row = (it.carry == kNone) ? -1 : (int)it.carry;
     uint32_t ct = it.dir_begin;
 
-    // zero the band
-    for (uint32_t i = lane; i < rows * k; i += 64) band[i] = T(0);
+    const uint32_t S = a.band_stride;  // padded band row (elements)
+    for (uint32_t i = lane; i < rows * S; i += 64) band[i] = T(0);
 
     const uint16_t *wp = a.img + brow_word;
+    const uint32_t pend = begin + nwords;  // nwords >= 1
+    const uint32_t plast = pend - 1;
     uint2 cur = a.dir[ct];
     cur.x = uni(cur.x);
     cur.y = uni(cur.y);
-    uint32_t p = begin;
-    const uint32_t pend = begin + nwords;
-    while (p < pend) {
-        // directory window: lane j holds tile ct+1+j
+    const uint32_t dlast = dir_end - 1;
+    // directory window (lane j holds tile ct+1+j) and words of the first chunk
+    uint2 dj;
+    uint32_t wrd[U];
+    {
         const uint32_t wj = ct + 1 + lane;
-        uint2 dj = make_uint2(0xFFFFFFFFu, 0u);
-        if (wj < dir_end) dj = a.dir[wj];
-        uint32_t wrd[U];
+        dj = a.dir[wj < dlast ? wj : dlast];
+        if (wj > dlast) dj = make_uint2(0xFFFFFFFFu, 0u);
 #pragma unroll
         for (int u = 0; u < U; u++) {
-            uint32_t pos = p + u * 64 + lane;
-            wrd[u] = (pos < pend) ? (uint32_t)wp[pos] : 0u;
+            const uint32_t pos = begin + u * 64 + lane;
+            wrd[u] = wp[pos < plast ? pos : plast];
         }
+    }
+    uint32_t p = begin;
+    while (p < pend) {
         uint32_t qn = 0;
 #pragma unroll
         for (int u = 0; u < U; u++) {
@@ -230,10 +237,24 @@ __global__ void __launch_bounds__(64 * kWPB) spmm_stream(SpmmArgs a)
             cur.y = rdlane(dj.y, An - 1);
             ct += An;
         }
+        // prefetch the next chunk (clamped, so always in bounds) before the
+        // gathers of this one
+        {
+            const uint32_t wj = ct + 1 + lane;
+            dj = a.dir[wj < dlast ? wj : dlast];
+            if (wj > dlast) dj = make_uint2(0xFFFFFFFFu, 0u);
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const uint32_t pos = pn + u * 64 + lane;
+                wrd[u] = wp[pos < plast ? pos : plast];
+            }
+        }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
-        if (K) drain<T, (K ? K : 1), P>(a, band, qcol, qrow, qn, lane);
-        else drain_generic<T, P>(a, band, qcol, qrow, qn, lane);
+        if (qn) {
+            if (K) drain<T, (K ? K : 1), P>(a, band, S, qcol, qrow, qn, lane);
+            else drain_generic<T, P>(a, band, S, qcol, qrow, qn, lane);
+        }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         p = pn;
@@ -251,7 +272,7 @@ __global__ void __launch_bounds__(64 * kWPB) spmm_stream(SpmmArgs a)
     }
     for (uint32_t i = lane; i < rows * k; i += 64) {
         uint32_t r = i / k, cc = i - r * k;
-        dst[r * ld + cc] = band[i];
+        dst[r * ld + cc] = band[r * S + cc];
     }
 }
 
@@ -324,7 +345,9 @@ fmx_status launch_t(fmx_ctx *ctx, const fmx_spm *spm, const Plan *plan, int k,
     a.ldy = ldy;
     a.nitems = plan->nitems;
     a.hs = plan->hs;
-    a.band_bytes = (uint32_t)(((uint64_t)plan->hs * k * sizeof(T) + 15) & ~15ull);
+    // pad band rows by one element to spread the LDS adds over the banks
+    a.band_stride = (uint32_t)k + ((k >= 4 && getenv("FMX_NO_PAD") == nullptr) ? 1u : 0u);
+    a.band_bytes = (uint32_t)(((uint64_t)plan->hs * a.band_stride * sizeof(T) + 15) & ~15ull);
     a.wave_bytes = a.band_bytes + 64 * U * 6;
     a.wave_bytes = (a.wave_bytes + 15) & ~15u;
     a.log2w = spm->log2w;

@@ -41,6 +41,18 @@ class fmx_spm_info(ctypes.Structure):
                 ("tile_bytes", _u64), ("entry_size", _u64), ("ntiles", _u64)]
 
 
+class fmx_eigen_opts(ctypes.Structure):
+    _fields_ = [("nev", ctypes.c_int), ("block_size", ctypes.c_int), ("num_blocks", ctypes.c_int),
+                ("max_restarts", ctypes.c_int), ("tol", ctypes.c_double),
+                ("precision", ctypes.c_int), ("verbose", ctypes.c_int), ("seed", _u64)]
+
+
+class fmx_eigen_stats(ctypes.Structure):
+    _fields_ = [("spmm_blocks", ctypes.c_int), ("fp64_spmm_blocks", ctypes.c_int),
+                ("restarts", ctypes.c_int), ("converged", ctypes.c_int),
+                ("max_resid", ctypes.c_double), ("seconds", ctypes.c_double)]
+
+
 # name -> (restype, argtypes); every symbol include/fmx.h declares.
 SIGNATURES = {
     "fmx_ctx_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp)]),
@@ -68,6 +80,10 @@ SIGNATURES = {
     "fmx_colnorm": (ctypes.c_int, [_vp, ctypes.c_int, _u64, ctypes.c_int, _vp, _i64, _vp, _vp]),
     "fmx_row_scale": (ctypes.c_int, [_vp, ctypes.c_int, _u64, ctypes.c_int, _vp, _vp, _i64, _vp,
                                      _i64, _vp]),
+    "fmx_eigen_opts_init": (ctypes.c_int, [ctypes.POINTER(fmx_eigen_opts), ctypes.c_int]),
+    "fmx_eigen_normalized": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(fmx_eigen_opts), _vp, _i64,
+                                            ctypes.c_int, _vp, _vp, _i64, _vp,
+                                            ctypes.POINTER(fmx_eigen_stats)]),
     "fmx_rmat_edges": (ctypes.c_int, [_vp, ctypes.c_int, _u64, _u64, ctypes.c_uint32,
                                       ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, _vp,
                                       _vp, _vp]),
@@ -336,6 +352,39 @@ def row_scale(d, X, Y=None, ctx=None):
                                   d.data_ptr(), X.data_ptr(), _ld(X), Y.data_ptr(), _ld(Y),
                                   ctx.stream_ptr()), "fmx_row_scale")
     return Y
+
+
+EIG_MIXED, EIG_FP32, EIG_FP64 = 0, 1, 2
+
+
+def compute_eigen(spm, nev=16, block_size=16, num_blocks=6, tol=1e-6, max_restarts=100,
+                  precision=EIG_MIXED, X0=None, seed=1, verbose=False):
+    """Top-nev eigenpairs of D^-1/2 A D^-1/2 (fm::eigen::compute_eigen with
+    NA_eigen_Operator).  Returns (evals, evecs n x nev FP64 device tensor,
+    relative residuals, stats dict)."""
+    torch = _torch()
+    ctx = spm.ctx
+    o = fmx_eigen_opts()
+    lib().fmx_eigen_opts_init(ctypes.byref(o), nev)
+    o.block_size, o.num_blocks, o.tol = block_size, num_blocks, tol
+    o.max_restarts, o.precision, o.seed, o.verbose = max_restarts, precision, seed, int(verbose)
+    n = spm.get_num_rows()
+    evals = np.zeros(nev, np.float64)
+    resid = np.zeros(nev, np.float64)
+    vecs = torch.empty((n, nev), dtype=torch.float64, device="cuda:%d" % ctx.device)
+    st = fmx_eigen_stats()
+    x0p, ldx0, x0dt = None, block_size, F32
+    if X0 is not None:
+        if X0.shape != (n, block_size) or X0.stride(1) != 1:
+            raise FmxError(EDIM, "warm start must be n x block_size row-major")
+        x0p, ldx0, x0dt = X0.data_ptr(), X0.stride(0), dtype_code(X0.dtype)
+    torch.cuda.synchronize()
+    ctx.check(lib().fmx_eigen_normalized(ctx.handle, spm.handle, ctypes.byref(o), x0p, ldx0, x0dt,
+                                         evals.ctypes.data, vecs.data_ptr(), nev,
+                                         resid.ctypes.data, ctypes.byref(st)),
+              "fmx_eigen_normalized")
+    stats = {f: getattr(st, f) for f, _ in fmx_eigen_stats._fields_}
+    return evals, vecs, resid, stats
 
 
 def rmat_edges(scale, m, seed, a=0.57, b=0.19, c=0.19, permute=True, ctx=None):

@@ -154,6 +154,47 @@ fmx_status fmx_colnorm(fmx_ctx *ctx, fmx_dtype dtype, uint64_t n, int k, const v
 fmx_status fmx_row_scale(fmx_ctx *ctx, fmx_dtype dtype, uint64_t n, int k, const double *d,
                          const void *X, int64_t ldx, void *Y, int64_t ldy, void *stream);
 
+/* ---- block Krylov eigensolver ---------------------------------------------- */
+#define FMX_EIG_MIXED 0  /* FP32 SpMM, BF16-MFMA first Gram pass, FP64 reorth/RR/residual */
+#define FMX_EIG_FP32  1  /* FP32 SpMM, FP64 Gram passes, FP64 RR/residual                 */
+#define FMX_EIG_FP64  2  /* everything FP64                                              */
+
+/* eigen_options (matrix/eigensolver/eigensolver.h:59-73): which is "LA"
+ * (largest algebraic), the solver is the block Krylov-Schur form. */
+typedef struct {
+    int nev;           /* wanted eigenpairs                                 */
+    int block_size;    /* columns per Krylov block (<= 32)                  */
+    int num_blocks;    /* blocks in the basis (block_size*num_blocks <= 128) */
+    int max_restarts;
+    double tol;        /* ||N x - theta x|| / |theta| (FP64) per pair       */
+    int precision;     /* FMX_EIG_*                                          */
+    int verbose;
+    uint64_t seed;     /* random start block, uniform [-1, 1)               */
+} fmx_eigen_opts;
+
+typedef struct {
+    int spmm_blocks;       /* block applications of the operator            */
+    int fp64_spmm_blocks;  /* FP64 residual applications                    */
+    int restarts;
+    int converged;
+    double max_resid;
+    double seconds;
+} fmx_eigen_stats;
+
+fmx_status fmx_eigen_opts_init(fmx_eigen_opts *opts, int nev);
+
+/* Top-nev eigenpairs of N = D^-1/2 A D^-1/2 for the symmetric binary
+ * matrix A (NA_eigen_Operator, matrix/test/test-eigen.cpp:62-136, solved as
+ * fm::eigen::compute_eigen does, eigensolver.cpp:118-308), with
+ * D^-1/2 := 0 for isolated vertices.  X0 (device, n x block_size, FP32 or
+ * FP64, leading dimension ldx0) is a warm start or NULL for a random one.
+ * evals[nev] and resid[nev] are host arrays (descending), evecs a device
+ * n x nev FP64 array (or NULL). */
+fmx_status fmx_eigen_normalized(fmx_ctx *ctx, const fmx_spm *spm, const fmx_eigen_opts *opts,
+                                const void *X0, int64_t ldx0, fmx_dtype x0_dtype,
+                                double *evals, double *evecs, int64_t ldv, double *resid,
+                                fmx_eigen_stats *stats);
+
 /* ---- seeded generators (device output) ------------------------------------
  * R-MAT edges with the project's SplitMix64 specification (the reference's
  * generator, flash-graph/tools/rmat-gen.cpp:43-48, is a time-seeded Boost
