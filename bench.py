@@ -59,7 +59,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--log2-h", type=int, default=int(os.environ.get("FMX_BENCH_LOG2H", "8")))
+    ap.add_argument("--log2-h", type=int, default=int(os.environ.get("FMX_BENCH_LOG2H", "7")))
     ap.add_argument("--log2-w", type=int, default=15)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-reps", type=int, default=5)

@@ -226,6 +226,12 @@ template fmx_status scan_exclusive<uint32_t>(fmx_ctx *, const uint32_t *, uint32
 template fmx_status scan_exclusive<uint64_t>(fmx_ctx *, const uint64_t *, uint64_t *, uint64_t,
                                              uint64_t *, hipStream_t);
 
+fmx_status scan_u64_exclusive(fmx_ctx *ctx, const uint64_t *in, uint64_t *out, uint64_t n, uint64_t *total,
+                              hipStream_t s)
+{
+    return scan_exclusive<uint64_t>(ctx, in, out, n, total, s);
+}
+
 fmx_status radix_sort_u64(fmx_ctx *ctx, uint64_t *keys, uint64_t *tmp, uint64_t n, int bits,
                           uint64_t **sorted, hipStream_t s)
 {

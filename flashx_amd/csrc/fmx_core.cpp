@@ -468,14 +468,15 @@ fmx_status fmx_spmm(fmx_ctx *ctx, const fmx_spm *cspm, fmx_dtype dtype, int k,
     if (k <= 0) return fail(ctx, FMX_EDIM, "k must be positive");
     if (ldx < k || ldy < k) return fail(ctx, FMX_EDIM, "leading dimension smaller than k");
     const int es = elem_size(dtype);
-    if ((uint64_t)k * es > 32768) return fail(ctx, FMX_EUNSUPPORTED, "dense rows wider than 32 KiB");
+    const int acc = dtype == FMX_F32 ? 8 : es;  // LDS accumulator bytes (FP32 sums in FP64)
+    if ((uint64_t)k * acc > 32768) return fail(ctx, FMX_EUNSUPPORTED, "dense rows wider than 32 KiB");
     if ((!X && spm->ncols) || (!Y && spm->nrows)) return fail(ctx, FMX_EINVAL, "null dense operand");
     if (spm->nrows == 0) return FMX_OK;
     FMX_HIP(ctx, hipSetDevice(ctx->device));
     Plan *plan = nullptr;
-    fmx_status st = get_plan(ctx, spm, band_rows(spm, k, es), &plan);
+    fmx_status st = get_plan(ctx, spm, band_rows(spm, k, acc), &plan);
     if (st) return st;
-    size_t need = (size_t)plan->nslots * plan->hs * k * es;
+    size_t need = (size_t)plan->nslots * plan->hs * k * acc;
     if (need > spm->partial.bytes) {
         if (spm->partial.ptr) FMX_HIP(ctx, hipFree(spm->partial.ptr));
         spm->partial.ptr = nullptr;

@@ -19,6 +19,7 @@
 //     partial slot when a power-law band was split over several items; split
 //     bands are summed slot by slot, in stream order, by spmm_reduce.
 // Each band is owned by exactly one wave, so results do not depend on timing.
+// FP32 data accumulate in an FP64 band and are rounded once on write-out.
 #include <cstdlib>
 
 #include "fmx_internal.h"
@@ -71,6 +72,11 @@ __device__ __forceinline__ void lds_add(T *p, T v)
     __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
+// LDS accumulator type: FP32 data accumulate in FP64 (ds_add_f64 is fast on
+// gfx950 where ds_add_f32 is not, and the sum is then rounded once).
+template <typename T> struct AccOf { using type = T; };
+template <> struct AccOf<float> { using type = double; };
+
 template <typename T, int K>
 struct Lanes {
     static constexpr int S = sizeof(T);
@@ -83,8 +89,8 @@ struct Lanes {
 // Drains queue entries [0, qn) (qn >= 1) into the band.  K > 0: compile-time
 // width.  Loads are unconditional (indices clamped to the last entry) so the
 // P gathers of a batch are all in flight before the first LDS add.
-template <typename T, int K, int P>
-__device__ __forceinline__ void drain(const SpmmArgs &a, T *band, uint32_t S, const uint32_t *qcol,
+template <typename T, int K, int P, typename A>
+__device__ __forceinline__ void drain(const SpmmArgs &a, A *band, uint32_t S, const uint32_t *qcol,
                                       const uint16_t *qrow, uint32_t qn, int lane)
 {
     using C = Lanes<T, K>;
@@ -106,17 +112,17 @@ __device__ __forceinline__ void drain(const SpmmArgs &a, T *band, uint32_t S, co
 #pragma unroll
         for (int pp = 0; pp < P; pp++) {
             if (rr[pp] != 0xFFFFFFFFu) {
-                T *dst = band + rr[pp] * S + q * C::E;
+                A *dst = band + rr[pp] * S + q * C::E;
 #pragma unroll
-                for (int i = 0; i < C::E; i++) lds_add(dst + i, v[pp].v[i]);
+                for (int i = 0; i < C::E; i++) lds_add(dst + i, (A)v[pp].v[i]);
             }
         }
     }
 }
 
 // Runtime width: flat (entry, element) pairs, one element per lane.
-template <typename T, int P>
-__device__ __forceinline__ void drain_generic(const SpmmArgs &a, T *band, uint32_t S, const uint32_t *qcol,
+template <typename T, int P, typename A>
+__device__ __forceinline__ void drain_generic(const SpmmArgs &a, A *band, uint32_t S, const uint32_t *qcol,
                                               const uint16_t *qrow, uint32_t qn, int lane)
 {
     const T *X = (const T *)a.X;
@@ -135,7 +141,7 @@ __device__ __forceinline__ void drain_generic(const SpmmArgs &a, T *band, uint32
         }
 #pragma unroll
         for (int pp = 0; pp < P; pp++)
-            if (dst[pp] != 0xFFFFFFFFu) lds_add(band + dst[pp], v[pp]);
+            if (dst[pp] != 0xFFFFFFFFu) lds_add(band + dst[pp], (A)v[pp]);
     }
 }
 
@@ -151,7 +157,8 @@ __global__ void __launch_bounds__(64 * kWPB) spmm_stream(SpmmArgs a)
     const uint32_t k = K ? K : (uint32_t)a.k;
 
     unsigned char *wbase = smem + wv * a.wave_bytes;
-    T *band = (T *)wbase;
+    using A = typename AccOf<T>::type;
+    A *band = (A *)wbase;
     uint32_t *qcol = (uint32_t *)(wbase + a.band_bytes);
     uint16_t *qrow = (uint16_t *)(qcol + QCAP);
 
@@ -166,7 +173,7 @@ __global__ void __launch_bounds__(64 * kWPB) spmm_stream(SpmmArgs a)
     uint32_t ct = it.dir_begin;
 
     const uint32_t S = a.band_stride;  // padded band row (elements)
-    for (uint32_t i = lane; i < rows * S; i += 64) band[i] = T(0);
+    for (uint32_t i = lane; i < rows * S; i += 64) band[i] = A(0);
 
     const uint16_t *wp = a.img + brow_word;
     const uint32_t pend = begin + nwords;  // nwords >= 1
@@ -252,43 +259,43 @@ __global__ void __launch_bounds__(64 * kWPB) spmm_stream(SpmmArgs a)
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         if (qn) {
-            if (K) drain<T, (K ? K : 1), P>(a, band, S, qcol, qrow, qn, lane);
-            else drain_generic<T, P>(a, band, S, qcol, qrow, qn, lane);
+            if (K) drain<T, (K ? K : 1), P, A>(a, band, S, qcol, qrow, qn, lane);
+            else drain_generic<T, P, A>(a, band, S, qcol, qrow, qn, lane);
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         p = pn;
     }
 
-    // write the band
-    T *dst;
-    uint64_t ld;
+    // write the band (rounded once to T), or the partial slot in A
     if (slot == kNone) {
-        dst = (T *)a.Y + row0 * (uint64_t)a.ldy;
-        ld = (uint64_t)a.ldy;
+        T *dst = (T *)a.Y + row0 * (uint64_t)a.ldy;
+        for (uint32_t i = lane; i < rows * k; i += 64) {
+            const uint32_t r = i / k, cc = i - r * k;
+            dst[r * (uint64_t)a.ldy + cc] = (T)band[r * S + cc];
+        }
     } else {
-        dst = (T *)a.partial + (uint64_t)slot * a.hs * k;
-        ld = k;
-    }
-    for (uint32_t i = lane; i < rows * k; i += 64) {
-        uint32_t r = i / k, cc = i - r * k;
-        dst[r * ld + cc] = band[r * S + cc];
+        A *dst = (A *)a.partial + (uint64_t)slot * a.hs * k;
+        for (uint32_t i = lane; i < rows * k; i += 64) {
+            const uint32_t r = i / k, cc = i - r * k;
+            dst[i] = band[r * S + cc];
+        }
     }
 }
 
 // Y rows of a split band = partial slots summed in stream order.
-template <typename T>
-__global__ void __launch_bounds__(256) spmm_reduce(const Split *splits, const T *partial,
+template <typename T, typename A>
+__global__ void __launch_bounds__(256) spmm_reduce(const Split *splits, const A *partial,
                                                     uint32_t hs, uint32_t k, T *Y, int64_t ldy)
 {
     const Split sp = splits[blockIdx.x];
     const uint32_t tot = sp.rows * k;
     for (uint32_t i = threadIdx.x; i < tot; i += blockDim.x) {
-        T acc = partial[(uint64_t)sp.slot_begin * hs * k + i];
+        A acc = partial[(uint64_t)sp.slot_begin * hs * k + i];
         for (uint32_t s = 1; s < sp.nslots; s++)
             acc += partial[(uint64_t)(sp.slot_begin + s) * hs * k + i];
         uint32_t r = i / k, c = i - r * k;
-        Y[(sp.row0 + r) * (uint64_t)ldy + c] = acc;
+        Y[(sp.row0 + r) * (uint64_t)ldy + c] = (T)acc;
     }
 }
 
@@ -347,7 +354,8 @@ fmx_status launch_t(fmx_ctx *ctx, const fmx_spm *spm, const Plan *plan, int k,
     a.hs = plan->hs;
     // pad band rows by one element to spread the LDS adds over the banks
     a.band_stride = (uint32_t)k + ((k >= 4 && getenv("FMX_NO_PAD") == nullptr) ? 1u : 0u);
-    a.band_bytes = (uint32_t)(((uint64_t)plan->hs * a.band_stride * sizeof(T) + 15) & ~15ull);
+    using A = typename AccOf<T>::type;
+    a.band_bytes = (uint32_t)(((uint64_t)plan->hs * a.band_stride * sizeof(A) + 15) & ~15ull);
     a.wave_bytes = a.band_bytes + 64 * U * 6;
     a.wave_bytes = (a.wave_bytes + 15) & ~15u;
     a.log2w = spm->log2w;
@@ -371,8 +379,8 @@ fmx_status launch_t(fmx_ctx *ctx, const fmx_spm *spm, const Plan *plan, int k,
     }
     if (st) return st;
     if (plan->nsplits) {
-        hipLaunchKernelGGL(spmm_reduce<T>, dim3(plan->nsplits), dim3(256), 0, stream,
-                           plan->d_splits, (const T *)partial, plan->hs, (uint32_t)k, (T *)Y, ldy);
+        hipLaunchKernelGGL((spmm_reduce<T, A>), dim3(plan->nsplits), dim3(256), 0, stream,
+                           plan->d_splits, (const A *)partial, plan->hs, (uint32_t)k, (T *)Y, ldy);
         FMX_HIP(ctx, hipGetLastError());
     }
     return FMX_OK;

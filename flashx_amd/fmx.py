@@ -84,6 +84,21 @@ SIGNATURES = {
     "fmx_eigen_normalized": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(fmx_eigen_opts), _vp, _i64,
                                             ctypes.c_int, _vp, _vp, _i64, _vp,
                                             ctypes.POINTER(fmx_eigen_stats)]),
+    "fmx_gnn_param_count": (_u64, []),
+    "fmx_gnn_acts_floats": (_u64, [_u64]),
+    "fmx_gnn_work_floats": (_u64, [_u64]),
+    "fmx_gnn_features": (ctypes.c_int, [_vp, _vp, _u64, _vp, _vp, _vp]),
+    "fmx_gnn_forward": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int, _vp]),
+    "fmx_gnn_backward": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int,
+                                        _vp]),
+    "fmx_subspace_loss": (ctypes.c_int, [_vp, _u64, ctypes.c_int, _vp, _vp,
+                                         ctypes.POINTER(ctypes.c_double), _vp, _vp]),
+    "fmx_adam": (ctypes.c_int, [_vp, _u64, _vp, _vp, _vp, _vp, ctypes.c_float, ctypes.c_float,
+                                ctypes.c_float, ctypes.c_float, ctypes.c_int, _vp]),
+    "fmx_dcsbm_params": (ctypes.c_int, [_u64, ctypes.c_int, ctypes.c_double, ctypes.c_double, _vp,
+                                        _vp]),
+    "fmx_dcsbm_edges": (ctypes.c_int, [_vp, _u64, ctypes.c_int, _vp, _vp, ctypes.c_uint32, _u64, _u64,
+                                       _vp, _vp, _vp]),
     "fmx_rmat_edges": (ctypes.c_int, [_vp, ctypes.c_int, _u64, _u64, ctypes.c_uint32,
                                       ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, _vp,
                                       _vp, _vp]),
@@ -385,6 +400,30 @@ def compute_eigen(spm, nev=16, block_size=16, num_blocks=6, tol=1e-6, max_restar
               "fmx_eigen_normalized")
     stats = {f: getattr(st, f) for f, _ in fmx_eigen_stats._fields_}
     return evals, vecs, resid, stats
+
+
+def dcsbm_params(n, nblocks=32, gamma=2.5, size_decay=1.06):
+    """Block boundaries and degree-weight table of the DC-SBM generator."""
+    bstart = np.zeros(nblocks + 1, np.uint32)
+    wtab = np.zeros(1024, np.uint32)
+    st = lib().fmx_dcsbm_params(n, nblocks, gamma, size_decay, bstart.ctypes.data, wtab.ctypes.data)
+    if st:
+        raise FmxError(st, "fmx_dcsbm_params: %s" % lib().fmx_strerror(st).decode())
+    return bstart, wtab
+
+
+def dcsbm_edges(n, seed, nblocks=32, ratio=20, gamma=2.5, mean_degree=16, size_decay=1.06, ctx=None):
+    """Seeded DC-SBM edge list (m = n * mean_degree / 2) on the device."""
+    torch = _torch()
+    ctx = ctx or default_context()
+    bstart, wtab = dcsbm_params(n, nblocks, gamma, size_decay)
+    m = int(n * mean_degree // 2)
+    src = torch.empty(m, dtype=torch.int32, device="cuda:%d" % ctx.device)
+    dst = torch.empty(m, dtype=torch.int32, device="cuda:%d" % ctx.device)
+    ctx.check(lib().fmx_dcsbm_edges(ctx.handle, n, nblocks, bstart.ctypes.data, wtab.ctypes.data, ratio, m,
+                                    seed, src.data_ptr(), dst.data_ptr(), ctx.stream_ptr()),
+              "fmx_dcsbm_edges")
+    return src, dst
 
 
 def rmat_edges(scale, m, seed, a=0.57, b=0.19, c=0.19, permute=True, ctx=None):

@@ -1,0 +1,271 @@
+/*
+ * fm/sparse_matrix.h -- C++ drop-in for FlashMatrix's sparse-matrix SpMM API.
+ *
+ * Same class and method names as the reference (matrix/sparse_matrix.h:
+ * 475-651, matrix/matrix_header.h:42-91, matrix/fm_utils.h:37-39), over the
+ * C ABI of include/fmx.h.  Dense operands live in HBM (fm::dense_matrix is a
+ * row-major device matrix here); multiply() is synchronous like the
+ * reference's (matrix/sparse_matrix.cpp:465).
+ *
+ * Error behaviour follows the reference: recoverable problems return false
+ * or a null pointer after logging (matrix/sparse_matrix.cpp:715-718,
+ * 755-766); a malformed image throws fm::wrong_format
+ * (matrix/matrix_header.cpp:52-60).
+ *
+ * Header-only; link with libfmx.so and the HIP runtime.
+ */
+#ifndef FM_SPARSE_MATRIX_H
+#define FM_SPARSE_MATRIX_H
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <limits>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../fmx.h"
+
+namespace fm {
+
+class wrong_format : public std::runtime_error {
+public:
+    explicit wrong_format(const std::string &msg) : std::runtime_error(msg) {}
+};
+
+namespace detail {
+inline fmx_ctx *&global_ctx()
+{
+    static fmx_ctx *ctx = nullptr;
+    return ctx;
+}
+inline void log_error(const std::string &what)
+{
+    fprintf(stderr, "[fm error] %s\n", what.c_str());
+}
+inline bool check(fmx_status st, const char *what)
+{
+    if (st == FMX_OK) return true;
+    std::string msg = std::string(what) + ": " + fmx_strerror(st);
+    if (global_ctx()) msg += std::string(" (") + fmx_ctx_last_error(global_ctx()) + ")";
+    if (st == FMX_EFORMAT) throw wrong_format(msg);
+    log_error(msg);
+    return false;
+}
+}  // namespace detail
+
+/* fm::init_flash_matrix / destroy_flash_matrix (matrix/sparse_matrix.cpp:835-872). */
+inline bool init_flash_matrix(int device = 0)
+{
+    if (detail::global_ctx()) return true;
+    return detail::check(fmx_ctx_create(device, &detail::global_ctx()), "init_flash_matrix");
+}
+
+inline void destroy_flash_matrix()
+{
+    if (detail::global_ctx()) fmx_ctx_destroy(detail::global_ctx());
+    detail::global_ctx() = nullptr;
+}
+
+/* block_2d_size (matrix/matrix_header.h:45-91). */
+class block_2d_size {
+    uint16_t nrow_log = 0, ncol_log = 0;
+
+public:
+    block_2d_size() {}
+    block_2d_size(size_t num_rows, size_t num_cols)
+    {
+        while ((1ull << nrow_log) < num_rows) nrow_log++;
+        while ((1ull << ncol_log) < num_cols) ncol_log++;
+        if ((1ull << nrow_log) != num_rows || (1ull << ncol_log) != num_cols || num_rows > 32768 ||
+            num_cols > 32768)
+            throw wrong_format("block sizes must be powers of two <= 32768");
+    }
+    size_t get_nrow_log() const { return nrow_log; }
+    size_t get_ncol_log() const { return ncol_log; }
+    size_t get_nrow_mask() const { return get_num_rows() - 1; }
+    size_t get_ncol_mask() const { return get_num_cols() - 1; }
+    size_t get_num_rows() const { return size_t(1) << nrow_log; }
+    size_t get_num_cols() const { return size_t(1) << ncol_log; }
+    size_t cal_num_block_rows(size_t tot) const { return (tot + get_num_rows() - 1) / get_num_rows(); }
+};
+
+/* A row-major dense matrix in HBM (the in-memory dense_matrix of
+ * matrix/dense_matrix.h as the SpMM sees it). */
+class dense_matrix {
+    size_t nrow_, ncol_;
+    int64_t ld_;
+    fmx_dtype type_;
+    void *data_;
+    bool owned_;
+
+    dense_matrix(size_t nrow, size_t ncol, int64_t ld, fmx_dtype t, void *d, bool owned)
+        : nrow_(nrow), ncol_(ncol), ld_(ld), type_(t), data_(d), owned_(owned) {}
+
+public:
+    typedef std::shared_ptr<dense_matrix> ptr;
+
+    static size_t elem_size(fmx_dtype t) { return (t == FMX_I32 || t == FMX_F32) ? 4 : 8; }
+
+    static ptr create(size_t nrow, size_t ncol, fmx_dtype t)
+    {
+        void *d = nullptr;
+        size_t bytes = nrow * ncol * elem_size(t);
+        if (hipMalloc(&d, bytes ? bytes : 1) != hipSuccess) {
+            detail::log_error("dense_matrix::create: out of device memory");
+            return ptr();
+        }
+        return ptr(new dense_matrix(nrow, ncol, (int64_t)ncol, t, d, true));
+    }
+    /* Wrap caller-owned device memory (not freed). */
+    static ptr wrap(void *dev, size_t nrow, size_t ncol, int64_t ld, fmx_dtype t)
+    {
+        return ptr(new dense_matrix(nrow, ncol, ld, t, dev, false));
+    }
+    static ptr create_from_host(const void *host, size_t nrow, size_t ncol, fmx_dtype t)
+    {
+        ptr m = create(nrow, ncol, t);
+        if (m && hipMemcpy(m->data_, host, nrow * ncol * elem_size(t), hipMemcpyHostToDevice) != hipSuccess)
+            return ptr();
+        return m;
+    }
+    ~dense_matrix()
+    {
+        if (owned_ && data_) (void)hipFree(data_);
+    }
+    bool copy_to_host(void *host) const
+    {
+        return hipMemcpy2D(host, ncol_ * elem_size(type_), data_, ld_ * elem_size(type_),
+                           ncol_ * elem_size(type_), nrow_, hipMemcpyDeviceToHost) == hipSuccess;
+    }
+    size_t get_num_rows() const { return nrow_; }
+    size_t get_num_cols() const { return ncol_; }
+    int64_t get_ld() const { return ld_; }
+    fmx_dtype get_type() const { return type_; }
+    void *get_raw_arr() { return data_; }
+    const void *get_raw_arr() const { return data_; }
+};
+
+/* fm::sparse_matrix (matrix/sparse_matrix.h:475-651) for a 2D-tiled image. */
+class sparse_matrix {
+    std::shared_ptr<fmx_spm> spm_;
+    fmx_spm_info info_{};
+    bool symmetric_ = true;
+    std::shared_ptr<sparse_matrix> transpose_;  // asymmetric matrices carry A^T
+
+    explicit sparse_matrix(fmx_spm *s, bool sym) : spm_(s, [](fmx_spm *p) { fmx_spm_destroy(p); }),
+                                                   symmetric_(sym)
+    {
+        fmx_spm_info_get(s, &info_);
+    }
+
+public:
+    typedef std::shared_ptr<sparse_matrix> ptr;
+
+    /* sparse_matrix::create(SpM_2d_index::ptr, SpM_2d_storage::ptr)
+     * (matrix/sparse_matrix.h:522): the image (header + block rows) and the
+     * block-row offsets of its index. */
+    static ptr create(const std::vector<char> &image, const std::vector<int64_t> &offs)
+    {
+        if (!init_flash_matrix()) return ptr();
+        fmx_spm *s = nullptr;
+        if (!detail::check(fmx_spm_upload(detail::global_ctx(), image.data(), image.size(), offs.data(),
+                                          offs.size() - 1, &s),
+                           "sparse_matrix::create"))
+            return ptr();
+        return ptr(new sparse_matrix(s, true));
+    }
+
+    /* The asymmetric form create(index, storage, t_index, t_storage)
+     * (matrix/sparse_matrix.h:527-528). */
+    static ptr create(const std::vector<char> &image, const std::vector<int64_t> &offs,
+                      const std::vector<char> &t_image, const std::vector<int64_t> &t_offs)
+    {
+        ptr a = create(image, offs), t = create(t_image, t_offs);
+        if (!a || !t) return ptr();
+        a->symmetric_ = t->symmetric_ = false;
+        a->transpose_ = t;
+        return a;
+    }
+
+    /* fm::create_2d_matrix (matrix/fm_utils.h:37-39, fm_utils.cpp:726-841)
+     * from a device edge list; builds A^T as well unless is_sym. */
+    static ptr create_2d_matrix(const uint32_t *src, const uint32_t *dst, size_t m,
+                                const block_2d_size &bs, bool is_sym, uint64_t n = 0,
+                                uint32_t flags = FMX_BUILD_DEDUP)
+    {
+        if (!init_flash_matrix()) return ptr();
+        fmx_spm *s = nullptr;
+        if (!detail::check(fmx_spm_build(detail::global_ctx(), src, dst, m, n, (uint32_t)bs.get_nrow_log(),
+                                         (uint32_t)bs.get_ncol_log(), flags, &s),
+                           "create_2d_matrix"))
+            return ptr();
+        ptr a(new sparse_matrix(s, is_sym));
+        if (!is_sym) {
+            fmx_spm *t = nullptr;
+            if (!detail::check(fmx_spm_build(detail::global_ctx(), dst, src, m, a->info_.nrows,
+                                             (uint32_t)bs.get_nrow_log(), (uint32_t)bs.get_ncol_log(),
+                                             flags, &t),
+                               "create_2d_matrix (transpose)"))
+                return ptr();
+            a->transpose_ = ptr(new sparse_matrix(t, false));
+        }
+        return a;
+    }
+
+    size_t get_num_rows() const { return info_.nrows; }
+    size_t get_num_cols() const { return info_.ncols; }
+    size_t get_nnz() const { return info_.nnz; }
+    bool is_symmetric() const { return symmetric_; }
+    size_t get_entry_size() const { return info_.entry_size; }
+    block_2d_size get_block_size() const { return block_2d_size(info_.block_h, info_.block_w); }
+    const fmx_spm *handle() const { return spm_.get(); }
+
+    /* sparse_matrix::transpose (matrix/sparse_matrix.h:630): a symmetric
+     * matrix is its own transpose. */
+    ptr transpose() const
+    {
+        if (symmetric_) return ptr(new sparse_matrix(*this));  // shares the image
+        return transpose_;
+    }
+
+    /* multiply(in, out) (matrix/sparse_matrix.h:636-637): out must be
+     * allocated; false on a type or dimension mismatch. */
+    bool multiply(const dense_matrix &in, dense_matrix &out) const
+    {
+        if (in.get_num_rows() != get_num_cols() || out.get_num_rows() != get_num_rows() ||
+            in.get_num_cols() != out.get_num_cols()) {
+            detail::log_error("the matrix size doesn't match");
+            return false;
+        }
+        if (in.get_type() != out.get_type()) {
+            detail::log_error("wrong matrix type in spmm creator");
+            return false;
+        }
+        if (!detail::check(fmx_spmm(detail::global_ctx(), spm_.get(), in.get_type(), (int)in.get_num_cols(),
+                                    in.get_raw_arr(), in.get_ld(), out.get_raw_arr(), out.get_ld(), nullptr),
+                           "sparse_matrix::multiply"))
+            return false;
+        return hipDeviceSynchronize() == hipSuccess;
+    }
+
+    /* multiply(dense_matrix::ptr, mem_size) (matrix/sparse_matrix.h:643-644):
+     * returns a new row-major Y = A * X, or null. */
+    dense_matrix::ptr multiply(dense_matrix::ptr right,
+                               size_t mem_size = std::numeric_limits<size_t>::max()) const
+    {
+        (void)mem_size;  // everything is resident in HBM
+        if (!right) return dense_matrix::ptr();
+        dense_matrix::ptr out = dense_matrix::create(get_num_rows(), right->get_num_cols(), right->get_type());
+        if (!out || !multiply(*right, *out)) return dense_matrix::ptr();
+        return out;
+    }
+
+};
+
+}  // namespace fm
+
+#endif

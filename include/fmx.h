@@ -204,6 +204,46 @@ fmx_status fmx_rmat_edges(fmx_ctx *ctx, int scale, uint64_t m, uint64_t seed,
                           int permute, uint32_t *src, uint32_t *dst,
                           void *stream);
 
+/* Degree-corrected stochastic block model (no reference counterpart; the
+ * closest is the plain SBM of flash-graph/utils/sbm.cpp:117-242).
+ * fmx_dcsbm_params fills bstart[nblocks+1] (contiguous blocks, sizes
+ * proportional to size_decay^-b) and wtab[1024] (quantiles of the degree
+ * weight theta = (1-u)^(-1/(gamma-1)), capped at sqrt(n), in 1/1024 units)
+ * on the host.  fmx_dcsbm_edges draws m edges on the device: u in proportion
+ * to theta, then an intra-block partner with odds ratio:1 against an
+ * inter-block one, both in proportion to theta; ids are relabelled with the
+ * R-MAT permutation.  n must be a power of two. */
+fmx_status fmx_dcsbm_params(uint64_t n, int nblocks, double gamma, double size_decay,
+                            uint32_t *bstart, uint32_t *wtab);
+fmx_status fmx_dcsbm_edges(fmx_ctx *ctx, uint64_t n, int nblocks, const uint32_t *bstart,
+                           const uint32_t *wtab, uint32_t ratio, uint64_t m, uint64_t seed,
+                           uint32_t *src, uint32_t *dst, void *stream);
+
+/* ---- learned warm start (no reference counterpart) ----------------------
+ * A 3-layer message-passing network over N = D^-1/2 A D^-1/2 predicting the
+ * eigensolver's starting block: H_{l+1} = relu(H_l Ws_l + N H_l Wn_l + b_l),
+ * Y = H_3 Wo; inputs X0 = [log(1+deg), 8 uniform features, 7 zeros] (n x 16),
+ * hidden 64, output 16.  Parameters are one flat FP32 device array of
+ * fmx_gnn_param_count() values; activations and work buffers are device
+ * arrays of fmx_gnn_acts_floats(n) / fmx_gnn_work_floats(n) floats.
+ * mode 0: BF16 MFMA products with FP32 accumulation; mode 1: FP32. */
+uint64_t fmx_gnn_param_count(void);
+uint64_t fmx_gnn_acts_floats(uint64_t n);
+uint64_t fmx_gnn_work_floats(uint64_t n);
+fmx_status fmx_gnn_features(fmx_ctx *ctx, const fmx_spm *spm, uint64_t seed, float *X0, double *dinv,
+                            void *stream);
+fmx_status fmx_gnn_forward(fmx_ctx *ctx, const fmx_spm *spm, const double *dinv, const float *params,
+                           const float *X0, float *acts, float *Y, float *work, int mode, void *stream);
+fmx_status fmx_gnn_backward(fmx_ctx *ctx, const fmx_spm *spm, const double *dinv, const float *params,
+                            const float *X0, const float *acts, const float *dY, float *grads,
+                            float *work, int mode, void *stream);
+/* loss = k - tr((Y^T Y)^-1 (U^T Y)^T (U^T Y)) for orthonormal U (n x k), and
+ * dY = dloss/dY (device, may be NULL). */
+fmx_status fmx_subspace_loss(fmx_ctx *ctx, uint64_t n, int k, const float *Y, const float *U,
+                             double *loss, float *dY, void *stream);
+fmx_status fmx_adam(fmx_ctx *ctx, uint64_t np, float *params, const float *grads, float *m, float *v,
+                    float lr, float beta1, float beta2, float eps, int step, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

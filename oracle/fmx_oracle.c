@@ -104,6 +104,69 @@ void orc_rmat_edges(int scale, uint64_t m, uint64_t seed,
 void orc_free(void *p) { free(p); }
 
 /* ------------------------------------------------------------------------ */
+/* Degree-corrected stochastic block model (the project's own seeded spec;   */
+/* no reference counterpart: its nearest relative, flash-graph/utils/sbm.cpp */
+/* :117-242, is a plain SBM).  Integer arithmetic throughout so that the     */
+/* device generator reproduces it bit for bit:                               */
+/*   weight w_v = wtab[splitmix64(seed^W, v) >> 54] (a 1024-quantile table   */
+/*   of a power law computed by the caller), C = exclusive prefix of w,      */
+/*   blocks are contiguous id ranges bstart[0..B];                           */
+/*   edge e: u = vertex of rank (r1 % Vtot) in C, b = block(u);              */
+/*   intra-block iff (r2 >> 40) * den < (num << 24) with num = ratio*vb,     */
+/*   den = num + vt - vb (vb, vt = volumes >> 8); the partner is drawn in    */
+/*   proportion to w inside b, or outside b by skipping b's volume range;    */
+/*   finally both ends go through the R-MAT relabelling orc_rmat_perm.       */
+/* ------------------------------------------------------------------------ */
+#define SBM_SALT_W 0x73626D5745494748ULL
+#define SBM_SALT_E 0x73626D4544474553ULL
+
+static uint64_t sbm_find(const uint64_t *C, uint64_t lo, uint64_t hi, uint64_t t)
+{
+    /* largest v in [lo, hi) with C[v] <= t */
+    while (hi - lo > 1) {
+        uint64_t mid = lo + (hi - lo) / 2;
+        if (C[mid] <= t) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+int orc_dcsbm_edges(uint64_t n, int scale, int nblocks, const uint32_t *bstart,
+                    const uint32_t *wtab, uint32_t ratio, uint64_t m, uint64_t seed,
+                    uint32_t *src, uint32_t *dst)
+{
+    uint64_t *C = (uint64_t *)malloc((n + 1) * sizeof(uint64_t));
+    if (!C) return ORC_ENOMEM;
+    C[0] = 0;
+    for (uint64_t v = 0; v < n; v++)
+        C[v + 1] = C[v] + wtab[orc_splitmix64(seed ^ SBM_SALT_W, v) >> 54];
+    const uint64_t vtot = C[n];
+    for (uint64_t e = 0; e < m; e++) {
+        uint64_t r1 = orc_splitmix64(seed ^ SBM_SALT_E, 3 * e);
+        uint64_t r2 = orc_splitmix64(seed ^ SBM_SALT_E, 3 * e + 1);
+        uint64_t r3 = orc_splitmix64(seed ^ SBM_SALT_E, 3 * e + 2);
+        uint64_t u = sbm_find(C, 0, n, r1 % vtot);
+        int lo = 0, hi = nblocks;
+        while (hi - lo > 1) { int mid = (lo + hi) / 2; if (bstart[mid] <= u) lo = mid; else hi = mid; }
+        const int b = lo;
+        const uint64_t c0 = C[bstart[b]], vb_full = C[bstart[b + 1]] - c0;
+        const uint64_t vb = vb_full >> 8, vt = vtot >> 8;
+        const uint64_t num = (uint64_t)ratio * vb, den = num + vt - vb;
+        uint64_t v;
+        if ((r2 >> 40) * den < (num << 24)) {
+            v = sbm_find(C, bstart[b], bstart[b + 1], c0 + r3 % vb_full);
+        } else {
+            uint64_t t = r3 % (vtot - vb_full);
+            if (t >= c0) t += vb_full;
+            v = sbm_find(C, 0, n, t);
+        }
+        src[e] = orc_rmat_perm((uint32_t)u, scale, seed);
+        dst[e] = orc_rmat_perm((uint32_t)v, scale, seed);
+    }
+    free(C);
+    return ORC_OK;
+}
+
+/* ------------------------------------------------------------------------ */
 /* Edge normalisation.                                                       */
 /* ------------------------------------------------------------------------ */
 static int bits_for(uint64_t n)
@@ -850,4 +913,187 @@ void orc_colnorm_f64(const double *A, int64_t lda, uint64_t n, int k, double *no
     for (uint64_t r = 0; r < n; r++)
         for (int j = 0; j < k; j++) norms[j] += A[r * lda + j] * A[r * lda + j];
     for (int j = 0; j < k; j++) norms[j] = sqrt(norms[j]);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Learned warm start: the 3-layer message-passing network, FP64.  No        */
+/* reference counterpart (the reference starts from a random block,          */
+/* FM_MultiVector.h:164-168); this is the checker of the device kernels.     */
+/*   H0 = X0 (n x 16); for l = 0..2:                                         */
+/*     Agg_l = D^-1/2 A D^-1/2 H_l,  Z_l = H_l Ws_l + Agg_l Wn_l + b_l,      */
+/*     H_{l+1} = relu(Z_l);                                                  */
+/*   Y = H_3 Wo (n x 16).                                                    */
+/* Parameters, flat: per layer Ws (din x 64), Wn (din x 64), b (64); then    */
+/* Wo (64 x 16); din = 16, 64, 64.                                           */
+/* Loss (rotation-invariant subspace loss against orthonormal U):            */
+/*   L = k - tr(G^-1 B), G = Y^T Y, B = C^T C, C = U^T Y;                    */
+/*   dL/dY = -2 (U C G^-1 - Y G^-1 B G^-1).                                  */
+/* ------------------------------------------------------------------------ */
+#define GNN_IN 16
+#define GNN_H 64
+#define GNN_OUT 16
+
+static int gnn_din(int l) { return l == 0 ? GNN_IN : GNN_H; }
+
+uint64_t orc_gnn_param_count(void)
+{
+    uint64_t c = 0;
+    for (int l = 0; l < 3; l++) c += 2ull * gnn_din(l) * GNN_H + GNN_H;
+    return c + (uint64_t)GNN_H * GNN_OUT;
+}
+
+static void gnn_agg(uint64_t n, const int64_t *rp, const int64_t *ci, const double *dinv,
+                    const double *H, int d, double *out)
+{
+    for (uint64_t r = 0; r < n; r++) {
+        for (int j = 0; j < d; j++) out[r * d + j] = 0.0;
+        for (int64_t e = rp[r]; e < rp[r + 1]; e++) {
+            const int64_t c = ci[e];
+            const double w = dinv[r] * dinv[c];
+            for (int j = 0; j < d; j++) out[r * d + j] += w * H[c * d + j];
+        }
+    }
+}
+
+/* small dense inverse (Gauss-Jordan with partial pivoting), k <= 64 */
+static int inv_small(const double *A, int k, double *Ai)
+{
+    double M[64 * 128];
+    for (int i = 0; i < k; i++)
+        for (int j = 0; j < 2 * k; j++) M[i * 2 * k + j] = j < k ? A[i * k + j] : (j - k == i ? 1.0 : 0.0);
+    for (int c = 0; c < k; c++) {
+        int p = c;
+        for (int r = c + 1; r < k; r++) if (fabs(M[r * 2 * k + c]) > fabs(M[p * 2 * k + c])) p = r;
+        if (M[p * 2 * k + c] == 0.0) return ORC_EINVAL;
+        for (int j = 0; j < 2 * k; j++) { double t = M[c * 2 * k + j]; M[c * 2 * k + j] = M[p * 2 * k + j]; M[p * 2 * k + j] = t; }
+        const double d = M[c * 2 * k + c];
+        for (int j = 0; j < 2 * k; j++) M[c * 2 * k + j] /= d;
+        for (int r = 0; r < k; r++) {
+            if (r == c) continue;
+            const double f = M[r * 2 * k + c];
+            if (f != 0.0) for (int j = 0; j < 2 * k; j++) M[r * 2 * k + j] -= f * M[c * 2 * k + j];
+        }
+    }
+    for (int i = 0; i < k; i++) for (int j = 0; j < k; j++) Ai[i * k + j] = M[i * 2 * k + k + j];
+    return ORC_OK;
+}
+
+int orc_subspace_loss(uint64_t n, int k, const double *Y, const double *U, double *loss, double *dY)
+{
+    if (k > 64) return ORC_EINVAL;
+    double G[64 * 64], C[64 * 64], B[64 * 64], Gi[64 * 64], M1[64 * 64], M2[64 * 64], T[64 * 64];
+    orc_gram_f64(Y, k, Y, k, n, k, k, G);
+    orc_gram_f64(U, k, Y, k, n, k, k, C);
+    for (int i = 0; i < k; i++) for (int j = 0; j < k; j++) {
+        double s = 0; for (int t = 0; t < k; t++) s += C[t * k + i] * C[t * k + j];
+        B[i * k + j] = s;
+    }
+    if (inv_small(G, k, Gi)) return ORC_EINVAL;
+    double tr = 0;
+    for (int i = 0; i < k; i++) for (int j = 0; j < k; j++) tr += Gi[i * k + j] * B[j * k + i];
+    *loss = k - tr;
+    /* M1 = C Gi ; M2 = Gi B Gi */
+    for (int i = 0; i < k; i++) for (int j = 0; j < k; j++) {
+        double s = 0, t2 = 0;
+        for (int t = 0; t < k; t++) { s += C[i * k + t] * Gi[t * k + j]; t2 += Gi[i * k + t] * B[t * k + j]; }
+        M1[i * k + j] = s; T[i * k + j] = t2;
+    }
+    for (int i = 0; i < k; i++) for (int j = 0; j < k; j++) {
+        double s = 0; for (int t = 0; t < k; t++) s += T[i * k + t] * Gi[t * k + j];
+        M2[i * k + j] = s;
+    }
+    if (dY)
+        for (uint64_t r = 0; r < n; r++)
+            for (int j = 0; j < k; j++) {
+                double a = 0, b = 0;
+                for (int t = 0; t < k; t++) { a += U[r * k + t] * M1[t * k + j]; b += Y[r * k + t] * M2[t * k + j]; }
+                dY[r * k + j] = -2.0 * (a - b);
+            }
+    return ORC_OK;
+}
+
+int orc_gnn_loss_grad(uint64_t n, const int64_t *rp, const int64_t *ci, const double *dinv,
+                      const double *X0, const double *U, const double *P, double *loss,
+                      double *grad, double *Yout)
+{
+    double *H[4], *Agg[3], *Z[3];
+    H[0] = (double *)X0;
+    for (int l = 0; l < 3; l++) {
+        Agg[l] = (double *)malloc(n * gnn_din(l) * sizeof(double));
+        Z[l] = (double *)malloc(n * GNN_H * sizeof(double));
+        H[l + 1] = (double *)malloc(n * GNN_H * sizeof(double));
+    }
+    double *Y = (double *)malloc(n * GNN_OUT * sizeof(double));
+    double *dY = (double *)malloc(n * GNN_OUT * sizeof(double));
+    double *dH = (double *)malloc(n * GNN_H * sizeof(double));
+    double *dZ = (double *)malloc(n * GNN_H * sizeof(double));
+    double *T = (double *)malloc(n * GNN_H * sizeof(double));
+    double *T2 = (double *)malloc(n * GNN_H * sizeof(double));
+    const double *Wp[3][3];
+    uint64_t off = 0, goff[3];
+    for (int l = 0; l < 3; l++) {
+        const int d = gnn_din(l);
+        goff[l] = off;
+        Wp[l][0] = P + off; off += (uint64_t)d * GNN_H;
+        Wp[l][1] = P + off; off += (uint64_t)d * GNN_H;
+        Wp[l][2] = P + off; off += GNN_H;
+    }
+    const double *Wo = P + off;
+    const uint64_t wo_off = off;
+    for (int l = 0; l < 3; l++) {
+        const int d = gnn_din(l);
+        gnn_agg(n, rp, ci, dinv, H[l], d, Agg[l]);
+        for (uint64_t r = 0; r < n; r++)
+            for (int j = 0; j < GNN_H; j++) {
+                double s = Wp[l][2][j];
+                for (int t = 0; t < d; t++)
+                    s += H[l][r * d + t] * Wp[l][0][t * GNN_H + j] + Agg[l][r * d + t] * Wp[l][1][t * GNN_H + j];
+                Z[l][r * GNN_H + j] = s;
+                H[l + 1][r * GNN_H + j] = s > 0 ? s : 0;
+            }
+    }
+    for (uint64_t r = 0; r < n; r++)
+        for (int j = 0; j < GNN_OUT; j++) {
+            double s = 0;
+            for (int t = 0; t < GNN_H; t++) s += H[3][r * GNN_H + t] * Wo[t * GNN_OUT + j];
+            Y[r * GNN_OUT + j] = s;
+        }
+    if (Yout) memcpy(Yout, Y, n * GNN_OUT * sizeof(double));
+    int rc = orc_subspace_loss(n, GNN_OUT, Y, U, loss, dY);
+    if (rc == ORC_OK && grad) {
+        /* dWo = H3^T dY ; dH3 = dY Wo^T */
+        orc_gram_f64(H[3], GNN_H, dY, GNN_OUT, n, GNN_H, GNN_OUT, grad + wo_off);
+        for (uint64_t r = 0; r < n; r++)
+            for (int t = 0; t < GNN_H; t++) {
+                double s = 0;
+                for (int j = 0; j < GNN_OUT; j++) s += dY[r * GNN_OUT + j] * Wo[t * GNN_OUT + j];
+                dH[r * GNN_H + t] = s;
+            }
+        for (int l = 2; l >= 0; l--) {
+            const int d = gnn_din(l);
+            for (uint64_t e = 0; e < n * GNN_H; e++) dZ[e] = Z[l][e] > 0 ? dH[e] : 0.0;
+            orc_gram_f64(H[l], d, dZ, GNN_H, n, d, GNN_H, grad + goff[l]);
+            orc_gram_f64(Agg[l], d, dZ, GNN_H, n, d, GNN_H, grad + goff[l] + (uint64_t)d * GNN_H);
+            double *db = grad + goff[l] + 2ull * d * GNN_H;
+            for (int j = 0; j < GNN_H; j++) db[j] = 0;
+            for (uint64_t r = 0; r < n; r++) for (int j = 0; j < GNN_H; j++) db[j] += dZ[r * GNN_H + j];
+            if (l == 0) break;
+            /* dH_l = dZ Ws^T + Ahat (dZ Wn^T) */
+            for (uint64_t r = 0; r < n; r++)
+                for (int t = 0; t < d; t++) {
+                    double a = 0, b = 0;
+                    for (int j = 0; j < GNN_H; j++) {
+                        a += dZ[r * GNN_H + j] * Wp[l][0][t * GNN_H + j];
+                        b += dZ[r * GNN_H + j] * Wp[l][1][t * GNN_H + j];
+                    }
+                    T[r * d + t] = a;
+                    T2[r * d + t] = b;
+                }
+            gnn_agg(n, rp, ci, dinv, T2, d, dH);
+            for (uint64_t e = 0; e < n * (uint64_t)d; e++) dH[e] += T[e];
+        }
+    }
+    for (int l = 0; l < 3; l++) { free(Agg[l]); free(Z[l]); free(H[l + 1]); }
+    free(Y); free(dY); free(dH); free(dZ); free(T); free(T2);
+    return rc;
 }

@@ -48,6 +48,11 @@ void orc_rmat_edges(int scale, uint64_t m, uint64_t seed,
                     int permute, uint32_t *src, uint32_t *dst);
 uint32_t orc_rmat_perm(uint32_t v, int scale, uint64_t seed);
 
+/* Degree-corrected SBM edge list (project spec, see the .c file). */
+int orc_dcsbm_edges(uint64_t n, int scale, int nblocks, const uint32_t *bstart,
+                    const uint32_t *wtab, uint32_t ratio, uint64_t m, uint64_t seed,
+                    uint32_t *src, uint32_t *dst);
+
 /* ---- tile construction (fm_utils.cpp:617-841) ---------------------------- */
 int orc_build_2d(const uint32_t *src, const uint32_t *dst, uint64_t m,
                  uint64_t n, int log2h, int log2w, unsigned flags,
@@ -88,6 +93,14 @@ void orc_gemm_tall_f64(const double *A, int64_t lda, uint64_t n, int ka,
                        double *Y, int64_t ldy);
 void orc_colnorm_f64(const double *A, int64_t lda, uint64_t n, int k,
                      double *norms);
+
+/* ---- learned warm start: network and loss (FP64 checker) ----------------- */
+uint64_t orc_gnn_param_count(void);
+int orc_subspace_loss(uint64_t n, int k, const double *Y, const double *U, double *loss,
+                      double *dY);
+int orc_gnn_loss_grad(uint64_t n, const int64_t *rowptr, const int64_t *col, const double *dinv,
+                      const double *X0, const double *U, const double *params, double *loss,
+                      double *grad, double *Y);
 
 #ifdef __cplusplus
 }

@@ -231,3 +231,72 @@ def colnorm_f64(A):
     out = np.empty(A.shape[1], np.float64)
     lib().orc_colnorm_f64(A.ctypes.data, A.shape[1], A.shape[0], A.shape[1], out.ctypes.data)
     return out
+
+
+def _bind_gnn():
+    L = lib()
+    if getattr(L, "_gnn_bound", False):
+        return L
+    L.orc_dcsbm_edges.restype = ctypes.c_int
+    L.orc_dcsbm_edges.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_int, _u32p, _u32p,
+                                  ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint64, _u32p, _u32p]
+    L.orc_gnn_param_count.restype = ctypes.c_uint64
+    L.orc_gnn_param_count.argtypes = []
+    L.orc_subspace_loss.restype = ctypes.c_int
+    L.orc_subspace_loss.argtypes = [ctypes.c_uint64, ctypes.c_int, _vp, _vp, ctypes.POINTER(ctypes.c_double), _vp]
+    L.orc_gnn_loss_grad.restype = ctypes.c_int
+    L.orc_gnn_loss_grad.argtypes = [ctypes.c_uint64, _vp, _vp, _vp, _vp, _vp, _vp,
+                                    ctypes.POINTER(ctypes.c_double), _vp, _vp]
+    L._gnn_bound = True
+    return L
+
+
+def dcsbm_edges(n, bstart, wtab, ratio, m, seed):
+    L = _bind_gnn()
+    scale = int(n).bit_length() - 1
+    bstart = np.ascontiguousarray(bstart, np.uint32)
+    wtab = np.ascontiguousarray(wtab, np.uint32)
+    src = np.empty(m, np.uint32)
+    dst = np.empty(m, np.uint32)
+    rc = L.orc_dcsbm_edges(n, scale, len(bstart) - 1, _p(bstart, _u32p), _p(wtab, _u32p), ratio, m, seed,
+                           _p(src, _u32p), _p(dst, _u32p))
+    if rc:
+        raise ValueError("orc_dcsbm_edges failed %d" % rc)
+    return src, dst
+
+
+def gnn_param_count():
+    return _bind_gnn().orc_gnn_param_count()
+
+
+def subspace_loss(Y, U):
+    L = _bind_gnn()
+    Y = np.ascontiguousarray(Y, np.float64)
+    U = np.ascontiguousarray(U, np.float64)
+    dY = np.empty_like(Y)
+    loss = ctypes.c_double()
+    rc = L.orc_subspace_loss(Y.shape[0], Y.shape[1], Y.ctypes.data, U.ctypes.data, ctypes.byref(loss),
+                             dY.ctypes.data)
+    if rc:
+        raise ValueError("orc_subspace_loss failed")
+    return loss.value, dY
+
+
+def gnn_loss_grad(rowptr, col, dinv, X0, U, params, want_grad=True):
+    L = _bind_gnn()
+    rowptr = np.ascontiguousarray(rowptr, np.int64)
+    col = np.ascontiguousarray(col, np.int64)
+    dinv = np.ascontiguousarray(dinv, np.float64)
+    X0 = np.ascontiguousarray(X0, np.float64)
+    U = np.ascontiguousarray(U, np.float64)
+    params = np.ascontiguousarray(params, np.float64)
+    n = len(rowptr) - 1
+    grad = np.zeros_like(params) if want_grad else None
+    Y = np.empty((n, 16), np.float64)
+    loss = ctypes.c_double()
+    rc = L.orc_gnn_loss_grad(n, rowptr.ctypes.data, col.ctypes.data, dinv.ctypes.data, X0.ctypes.data,
+                             U.ctypes.data, params.ctypes.data, ctypes.byref(loss),
+                             grad.ctypes.data if want_grad else None, Y.ctypes.data)
+    if rc:
+        raise ValueError("orc_gnn_loss_grad failed")
+    return loss.value, grad, Y
