@@ -1012,10 +1012,51 @@ int orc_subspace_loss(uint64_t n, int k, const double *Y, const double *U, doubl
     return ORC_OK;
 }
 
+/* Round-to-nearest-even to bfloat16 through FP32 (the GPU kernels round
+ * their FP32 operands this way before a BF16 matrix-core product). */
+static double bf16_rne(double x)
+{
+    float f = (float)x;
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    if ((u & 0x7F800000u) != 0x7F800000u) u += 0x7FFFu + ((u >> 16) & 1u);
+    u &= 0xFFFF0000u;
+    memcpy(&f, &u, 4);
+    return (double)f;
+}
+
+void orc_bf16_round(const double *in, double *out, uint64_t n)
+{
+    for (uint64_t i = 0; i < n; i++) out[i] = bf16_rne(in[i]);
+}
+
+/* Copy of a[0..n) with every value rounded to BF16 when `on`, else a itself. */
+static const double *maybe_bf16(const double *a, uint64_t n, int on, double **keep)
+{
+    *keep = NULL;
+    if (!on) return a;
+    double *b = (double *)malloc(n * sizeof(double));
+    orc_bf16_round(a, b, n);
+    *keep = b;
+    return b;
+}
+
 int orc_gnn_loss_grad(uint64_t n, const int64_t *rp, const int64_t *ci, const double *dinv,
                       const double *X0, const double *U, const double *P, double *loss,
                       double *grad, double *Yout)
 {
+    return orc_gnn_loss_grad_ex(n, rp, ci, dinv, X0, U, P, loss, grad, Yout, 0);
+}
+
+/* bf16_operands = 1 rounds the operands of every dense product (activations,
+ * aggregated features, weights, output and hidden gradients) to BF16 where
+ * the BF16 matrix-core mode of the GPU kernels does; sums stay FP64. */
+int orc_gnn_loss_grad_ex(uint64_t n, const int64_t *rp, const int64_t *ci, const double *dinv,
+                         const double *X0, const double *U, const double *P, double *loss,
+                         double *grad, double *Yout, int bf16_operands)
+{
+    const int rb = bf16_operands;
+    double *tmp_a, *tmp_b, *tmp_w0, *tmp_w1;
     double *H[4], *Agg[3], *Z[3];
     H[0] = (double *)X0;
     for (int l = 0; l < 3; l++) {
@@ -1043,52 +1084,77 @@ int orc_gnn_loss_grad(uint64_t n, const int64_t *rp, const int64_t *ci, const do
     for (int l = 0; l < 3; l++) {
         const int d = gnn_din(l);
         gnn_agg(n, rp, ci, dinv, H[l], d, Agg[l]);
+        const double *Hr = maybe_bf16(H[l], n * d, rb, &tmp_a);
+        const double *Ar = maybe_bf16(Agg[l], n * d, rb, &tmp_b);
+        const double *W0 = maybe_bf16(Wp[l][0], (uint64_t)d * GNN_H, rb, &tmp_w0);
+        const double *W1 = maybe_bf16(Wp[l][1], (uint64_t)d * GNN_H, rb, &tmp_w1);
         for (uint64_t r = 0; r < n; r++)
             for (int j = 0; j < GNN_H; j++) {
-                double s = Wp[l][2][j];
+                double s = 0;
                 for (int t = 0; t < d; t++)
-                    s += H[l][r * d + t] * Wp[l][0][t * GNN_H + j] + Agg[l][r * d + t] * Wp[l][1][t * GNN_H + j];
+                    s += Hr[r * d + t] * W0[t * GNN_H + j] + Ar[r * d + t] * W1[t * GNN_H + j];
+                s += Wp[l][2][j];
                 Z[l][r * GNN_H + j] = s;
                 H[l + 1][r * GNN_H + j] = s > 0 ? s : 0;
             }
+        free(tmp_a); free(tmp_b); free(tmp_w0); free(tmp_w1);
     }
-    for (uint64_t r = 0; r < n; r++)
-        for (int j = 0; j < GNN_OUT; j++) {
-            double s = 0;
-            for (int t = 0; t < GNN_H; t++) s += H[3][r * GNN_H + t] * Wo[t * GNN_OUT + j];
-            Y[r * GNN_OUT + j] = s;
-        }
+    {
+        const double *Hr = maybe_bf16(H[3], n * GNN_H, rb, &tmp_a);
+        const double *Wr = maybe_bf16(Wo, (uint64_t)GNN_H * GNN_OUT, rb, &tmp_w0);
+        for (uint64_t r = 0; r < n; r++)
+            for (int j = 0; j < GNN_OUT; j++) {
+                double s = 0;
+                for (int t = 0; t < GNN_H; t++) s += Hr[r * GNN_H + t] * Wr[t * GNN_OUT + j];
+                Y[r * GNN_OUT + j] = s;
+            }
+        free(tmp_a); free(tmp_w0);
+    }
     if (Yout) memcpy(Yout, Y, n * GNN_OUT * sizeof(double));
     int rc = orc_subspace_loss(n, GNN_OUT, Y, U, loss, dY);
     if (rc == ORC_OK && grad) {
         /* dWo = H3^T dY ; dH3 = dY Wo^T */
-        orc_gram_f64(H[3], GNN_H, dY, GNN_OUT, n, GNN_H, GNN_OUT, grad + wo_off);
-        for (uint64_t r = 0; r < n; r++)
-            for (int t = 0; t < GNN_H; t++) {
-                double s = 0;
-                for (int j = 0; j < GNN_OUT; j++) s += dY[r * GNN_OUT + j] * Wo[t * GNN_OUT + j];
-                dH[r * GNN_H + t] = s;
-            }
+        {
+            const double *Hr = maybe_bf16(H[3], n * GNN_H, rb, &tmp_a);
+            const double *dYr = maybe_bf16(dY, n * GNN_OUT, rb, &tmp_b);
+            const double *Wr = maybe_bf16(Wo, (uint64_t)GNN_H * GNN_OUT, rb, &tmp_w0);
+            orc_gram_f64(Hr, GNN_H, dYr, GNN_OUT, n, GNN_H, GNN_OUT, grad + wo_off);
+            for (uint64_t r = 0; r < n; r++)
+                for (int t = 0; t < GNN_H; t++) {
+                    double s = 0;
+                    for (int j = 0; j < GNN_OUT; j++) s += dYr[r * GNN_OUT + j] * Wr[t * GNN_OUT + j];
+                    dH[r * GNN_H + t] = s;
+                }
+            free(tmp_a); free(tmp_b); free(tmp_w0);
+        }
         for (int l = 2; l >= 0; l--) {
             const int d = gnn_din(l);
             for (uint64_t e = 0; e < n * GNN_H; e++) dZ[e] = Z[l][e] > 0 ? dH[e] : 0.0;
-            orc_gram_f64(H[l], d, dZ, GNN_H, n, d, GNN_H, grad + goff[l]);
-            orc_gram_f64(Agg[l], d, dZ, GNN_H, n, d, GNN_H, grad + goff[l] + (uint64_t)d * GNN_H);
+            double *tmp_z;
+            const double *dZr = maybe_bf16(dZ, n * GNN_H, rb, &tmp_z);
+            const double *Hr = maybe_bf16(H[l], n * d, rb, &tmp_a);
+            const double *Ar = maybe_bf16(Agg[l], n * d, rb, &tmp_b);
+            orc_gram_f64(Hr, d, dZr, GNN_H, n, d, GNN_H, grad + goff[l]);
+            orc_gram_f64(Ar, d, dZr, GNN_H, n, d, GNN_H, grad + goff[l] + (uint64_t)d * GNN_H);
+            free(tmp_a); free(tmp_b);
             double *db = grad + goff[l] + 2ull * d * GNN_H;
             for (int j = 0; j < GNN_H; j++) db[j] = 0;
             for (uint64_t r = 0; r < n; r++) for (int j = 0; j < GNN_H; j++) db[j] += dZ[r * GNN_H + j];
-            if (l == 0) break;
+            if (l == 0) { free(tmp_z); break; }
             /* dH_l = dZ Ws^T + Ahat (dZ Wn^T) */
+            const double *W0 = maybe_bf16(Wp[l][0], (uint64_t)d * GNN_H, rb, &tmp_w0);
+            const double *W1 = maybe_bf16(Wp[l][1], (uint64_t)d * GNN_H, rb, &tmp_w1);
             for (uint64_t r = 0; r < n; r++)
                 for (int t = 0; t < d; t++) {
                     double a = 0, b = 0;
                     for (int j = 0; j < GNN_H; j++) {
-                        a += dZ[r * GNN_H + j] * Wp[l][0][t * GNN_H + j];
-                        b += dZ[r * GNN_H + j] * Wp[l][1][t * GNN_H + j];
+                        a += dZr[r * GNN_H + j] * W0[t * GNN_H + j];
+                        b += dZr[r * GNN_H + j] * W1[t * GNN_H + j];
                     }
                     T[r * d + t] = a;
                     T2[r * d + t] = b;
                 }
+            free(tmp_z); free(tmp_w0); free(tmp_w1);
             gnn_agg(n, rp, ci, dinv, T2, d, dH);
             for (uint64_t e = 0; e < n * (uint64_t)d; e++) dH[e] += T[e];
         }

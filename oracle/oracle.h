@@ -101,6 +101,10 @@ int orc_subspace_loss(uint64_t n, int k, const double *Y, const double *U, doubl
 int orc_gnn_loss_grad(uint64_t n, const int64_t *rowptr, const int64_t *col, const double *dinv,
                       const double *X0, const double *U, const double *params, double *loss,
                       double *grad, double *Y);
+int orc_gnn_loss_grad_ex(uint64_t n, const int64_t *rowptr, const int64_t *col, const double *dinv,
+                         const double *X0, const double *U, const double *params, double *loss,
+                         double *grad, double *Y, int bf16_operands);
+void orc_bf16_round(const double *in, double *out, uint64_t n);
 
 #ifdef __cplusplus
 }

@@ -247,6 +247,11 @@ def _bind_gnn():
     L.orc_gnn_loss_grad.restype = ctypes.c_int
     L.orc_gnn_loss_grad.argtypes = [ctypes.c_uint64, _vp, _vp, _vp, _vp, _vp, _vp,
                                     ctypes.POINTER(ctypes.c_double), _vp, _vp]
+    L.orc_gnn_loss_grad_ex.restype = ctypes.c_int
+    L.orc_gnn_loss_grad_ex.argtypes = [ctypes.c_uint64, _vp, _vp, _vp, _vp, _vp, _vp,
+                                       ctypes.POINTER(ctypes.c_double), _vp, _vp, ctypes.c_int]
+    L.orc_bf16_round.restype = None
+    L.orc_bf16_round.argtypes = [_vp, _vp, ctypes.c_uint64]
     L._gnn_bound = True
     return L
 
@@ -282,7 +287,19 @@ def subspace_loss(Y, U):
     return loss.value, dY
 
 
-def gnn_loss_grad(rowptr, col, dinv, X0, U, params, want_grad=True):
+def bf16_round(x):
+    """Round-to-nearest-even to bfloat16 (through FP32), returned as float64."""
+    L = _bind_gnn()
+    x = np.ascontiguousarray(x, np.float64)
+    out = np.empty_like(x)
+    L.orc_bf16_round(x.ctypes.data, out.ctypes.data, x.size)
+    return out
+
+
+def gnn_loss_grad(rowptr, col, dinv, X0, U, params, want_grad=True, bf16=False):
+    """FP64 forward, subspace loss and gradient of the warm-start network;
+    bf16=True rounds every dense-product operand to BF16 as the GPU's BF16
+    matrix-core mode does."""
     L = _bind_gnn()
     rowptr = np.ascontiguousarray(rowptr, np.int64)
     col = np.ascontiguousarray(col, np.int64)
@@ -294,9 +311,9 @@ def gnn_loss_grad(rowptr, col, dinv, X0, U, params, want_grad=True):
     grad = np.zeros_like(params) if want_grad else None
     Y = np.empty((n, 16), np.float64)
     loss = ctypes.c_double()
-    rc = L.orc_gnn_loss_grad(n, rowptr.ctypes.data, col.ctypes.data, dinv.ctypes.data, X0.ctypes.data,
-                             U.ctypes.data, params.ctypes.data, ctypes.byref(loss),
-                             grad.ctypes.data if want_grad else None, Y.ctypes.data)
+    rc = L.orc_gnn_loss_grad_ex(n, rowptr.ctypes.data, col.ctypes.data, dinv.ctypes.data, X0.ctypes.data,
+                                U.ctypes.data, params.ctypes.data, ctypes.byref(loss),
+                                grad.ctypes.data if want_grad else None, Y.ctypes.data, int(bool(bf16)))
     if rc:
         raise ValueError("orc_gnn_loss_grad failed")
     return loss.value, grad, Y

@@ -81,3 +81,35 @@ def test_dcsbm_invariants(orc):
     blk = np.searchsorted(bstart, inv, side="right") - 1
     intra = np.mean(blk[s1] == blk[d1])
     assert 0.3 < intra < 0.5, intra  # 20x odds over 31 other blocks -> ~0.39
+
+
+def test_bf16_rounding_matches_torch(orc):
+    """The checker's BF16 rounding against torch's float32 -> bfloat16
+    conversion (round to nearest even), including exact ties."""
+    import torch
+    rng = np.random.default_rng(4)
+    x = np.concatenate([rng.standard_normal(5000) * 10.0 ** rng.integers(-6, 6, 5000),
+                        [1 + 2.0 ** -8, 1 + 3 * 2.0 ** -8, -(1 + 2.0 ** -8), 0.0, -0.0, 65504.0]])
+    x = x.astype(np.float32).astype(np.float64)
+    want = torch.from_numpy(x.astype(np.float32)).to(torch.bfloat16).to(torch.float64).numpy()
+    got = orc.bf16_round(x)
+    assert np.array_equal(got, want)
+    assert got[-6] == 1.0 and got[-5] == 1 + 2.0 ** -6  # ties to even
+
+
+def test_gnn_bf16_operand_mode(orc):
+    """Rounding the product operands to BF16 moves the output by a BF16-sized
+    amount; the gradient moves more (the subspace loss inverts Y^T Y), which
+    is why the GPU's BF16 mode is checked against this mode, not FP64."""
+    A, dinv = tiny_graph()
+    n = A.shape[0]
+    rng = np.random.default_rng(5)
+    X0 = rng.standard_normal((n, 16))
+    U, _ = np.linalg.qr(rng.standard_normal((n, 16)))
+    P = rng.standard_normal(orc.gnn_param_count()) * 0.2
+    l64, g64, Y64 = orc.gnn_loss_grad(A.indptr, A.indices, dinv, X0, U, P)
+    l16, g16, Y16 = orc.gnn_loss_grad(A.indptr, A.indices, dinv, X0, U, P, bf16=True)
+    rel = np.abs(Y16 - Y64).max() / np.abs(Y64).max()
+    assert 1e-5 < rel < 5e-2
+    assert abs(l16 - l64) < 5e-2 * max(1.0, abs(l64))
+    assert 1e-4 * np.linalg.norm(g64) < np.linalg.norm(g16 - g64) < 0.3 * np.linalg.norm(g64)
