@@ -3,6 +3,7 @@
 //
 // Reference paths are relative to the FlashX source tree.
 #include <algorithm>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <numeric>
@@ -106,12 +107,14 @@ fmx_status analyse_image(fmx_ctx *ctx, fmx_spm *s)
             return fail(ctx, FMX_EFORMAT, "block row larger than 2^31 words");
         const uint64_t rows_here = std::min<uint64_t>(s->H, s->nrows - br * s->H);
         long prev_bc = -1;
+        uint64_t empty_tiles = 0;
         while (p < end) {
             if (end - p < 16) return fail(ctx, FMX_EFORMAT, "truncated tile header");
             uint32_t tbr = rd32(p), tbc = rd32(p + 4), tnnz = rd32(p + 8);
             uint32_t tnrow = rd16(p + 12), tncoo = rd16(p + 14);
             if (tbr != br) return fail(ctx, FMX_EFORMAT, "tile in the wrong block row");
             uint64_t size = 16;
+            if (!tnnz) empty_tiles++;
             if (tnnz) {
                 if ((long)tbc <= prev_bc) return fail(ctx, FMX_EFORMAT, "tiles not in ascending block column");
                 if ((uint64_t)tbc >= s->nbc) return fail(ctx, FMX_EFORMAT, "block column out of range");
@@ -164,6 +167,11 @@ fmx_status analyse_image(fmx_ctx *ctx, fmx_spm *s)
         }
         if (p != end) return fail(ctx, FMX_EFORMAT, "block row size disagrees with its tiles");
         if (s->dir.size() == s->brow_dir[br]) return fail(ctx, FMX_EFORMAT, "block row without tiles");
+        // an empty tile only stands for an empty block row (fm_utils.cpp:
+        // 317,340-350); the SpMM decoder relies on every other tile being at
+        // least 11 words long
+        if (empty_tiles && s->dir.size() - s->brow_dir[br] > 1)
+            return fail(ctx, FMX_EFORMAT, "empty tile in a non-empty block row");
     }
     s->brow_dir[nbr] = (uint32_t)s->dir.size();
     s->ntiles = s->dir.size();
@@ -436,6 +444,75 @@ fmx_status fmx_spm_download(const fmx_spm *spm, void *image, size_t cap,
         FMX_HIP(ctx, hipMemcpy(image, spm->d_image, spm->bytes, hipMemcpyDeviceToHost));
     }
     if (offs) memcpy(offs, spm->offs.data(), spm->offs.size() * sizeof(int64_t));
+    return FMX_OK;
+}
+
+namespace {
+
+bool read_file(const char *path, std::vector<uint8_t> &buf)
+{
+    FILE *f = fopen(path, "rb");
+    if (!f) return false;
+    bool ok = fseek(f, 0, SEEK_END) == 0;
+    long n = ok ? ftell(f) : -1;
+    ok = ok && n >= 0 && fseek(f, 0, SEEK_SET) == 0;
+    if (ok) {
+        buf.resize((size_t)n);
+        ok = n == 0 || fread(buf.data(), (size_t)n, 1, f) == 1;
+    }
+    fclose(f);
+    return ok;
+}
+
+bool write_file(const char *path, const void *a, size_t na, const void *b, size_t nb)
+{
+    FILE *f = fopen(path, "wb");
+    if (!f) return false;
+    bool ok = (na == 0 || fwrite(a, na, 1, f) == 1) && (nb == 0 || fwrite(b, nb, 1, f) == 1);
+    ok = (fclose(f) == 0) && ok;
+    return ok;
+}
+
+}  // namespace
+
+fmx_status fmx_spm_load(fmx_ctx *ctx, const char *mat_file, const char *idx_file, fmx_spm **out)
+{
+    if (!ctx || !mat_file || !idx_file || !out) return fail(ctx, FMX_EINVAL, "null argument");
+    *out = nullptr;
+    std::vector<uint8_t> idx, img;
+    if (!read_file(idx_file, idx)) return fail(ctx, FMX_EINVAL, std::string("can't read ") + idx_file);
+    if (!read_file(mat_file, img)) return fail(ctx, FMX_EINVAL, std::string("can't read ") + mat_file);
+    // SpM_2d_index: the matrix header page, then nbr+1 block-row offsets
+    if (idx.size() < kHeaderBytes + 8 || (idx.size() - kHeaderBytes) % 8)
+        return fail(ctx, FMX_EFORMAT, "index file is not a header plus 8-byte offsets");
+    if (img.size() < kHeaderBytes) return fail(ctx, FMX_EFORMAT, "matrix file shorter than its header");
+    // both files describe the same matrix: type, entry size, shape, layout, tiles
+    if (memcmp(idx.data(), img.data(), 64) != 0)
+        return fail(ctx, FMX_EFORMAT, "index and matrix headers differ");
+    const size_t nent = (idx.size() - kHeaderBytes) / 8;
+    std::vector<int64_t> offs(nent);
+    memcpy(offs.data(), idx.data() + kHeaderBytes, nent * 8);
+    fmx_status st = fmx_spm_upload(ctx, img.data(), img.size(), offs.data(), nent - 1, out);
+    if (st) return st;
+    if ((*out)->bytes != img.size()) {
+        fmx_spm_destroy(*out);
+        *out = nullptr;
+        return fail(ctx, FMX_EFORMAT, "matrix file is longer than its block rows");
+    }
+    return FMX_OK;
+}
+
+fmx_status fmx_spm_dump(const fmx_spm *spm, const char *mat_file, const char *idx_file)
+{
+    if (!spm || !mat_file || !idx_file) return FMX_EINVAL;
+    fmx_ctx *ctx = spm->ctx;
+    std::vector<uint8_t> img(spm->bytes);
+    fmx_status st = fmx_spm_download(spm, img.data(), img.size(), nullptr, nullptr);
+    if (st) return st;
+    if (!write_file(mat_file, img.data(), img.size(), nullptr, 0))
+        return fail(ctx, FMX_EINVAL, std::string("can't write ") + mat_file);
+    if (!write_file(idx_file, img.data(), kHeaderBytes, spm->offs.data(), spm->offs.size() * sizeof(int64_t)))
+        return fail(ctx, FMX_EINVAL, std::string("can't write ") + idx_file);
     return FMX_OK;
 }
 

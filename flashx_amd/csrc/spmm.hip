@@ -6,20 +6,28 @@
 //
 // Execution model (one wave = one work item, no cross-wave sharing):
 //   * an item is a contiguous word range of one block row (tile headers and
-//     all), plus a band of <= Hs output rows kept in LDS;
-//   * the wave streams the range 64*U words at a time with coalesced u16
-//     loads, classifies every word (tile header / row head / column) with
-//     ballots against a 64-entry window of the tile directory, resolves each
-//     column's row with a ballot prefix (the nearest head to its left, or the
-//     row carried over), and appends (band row, global column) to an LDS queue;
+//     all) plus a band of <= Hs output rows kept in LDS;
+//   * the range is decoded 64*R words at a time (a "chunk", 16-byte aligned):
+//     lane l owns R consecutive words, finds the tile holding its first word
+//     by a binary search of an LDS window of the tile directory, and
+//     classifies its words (tile header / row head / column) with per-lane
+//     state.  The row in effect at a lane's first word is the last row head
+//     of the nearest lane below it (ballot + shuffle), and a wave prefix sum
+//     of per-lane column counts writes the (global column, band row) entries
+//     in stream order to an LDS queue;
 //   * the queue is drained with vector gathers of X rows (16 bytes per lane,
-//     L lanes per non-zero) issued P at a time, accumulated with LDS adds into
-//     the band;
+//     L lanes per non-zero).  Lane group g takes P consecutive entries, so a
+//     run of one row (a row part of a tile) is summed in registers and reaches
+//     the band with one LDS add per element at each row change;
+//   * for compile-time widths a chunk is exactly one drain batch and the loop
+//     is software-pipelined: the gathers of chunk i are in flight while chunk
+//     i+1 is decoded and the words of chunk i+2 are requested;
 //   * the band is written to Y once (rows without non-zeros are 0), or to a
 //     partial slot when a power-law band was split over several items; split
 //     bands are summed slot by slot, in stream order, by spmm_reduce.
 // Each band is owned by exactly one wave, so results do not depend on timing.
 // FP32 data accumulate in an FP64 band and are rounded once on write-out.
+#include <algorithm>
 #include <cstdlib>
 
 #include "fmx_internal.h"
@@ -27,7 +35,7 @@
 namespace fmx {
 namespace {
 
-constexpr int kWPB = 4;  // waves per workgroup (independent items)
+constexpr int kWPB = 1;  // waves per workgroup: one item, so LDS packs per wave
 
 struct SpmmArgs {
     const uint16_t *img;
@@ -53,7 +61,6 @@ struct alignas(sizeof(T) * E) Vec {
     T v[E];
 };
 
-__device__ __forceinline__ uint64_t mask_le(int lane) { return (2ull << lane) - 1ull; }
 __device__ __forceinline__ uint64_t mask_lt(int lane) { return (1ull << lane) - 1ull; }
 
 __device__ __forceinline__ uint32_t rdlane(uint32_t v, int l)
@@ -72,6 +79,15 @@ __device__ __forceinline__ void lds_add(T *p, T v)
     __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
+// Orders this wave's LDS writes before its later LDS reads.  LDS executes one
+// wave's instructions in order, so only compiler reordering must be stopped;
+// a fence would also wait for every outstanding global load.
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_wave_barrier();
+    asm volatile("" ::: "memory");
+}
+
 // LDS accumulator type: FP32 data accumulate in FP64 (ds_add_f64 is fast on
 // gfx950 where ds_add_f32 is not, and the sum is then rounded once).
 template <typename T> struct AccOf { using type = T; };
@@ -86,37 +102,259 @@ struct Lanes {
     static constexpr int NZP = 64 / L;                     // non-zeros per pass
 };
 
-// Drains queue entries [0, qn) (qn >= 1) into the band.  K > 0: compile-time
-// width.  Loads are unconditional (indices clamped to the last entry) so the
-// P gathers of a batch are all in flight before the first LDS add.
+// Band column of output column c for compile-time widths: lane q of a
+// non-zero's L lanes holds columns q*E .. q*E+E-1 and adds its i-th one at
+// band column i*L+q, so the L lanes of one add touch consecutive words.
+template <typename T, int K>
+__device__ __forceinline__ uint32_t band_col(uint32_t c)
+{
+    using C = Lanes<T, K>;
+    return (c % C::E) * C::L + c / C::E;
+}
+
+// Queue entry e lives at slot e + e/P: the P-entry segments of the lane
+// groups then start in different LDS banks.
+template <int P>
+__device__ __forceinline__ uint32_t qslot(uint32_t e) { return e + e / P; }
+
+// ---------------------------------------------------------------------------
+// Lane-sequential decode
+template <int R>
+struct LaneCfg {
+    static constexpr int CW = 64 * R;                                // words per chunk
+    static constexpr int NB = 1 + (R + 10) / 11;                     // tiles a lane can touch
+    static constexpr int DWIN = ((CW - 1) / 11 + 2 + 63) / 64 * 64;  // window entries
+    static constexpr int DPL = DWIN / 64;
+};
+// A stored tile is >= 11 words (16-byte header, one entry, end marker; a
+// 16-byte stub only for an empty block row, which is an item by itself), so
+// a chunk meets at most (CW-1)/11+1 tile starts after the tile holding its
+// first word, and R words at most NB-1.
+
+struct ItemCtx {
+    uint64_t bw;          // absolute word index of the block row
+    int32_t begin, pend;  // item words [begin, pend), relative to the block row
+    uint32_t sb_base, rows;
+    int log2w;
+    uint64_t ncols;
+};
+
+// Decodes the chunk starting at absolute word c (8-word aligned) from this
+// lane's R words in wreg and the directory window win (DWIN entries and a
+// sentinel).  Writes the chunk's entries in stream order at queue slots
+// qslot<SEGP>(e) -- band row 0xFFFF for an entry outside the band -- and
+// returns their number; carry becomes the row in effect after the chunk.
+template <int R, int SEGP>
+__device__ __forceinline__ uint32_t decode_chunk(const ItemCtx &ic, uint64_t c, const uint32_t *wreg,
+                                                 const uint2 *win, int &carry, uint32_t *qcol, uint16_t *qrow,
+                                                 int lane)
+{
+    using LC = LaneCfg<R>;
+    constexpr int NB = LC::NB, DWIN = LC::DWIN;
+    const int32_t p0 = (int32_t)((int64_t)c - (int64_t)ic.bw) + lane * R;
+    const int32_t q = p0 > ic.begin ? p0 : ic.begin;
+    // last window tile starting at or before q (window entry 0 holds the
+    // chunk's first item word, sentinels compare greater)
+    uint32_t j = 0;
+#pragma unroll
+    for (int st = DWIN / 2; st >= 1; st >>= 1)
+        if (win[j + st].x <= (uint32_t)q) j += st;
+    uint32_t ts0[NB], tb0[NB];
+#pragma unroll
+    for (int b = 0; b < NB; b++) {
+        const uint32_t jj = j + b < DWIN ? j + b : DWIN;
+        const uint2 e = win[jj];
+        ts0[b] = e.x;
+        tb0[b] = e.y;
+    }
+    // pass 1: column count and last row head of this lane
+    int cnt = 0, lrow = 0;
+    bool hh = false;
+    {
+        uint32_t ts[NB];
+#pragma unroll
+        for (int b = 0; b < NB; b++) ts[b] = ts0[b];
+#pragma unroll
+        for (int i = 0; i < R; i++) {
+            const int32_t pos = p0 + i;
+            const uint32_t w = (wreg[i >> 1] >> ((i & 1) * 16)) & 0xFFFFu;
+            const bool valid = pos >= ic.begin && pos < ic.pend;
+            if (NB > 1 && valid && (uint32_t)pos >= ts[NB > 1 ? 1 : 0]) {
+#pragma unroll
+                for (int b = 0; b + 1 < NB; b++) ts[b] = ts[b + 1];
+                ts[NB - 1] = 0xFFFFFFFFu;
+            }
+            const bool body = valid && ((uint32_t)pos - ts[0]) >= 8u;
+            if (body && (w & 0x8000u)) {
+                lrow = (int)(w & 0x7FFFu);
+                hh = true;
+            }
+            cnt += (body && !(w & 0x8000u)) ? 1 : 0;
+        }
+    }
+    // row in effect at this lane's first word; row after the chunk
+    const uint64_t hm = __ballot(hh);
+    const uint64_t below = hm & mask_lt(lane);
+    const int src = below ? 63 - __clzll(below) : 0;
+    const int prow = __shfl(lrow, src);
+    const int row_in = below ? prow : carry;
+    if (hm) carry = (int)rdlane((uint32_t)lrow, 63 - __clzll(hm));
+    int incl = cnt;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int t = __shfl_up(incl, d);
+        if (lane >= d) incl += t;
+    }
+    const uint32_t qn = rdlane((uint32_t)incl, 63);
+    // pass 2: entries in stream order
+    {
+        uint32_t ts[NB], tb[NB];
+#pragma unroll
+        for (int b = 0; b < NB; b++) {
+            ts[b] = ts0[b];
+            tb[b] = tb0[b];
+        }
+        int r = row_in;
+        uint32_t o = (uint32_t)(incl - cnt);
+#pragma unroll
+        for (int i = 0; i < R; i++) {
+            const int32_t pos = p0 + i;
+            const uint32_t w = (wreg[i >> 1] >> ((i & 1) * 16)) & 0xFFFFu;
+            const bool valid = pos >= ic.begin && pos < ic.pend;
+            if (NB > 1 && valid && (uint32_t)pos >= ts[NB > 1 ? 1 : 0]) {
+#pragma unroll
+                for (int b = 0; b + 1 < NB; b++) {
+                    ts[b] = ts[b + 1];
+                    tb[b] = tb[b + 1];
+                }
+                ts[NB - 1] = 0xFFFFFFFFu;
+            }
+            const bool body = valid && ((uint32_t)pos - ts[0]) >= 8u;
+            if (body && (w & 0x8000u)) r = (int)(w & 0x7FFFu);
+            if (body && !(w & 0x8000u)) {
+                const uint32_t br = (uint32_t)(r - (int)ic.sb_base);
+                const uint32_t gcol = (tb[0] << ic.log2w) | w;
+                const bool keep = r >= 0 && br < ic.rows && gcol < ic.ncols;
+                const uint32_t sl = qslot<SEGP>(o);
+                qcol[sl] = keep ? gcol : 0u;
+                qrow[sl] = keep ? (uint16_t)br : (uint16_t)0xFFFFu;
+                o++;
+            }
+        }
+    }
+    return qn;
+}
+
+// This lane's words of chunk c, clamped to the aligned 8-word group holding
+// the item's last word (always inside the image allocation).
+template <int R>
+__device__ __forceinline__ void load_words(const uint16_t *img, uint64_t c, uint64_t last8, int lane,
+                                           uint32_t *wreg)
+{
+    static_assert(R % 8 == 0, "lane word count");
+#pragma unroll
+    for (int h = 0; h < R / 8; h++) {
+        uint64_t g = c + (uint64_t)lane * R + h * 8;
+        g = g < last8 ? g : last8;
+        const uint4 v = *(const uint4 *)(img + g);
+        wreg[h * 4 + 0] = v.x;
+        wreg[h * 4 + 1] = v.y;
+        wreg[h * 4 + 2] = v.z;
+        wreg[h * 4 + 3] = v.w;
+    }
+}
+
+// Directory entries ct + h*64 + lane (clamped); `live` marks those inside the
+// block row, the others become sentinels when the window is stored.
+template <int DPL>
+__device__ __forceinline__ void load_window(const uint2 *dir, uint32_t ct, uint32_t dlast, int lane, uint2 *dreg,
+                                            bool *live)
+{
+#pragma unroll
+    for (int h = 0; h < DPL; h++) {
+        const uint32_t t = ct + h * 64 + lane;
+        live[h] = t <= dlast;
+        dreg[h] = dir[t < dlast ? t : dlast];
+    }
+}
+
+template <int DPL>
+__device__ __forceinline__ void store_window(uint2 *win, const uint2 *dreg, const bool *live, int lane)
+{
+#pragma unroll
+    for (int h = 0; h < DPL; h++) win[h * 64 + lane] = live[h] ? dreg[h] : make_uint2(0xFFFFFFFFu, 0u);
+}
+
+// Start of the next chunk's window: entries of this window starting at or
+// before its first word (the window is sorted, sentinels compare greater).
+template <int DPL>
+__device__ __forceinline__ uint32_t window_advance(const uint2 *dreg, const bool *live, uint32_t pn)
+{
+    int n = 0;
+#pragma unroll
+    for (int h = 0; h < DPL; h++) n += __popcll(__ballot(live[h] && dreg[h].x <= pn));
+    return (uint32_t)n;
+}
+
+// ---------------------------------------------------------------------------
+// Drains (compile-time width K): gathers of P consecutive entries per lane
+// group, summed in registers per row run, added to the band at row changes.
 template <typename T, int K, int P, typename A>
-__device__ __forceinline__ void drain(const SpmmArgs &a, A *band, uint32_t S, const uint32_t *qcol,
-                                      const uint16_t *qrow, uint32_t qn, int lane)
+__device__ __forceinline__ void gather_batch(const SpmmArgs &a, const uint32_t *qcol, const uint16_t *qrow,
+                                             uint32_t base, uint32_t qn, int lane, Vec<T, Lanes<T, K>::E> *v,
+                                             uint32_t *rr)
 {
     using C = Lanes<T, K>;
     const T *X = (const T *)a.X;
     const int q = lane % C::L;
     const int g = lane / C::L;
-    const uint32_t last = qn - 1;
-    for (uint32_t base = 0; base < qn; base += C::NZP * P) {
-        Vec<T, C::E> v[P];
-        uint32_t rr[P];
+    uint32_t gcs[P];
 #pragma unroll
-        for (int pp = 0; pp < P; pp++) {
-            const uint32_t e = base + pp * C::NZP + g;
-            const uint32_t ec = e < last ? e : last;
-            const uint32_t gc = qcol[ec];
-            rr[pp] = e <= last ? (uint32_t)qrow[ec] : 0xFFFFFFFFu;
-            v[pp] = *(const Vec<T, C::E> *)(X + (uint64_t)gc * a.ldx + q * C::E);
-        }
+    for (int pp = 0; pp < P; pp++) {
+        const uint32_t e = base + g * P + pp;
+        const uint32_t sl = qslot<P>(e);
+        const bool ok = e < qn;
+        const uint32_t qr = qrow[sl];
+        gcs[pp] = ok ? qcol[sl] : 0u;
+        rr[pp] = (ok && qr != 0xFFFFu) ? qr : 0xFFFFFFFFu;
+    }
+    // every queue read before the first gather, every gather before the first use
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int pp = 0; pp < P; pp++) {
-            if (rr[pp] != 0xFFFFFFFFu) {
-                A *dst = band + rr[pp] * S + q * C::E;
+    for (int pp = 0; pp < P; pp++) v[pp] = *(const Vec<T, C::E> *)(X + (uint64_t)gcs[pp] * a.ldx + q * C::E);
+    __builtin_amdgcn_sched_barrier(0);
+}
+
+template <typename T, int K, int P, typename A>
+__device__ __forceinline__ void accumulate_batch(A *band, uint32_t S, int lane, const Vec<T, Lanes<T, K>::E> *v,
+                                                 const uint32_t *rr)
+{
+    using C = Lanes<T, K>;
+    const int q = lane % C::L;
+    A acc[C::E];
+    uint32_t cur = rr[0];
 #pragma unroll
-                for (int i = 0; i < C::E; i++) lds_add(dst + i, (A)v[pp].v[i]);
+    for (int i = 0; i < C::E; i++) acc[i] = (A)v[0].v[i];
+#pragma unroll
+    for (int pp = 1; pp < P; pp++) {
+        if (rr[pp] != cur) {
+            if (cur != 0xFFFFFFFFu) {
+                A *dst = band + cur * S + q;
+#pragma unroll
+                for (int i = 0; i < C::E; i++) lds_add(dst + i * C::L, acc[i]);
             }
+            cur = rr[pp];
+#pragma unroll
+            for (int i = 0; i < C::E; i++) acc[i] = (A)v[pp].v[i];
+        } else {
+#pragma unroll
+            for (int i = 0; i < C::E; i++) acc[i] += (A)v[pp].v[i];
         }
+    }
+    if (cur != 0xFFFFFFFFu) {
+        A *dst = band + cur * S + q;
+#pragma unroll
+        for (int i = 0; i < C::E; i++) lds_add(dst + i * C::L, acc[i]);
     }
 }
 
@@ -136,8 +374,9 @@ __device__ __forceinline__ void drain_generic(const SpmmArgs &a, A *band, uint32
             const uint32_t f = base + pp * 64 + lane;
             const uint32_t fc = f < tot ? f : tot - 1;
             const uint32_t e = fc / k, j = fc - e * k;
+            const uint32_t qr = qrow[e];
             v[pp] = X[(uint64_t)qcol[e] * a.ldx + j];
-            dst[pp] = f < tot ? (uint32_t)qrow[e] * S + j : 0xFFFFFFFFu;
+            dst[pp] = (f < tot && qr != 0xFFFFu) ? qr * S + j : 0xFFFFFFFFu;
         }
 #pragma unroll
         for (int pp = 0; pp < P; pp++)
@@ -145,142 +384,192 @@ __device__ __forceinline__ void drain_generic(const SpmmArgs &a, A *band, uint32
     }
 }
 
-template <typename T, int K, int U, int P>
-__global__ void __launch_bounds__(64 * kWPB) spmm_stream(SpmmArgs a)
+template <typename T, int K, typename A>
+__device__ __forceinline__ void write_band(const SpmmArgs &a, const A *band, uint32_t S, uint64_t row0,
+                                           uint32_t rows, uint32_t slot, int lane)
 {
-    extern __shared__ __align__(16) unsigned char smem[];
-    constexpr int QCAP = 64 * U;
-    const int lane = threadIdx.x & 63;
-    const uint32_t wv = uni(threadIdx.x >> 6);
-    const uint32_t item_id = blockIdx.x * kWPB + wv;
-    if (item_id >= a.nitems) return;
     const uint32_t k = K ? K : (uint32_t)a.k;
-
-    unsigned char *wbase = smem + wv * a.wave_bytes;
-    using A = typename AccOf<T>::type;
-    A *band = (A *)wbase;
-    uint32_t *qcol = (uint32_t *)(wbase + a.band_bytes);
-    uint16_t *qrow = (uint16_t *)(qcol + QCAP);
-
-    const Item &it = a.items[item_id];
-    const uint64_t brow_word = it.brow_word;
-    const uint32_t begin = it.begin, nwords = it.nwords;
-    const uint32_t dir_end = it.dir_end;
-    const uint64_t row0 = ((uint64_t)it.row0_hi << 32) | it.row0_lo;
-    const uint32_t sb_base = it.sb_base, rows = it.rows;
-    const uint32_t slot = it.slot;
-    int cur_row = (it.carry == kNone) ? -1 : (int)it.carry;
-    uint32_t ct = it.dir_begin;
-
-    const uint32_t S = a.band_stride;  // padded band row (elements)
-    for (uint32_t i = lane; i < rows * S; i += 64) band[i] = A(0);
-
-    const uint16_t *wp = a.img + brow_word;
-    const uint32_t pend = begin + nwords;  // nwords >= 1
-    const uint32_t plast = pend - 1;
-    uint2 cur = a.dir[ct];
-    cur.x = uni(cur.x);
-    cur.y = uni(cur.y);
-    const uint32_t dlast = dir_end - 1;
-    // directory window (lane j holds tile ct+1+j) and words of the first chunk
-    uint2 dj;
-    uint32_t wrd[U];
-    {
-        const uint32_t wj = ct + 1 + lane;
-        dj = a.dir[wj < dlast ? wj : dlast];
-        if (wj > dlast) dj = make_uint2(0xFFFFFFFFu, 0u);
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-            const uint32_t pos = begin + u * 64 + lane;
-            wrd[u] = wp[pos < plast ? pos : plast];
-        }
-    }
-    uint32_t p = begin;
-    while (p < pend) {
-        uint32_t qn = 0;
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-            const uint32_t p0 = p + u * 64;
-            if (p0 >= pend) break;
-            const uint32_t pos = p0 + lane;
-            const bool valid = pos < pend;
-            // tiles starting inside (p0, p0+64)
-            const int A = __popcll(__ballot(dj.x <= p0));
-            const int A2 = __popcll(__ballot(dj.x < p0 + 64));
-            uint64_t B = 0;
-            for (int j = A; j < A2; j++) B |= 1ull << (rdlane(dj.x, j) - p0);
-            uint32_t bstart = cur.x, bbc = cur.y;
-            if (A) { bstart = rdlane(dj.x, A - 1); bbc = rdlane(dj.y, A - 1); }
-            const int c = __popcll(B & mask_le(lane));
-            const int src = c ? (A + c - 1) : 0;
-            const uint32_t sx = __shfl(dj.x, src), sy = __shfl(dj.y, src);
-            const uint32_t tstart = c ? sx : bstart, tbc = c ? sy : bbc;
-            const uint32_t w = wrd[u];
-            const bool hdr = (pos - tstart) < 8u;
-            const bool head = valid && !hdr && (w & 0x8000u);
-            const bool col = valid && !hdr && !(w & 0x8000u);
-            const uint64_t hm = __ballot(head);
-            const uint64_t m = hm & mask_le(lane);
-            const int hl = m ? 63 - __clzll(m) : 0;
-            const int hrow = __shfl((int)(w & 0x7FFFu), hl);
-            const int row = m ? hrow : cur_row;
-            if (hm) cur_row = (int)rdlane(w & 0x7FFFu, 63 - __clzll(hm));
-            const uint32_t brow = (uint32_t)(row - (int)sb_base);
-            const uint32_t gcol = (tbc << a.log2w) | w;
-            const bool keep = col && row >= 0 && brow < rows && gcol < a.ncols;
-            const uint64_t cm = __ballot(keep);
-            if (keep) {
-                const uint32_t r = qn + __popcll(cm & mask_lt(lane));
-                qcol[r] = gcol;
-                qrow[r] = (uint16_t)brow;
-            }
-            qn += __popcll(cm);
-        }
-        // tile holding the next chunk's first word
-        const uint32_t pn = p + 64 * U;
-        const int An = __popcll(__ballot(dj.x <= pn));
-        if (An) {
-            cur.x = rdlane(dj.x, An - 1);
-            cur.y = rdlane(dj.y, An - 1);
-            ct += An;
-        }
-        // prefetch the next chunk (clamped, so always in bounds) before the
-        // gathers of this one
-        {
-            const uint32_t wj = ct + 1 + lane;
-            dj = a.dir[wj < dlast ? wj : dlast];
-            if (wj > dlast) dj = make_uint2(0xFFFFFFFFu, 0u);
-#pragma unroll
-            for (int u = 0; u < U; u++) {
-                const uint32_t pos = pn + u * 64 + lane;
-                wrd[u] = wp[pos < plast ? pos : plast];
-            }
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        if (qn) {
-            if (K) drain<T, (K ? K : 1), P, A>(a, band, S, qcol, qrow, qn, lane);
-            else drain_generic<T, P, A>(a, band, S, qcol, qrow, qn, lane);
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        p = pn;
-    }
-
-    // write the band (rounded once to T), or the partial slot in A
+    auto bcol = [&](uint32_t cc) -> uint32_t {
+        if constexpr (K > 0) return band_col<T, (K ? K : 1)>(cc);
+        else return cc;
+    };
     if (slot == kNone) {
         T *dst = (T *)a.Y + row0 * (uint64_t)a.ldy;
         for (uint32_t i = lane; i < rows * k; i += 64) {
             const uint32_t r = i / k, cc = i - r * k;
-            dst[r * (uint64_t)a.ldy + cc] = (T)band[r * S + cc];
+            dst[r * (uint64_t)a.ldy + cc] = (T)band[r * S + bcol(cc)];
         }
     } else {
         A *dst = (A *)a.partial + (uint64_t)slot * a.hs * k;
         for (uint32_t i = lane; i < rows * k; i += 64) {
             const uint32_t r = i / k, cc = i - r * k;
-            dst[i] = band[r * S + cc];
+            dst[i] = band[r * S + bcol(cc)];
         }
     }
+}
+
+struct ItemState {
+    ItemCtx ic;
+    uint32_t dlast, slot, ct;
+    uint64_t row0;
+    int carry;
+};
+
+__device__ __forceinline__ ItemState item_state(const SpmmArgs &a, const Item &it)
+{
+    ItemState s;
+    s.ic.bw = it.brow_word;
+    s.ic.begin = (int32_t)it.begin;
+    s.ic.pend = (int32_t)(it.begin + it.nwords);  // nwords >= 1
+    s.ic.sb_base = it.sb_base;
+    s.ic.rows = it.rows;
+    s.ic.log2w = a.log2w;
+    s.ic.ncols = a.ncols;
+    s.dlast = it.dir_end - 1;
+    s.row0 = ((uint64_t)it.row0_hi << 32) | it.row0_lo;
+    s.slot = it.slot;
+    s.carry = (it.carry == kNone) ? -1 : (int)it.carry;
+    s.ct = it.dir_begin;
+    return s;
+}
+
+// ---------------------------------------------------------------------------
+// Non-pipelined kernel: runtime widths (K = 0) and widths whose drain batch
+// is smaller than a chunk.  Chunks of 64*R words, drained batch by batch.
+template <typename T, int K, int R, int P>
+__global__ void __launch_bounds__(64 * kWPB) __attribute__((amdgpu_waves_per_eu(1, 2)))
+spmm_lane(SpmmArgs a)
+{
+    using LC = LaneCfg<R>;
+    constexpr int CW = LC::CW, DWIN = LC::DWIN, DPL = LC::DPL;
+    constexpr int SEGP = K ? P : (1 << 30);
+    constexpr int QS = CW + CW / (K ? P : CW) + 1;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int lane = threadIdx.x & 63;
+    const uint32_t wv = uni(threadIdx.x >> 6);
+    const uint32_t item_id = blockIdx.x * kWPB + wv;
+    if (item_id >= a.nitems) return;
+
+    unsigned char *wbase = smem + wv * a.wave_bytes;
+    using A = typename AccOf<T>::type;
+    A *band = (A *)wbase;
+    uint32_t *qcol = (uint32_t *)(wbase + a.band_bytes);
+    uint16_t *qrow = (uint16_t *)(qcol + QS);
+    uint2 *win = (uint2 *)(wbase + ((a.band_bytes + QS * 6 + 7) & ~7u));
+
+    ItemState st = item_state(a, a.items[item_id]);
+    const uint32_t S = a.band_stride;
+    for (uint32_t i = lane; i < st.ic.rows * S; i += 64) band[i] = A(0);
+    if (lane == 0) win[DWIN] = make_uint2(0xFFFFFFFFu, 0u);
+
+    const uint64_t abs_end = st.ic.bw + (uint64_t)st.ic.pend;
+    const uint64_t last8 = (abs_end - 1) & ~7ull;
+    uint64_t c = (st.ic.bw + (uint64_t)st.ic.begin) & ~7ull;
+    uint32_t wreg[R / 2];
+    uint2 dreg[DPL];
+    bool live[DPL];
+    load_words<R>(a.img, c, last8, lane, wreg);
+    load_window<DPL>(a.dir, st.ct, st.dlast, lane, dreg, live);
+
+    while (c < abs_end) {
+        store_window<DPL>(win, dreg, live, lane);
+        wave_sync();
+        const uint32_t qn = decode_chunk<R, SEGP>(st.ic, c, wreg, win, st.carry, qcol, qrow, lane);
+        const uint64_t cn = c + CW;
+        if (cn < abs_end) {
+            st.ct += window_advance<DPL>(dreg, live, (uint32_t)((int64_t)cn - (int64_t)st.ic.bw)) - 1;
+            load_words<R>(a.img, cn, last8, lane, wreg);
+            load_window<DPL>(a.dir, st.ct, st.dlast, lane, dreg, live);
+        }
+        wave_sync();
+        if constexpr (K > 0) {
+            using C = Lanes<T, K>;
+            for (uint32_t base = 0; base < qn; base += C::NZP * P) {
+                Vec<T, C::E> v[P];
+                uint32_t rr[P];
+                gather_batch<T, K, P, A>(a, qcol, qrow, base, qn, lane, v, rr);
+                accumulate_batch<T, K, P, A>(band, S, lane, v, rr);
+            }
+        } else {
+            if (qn) drain_generic<T, P, A>(a, band, S, qcol, qrow, qn, lane);
+        }
+        wave_sync();
+        c = cn;
+    }
+    write_band<T, K, A>(a, band, S, st.row0, st.ic.rows, st.slot, lane);
+}
+
+// Software-pipelined kernel for compile-time widths whose drain batch is a
+// whole chunk (64*R = NZP*P words): the gathers of chunk i are in flight
+// while chunk i+1 is decoded, and the words and window of chunk i+2 are
+// requested before they are consumed.  Every iteration issues the same loads
+// in the same order (past the item's end they are clamped and decode to no
+// entries), so the waits before the decode cover only the earlier requests.
+template <typename T, int K, int P>
+__global__ void __launch_bounds__(64 * kWPB) __attribute__((amdgpu_waves_per_eu(1, 2)))
+spmm_pipe(SpmmArgs a)
+{
+    using C = Lanes<T, K>;
+    constexpr int R = C::NZP * P / 64;
+    using LC = LaneCfg<R>;
+    constexpr int CW = LC::CW, DWIN = LC::DWIN, DPL = LC::DPL;
+    constexpr int QS = CW + CW / P + 1;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int lane = threadIdx.x & 63;
+    const uint32_t wv = uni(threadIdx.x >> 6);
+    const uint32_t item_id = blockIdx.x * kWPB + wv;
+    if (item_id >= a.nitems) return;
+
+    unsigned char *wbase = smem + wv * a.wave_bytes;
+    using A = typename AccOf<T>::type;
+    A *band = (A *)wbase;
+    uint32_t *qcol = (uint32_t *)(wbase + a.band_bytes);
+    uint16_t *qrow = (uint16_t *)(qcol + QS);
+    uint2 *win = (uint2 *)(wbase + ((a.band_bytes + QS * 6 + 7) & ~7u));
+
+    ItemState st = item_state(a, a.items[item_id]);
+    const uint32_t S = a.band_stride;
+    for (uint32_t i = lane; i < st.ic.rows * S; i += 64) band[i] = A(0);
+    if (lane == 0) win[DWIN] = make_uint2(0xFFFFFFFFu, 0u);
+
+    const uint64_t abs_end = st.ic.bw + (uint64_t)st.ic.pend;
+    const uint64_t last8 = (abs_end - 1) & ~7ull;
+    uint64_t c = (st.ic.bw + (uint64_t)st.ic.begin) & ~7ull;
+    uint32_t wreg[R / 2];
+    uint2 dreg[DPL];
+    bool live[DPL];
+    load_words<R>(a.img, c, last8, lane, wreg);
+    load_window<DPL>(a.dir, st.ct, st.dlast, lane, dreg, live);
+
+    // decodes chunk cc (its words and window are in registers) and requests
+    // chunk cc + CW
+    auto decode_step = [&](uint64_t cc) -> uint32_t {
+        store_window<DPL>(win, dreg, live, lane);
+        wave_sync();
+        const uint32_t n = decode_chunk<R, P>(st.ic, cc, wreg, win, st.carry, qcol, qrow, lane);
+        const uint64_t cn = cc + CW;
+        const uint32_t adv = window_advance<DPL>(dreg, live, (uint32_t)((int64_t)cn - (int64_t)st.ic.bw));
+        if (cn < abs_end) st.ct += adv - 1;
+        load_words<R>(a.img, cn, last8, lane, wreg);
+        load_window<DPL>(a.dir, st.ct, st.dlast, lane, dreg, live);
+        wave_sync();
+        return n;
+    };
+    uint32_t qn = decode_step(c);
+    while (true) {
+        Vec<T, C::E> v[P];
+        uint32_t rr[P];
+        gather_batch<T, K, P, typename AccOf<T>::type>(a, qcol, qrow, 0, qn, lane, v, rr);
+        c += CW;
+        const bool more = c < abs_end;
+        const uint32_t qn_next = decode_step(c);  // no entries past the item's end
+        accumulate_batch<T, K, P, A>(band, S, lane, v, rr);
+        if (!more) break;
+        qn = qn_next;
+        wave_sync();
+    }
+    wave_sync();
+    write_band<T, K, A>(a, band, S, st.row0, st.ic.rows, st.slot, lane);
 }
 
 // Y rows of a split band = partial slots summed in stream order.
@@ -299,6 +588,8 @@ __global__ void __launch_bounds__(256) spmm_reduce(const Split *splits, const A 
     }
 }
 
+// Gathers in flight per lane group: ~128 VGPRs of loaded data, and no more
+// entries than a 512-word chunk can hold.
 template <typename T, int K>
 constexpr int passes_for()
 {
@@ -308,23 +599,47 @@ constexpr int passes_for()
         constexpr int NZP = Lanes<T, K>::NZP;
         constexpr int E = Lanes<T, K>::E;
         constexpr int vgprs_per_pass = E * (int)sizeof(T) / 4;
-        constexpr int p = 128 / vgprs_per_pass;          // ~128 VGPRs of loads in flight
-        constexpr int pmax = (64 * 8) / NZP;              // no more than the queue
+        constexpr int p = 128 / vgprs_per_pass;
+        constexpr int pmax = 512 / NZP;
         return p < pmax ? (p < 1 ? 1 : p) : pmax;
     }
+}
+
+// The pipelined kernel needs a drain batch of a whole 512-word chunk.
+template <typename T, int K>
+constexpr bool use_pipe()
+{
+    if constexpr (K == 0) return false;
+    else return Lanes<T, K>::NZP * passes_for<T, K>() == 512;
+}
+
+constexpr int kLaneR = 16;  // words per lane in the non-pipelined kernel
+
+template <typename T, int K>
+uint32_t wave_bytes_for(uint32_t band_bytes)
+{
+    constexpr int P = passes_for<T, K>();
+    constexpr int R = use_pipe<T, K>() ? 8 : kLaneR;
+    constexpr int CW = 64 * R;
+    constexpr int QS = CW + CW / (K ? P : CW) + 1;
+    const uint32_t win_off = (band_bytes + QS * 6 + 7) & ~7u;
+    return (win_off + (LaneCfg<R>::DWIN + 1) * 8 + 15) & ~15u;
 }
 
 template <typename T, int K>
 fmx_status launch_k(fmx_ctx *ctx, SpmmArgs a, const Plan *plan, hipStream_t stream)
 {
-    constexpr int U = 8;
     constexpr int P = passes_for<T, K>();
-    auto kern = spmm_stream<T, K, U, P>;
+    void (*kern)(SpmmArgs);
+    if constexpr (use_pipe<T, K>()) kern = spmm_pipe<T, K, P>;
+    else kern = spmm_lane<T, K, kLaneR, P>;
+    a.wave_bytes = wave_bytes_for<T, K>(a.band_bytes);
     const uint32_t lds = a.wave_bytes * kWPB;
+    if (lds > ctx->lds_per_block) return fail(ctx, FMX_EUNSUPPORTED, "band does not fit in LDS");
     static bool attr_set = false;
     if (!attr_set) {
-        FMX_HIP(ctx, hipFuncSetAttribute((const void *)kern,
-                                         hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+        FMX_HIP(ctx, hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         160 * 1024));
         attr_set = true;
     }
     const uint32_t blocks = (plan->nitems + kWPB - 1) / kWPB;
@@ -340,7 +655,6 @@ fmx_status launch_t(fmx_ctx *ctx, const fmx_spm *spm, const Plan *plan, int k,
                     const void *X, int64_t ldx, void *Y, int64_t ldy, void *partial,
                     hipStream_t stream)
 {
-    constexpr int U = 8;
     SpmmArgs a;
     a.img = (const uint16_t *)spm->d_image;
     a.dir = spm->d_dir;
@@ -356,13 +670,10 @@ fmx_status launch_t(fmx_ctx *ctx, const fmx_spm *spm, const Plan *plan, int k,
     a.band_stride = (uint32_t)k + ((k >= 4 && getenv("FMX_NO_PAD") == nullptr) ? 1u : 0u);
     using A = typename AccOf<T>::type;
     a.band_bytes = (uint32_t)(((uint64_t)plan->hs * a.band_stride * sizeof(A) + 15) & ~15ull);
-    a.wave_bytes = a.band_bytes + 64 * U * 6;
-    a.wave_bytes = (a.wave_bytes + 15) & ~15u;
+    a.wave_bytes = 0;
     a.log2w = spm->log2w;
     a.k = k;
     a.ncols = spm->ncols;
-    if ((uint64_t)a.wave_bytes * kWPB > ctx->lds_per_block)
-        return fail(ctx, FMX_EUNSUPPORTED, "band does not fit in LDS");
     // 16-byte vector gathers need aligned rows; otherwise the runtime-width path.
     const int vb = (k * (int)sizeof(T) < 16) ? k * (int)sizeof(T) : 16;
     const bool aligned = ((uintptr_t)X % vb == 0) && ((ldx * (int64_t)sizeof(T)) % vb == 0);

@@ -69,6 +69,8 @@ SIGNATURES = {
     "fmx_spm_download": (ctypes.c_int, [_vp, _vp, ctypes.c_size_t,
                                         ctypes.POINTER(ctypes.c_size_t), _vp]),
     "fmx_spm_destroy": (ctypes.c_int, [_vp]),
+    "fmx_spm_load": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(_vp)]),
+    "fmx_spm_dump": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_char_p]),
     "fmx_spmm": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int, _vp, _i64, _vp, _i64, _vp]),
     "fmx_spmm_alg_bytes": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int,
                                           ctypes.POINTER(_u64)]),
@@ -202,6 +204,21 @@ class SparseMatrix:
         ctx.check(lib().fmx_spm_upload(ctx.handle, img.ctypes.data, img.nbytes, offs.ctypes.data,
                                        len(offs) - 1, ctypes.byref(h)), "fmx_spm_upload")
         return cls(ctx, h)
+
+    @classmethod
+    def load(cls, mat_file, idx_file, ctx=None):
+        """From a .mat image file and its .mat_idx index file
+        (SpM_2d_index::load + SpM_2d_storage::load + sparse_matrix::create)."""
+        ctx = ctx or default_context()
+        h = _vp()
+        ctx.check(lib().fmx_spm_load(ctx.handle, os.fsencode(mat_file), os.fsencode(idx_file), ctypes.byref(h)),
+                  "fmx_spm_load")
+        return cls(ctx, h)
+
+    def dump(self, mat_file, idx_file):
+        """Writes the .mat / .mat_idx pair (SpM_2d_storage::dump, SpM_2d_index::dump)."""
+        self.ctx.check(lib().fmx_spm_dump(self.handle, os.fsencode(mat_file), os.fsencode(idx_file)),
+                       "fmx_spm_dump")
 
     @classmethod
     def build(cls, src, dst, n=0, log2_h=15, log2_w=15, flags=BUILD_DEDUP, ctx=None):

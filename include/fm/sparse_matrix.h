@@ -21,6 +21,8 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <cstring>
+#include <algorithm>
 #include <limits>
 #include <memory>
 #include <stdexcept>
@@ -149,6 +151,118 @@ public:
     const void *get_raw_arr() const { return data_; }
 };
 
+/* I/O failure while loading (the reference throws safs::io_exception,
+ * matrix/sparse_matrix_format.cpp:262-276). */
+class io_exception : public std::runtime_error {
+public:
+    explicit io_exception(const std::string &msg) : std::runtime_error(msg) {}
+};
+
+namespace detail {
+inline std::vector<char> read_all(const std::string &file)
+{
+    FILE *f = fopen(file.c_str(), "rb");
+    if (!f) throw io_exception("can't open " + file);
+    std::vector<char> buf;
+    if (fseek(f, 0, SEEK_END) == 0) {
+        long n = ftell(f);
+        if (n > 0 && fseek(f, 0, SEEK_SET) == 0) {
+            buf.resize((size_t)n);
+            if (fread(buf.data(), (size_t)n, 1, f) != 1) buf.clear();
+        }
+    }
+    fclose(f);
+    if (buf.empty()) throw io_exception("can't read " + file);
+    return buf;
+}
+
+inline bool write_all(const std::string &file, const void *a, size_t na, const void *b, size_t nb)
+{
+    FILE *f = fopen(file.c_str(), "wb");
+    if (!f) {
+        log_error("can't open " + file);
+        return false;
+    }
+    bool ok = (na == 0 || fwrite(a, na, 1, f) == 1) && (nb == 0 || fwrite(b, nb, 1, f) == 1);
+    ok = (fclose(f) == 0) && ok;
+    if (!ok) log_error("can't write to " + file);
+    return ok;
+}
+}  // namespace detail
+
+/* SpM_2d_index (matrix/sparse_matrix_format.h:441-485): the matrix header
+ * page and the nbr+1 block-row offsets, as stored in a .mat_idx file. */
+class SpM_2d_index {
+    std::vector<char> header_;
+    std::vector<int64_t> offs_;
+
+public:
+    typedef std::shared_ptr<SpM_2d_index> ptr;
+
+    static ptr create(const std::vector<char> &header_page, const std::vector<int64_t> &offs)
+    {
+        ptr p(new SpM_2d_index());
+        p->header_.assign(header_page.begin(), header_page.begin() + std::min<size_t>(header_page.size(), 4096));
+        p->offs_ = offs;
+        return p;
+    }
+
+    /* SpM_2d_index::load (matrix/sparse_matrix_format.cpp:259-281). */
+    static ptr load(const std::string &idx_file)
+    {
+        std::vector<char> raw = detail::read_all(idx_file);
+        if (raw.size() < 4096 + 8 || (raw.size() - 4096) % 8) throw wrong_format("bad index file " + idx_file);
+        ptr p(new SpM_2d_index());
+        p->header_.assign(raw.begin(), raw.begin() + 4096);
+        p->offs_.resize((raw.size() - 4096) / 8);
+        memcpy(p->offs_.data(), raw.data() + 4096, p->offs_.size() * 8);
+        return p;
+    }
+
+    /* SpM_2d_index::dump (matrix/sparse_matrix_format.cpp:164-181). */
+    void dump(const std::string &file) const
+    {
+        detail::write_all(file, header_.data(), header_.size(), offs_.data(), offs_.size() * sizeof(int64_t));
+    }
+
+    size_t get_num_block_rows() const { return offs_.empty() ? 0 : offs_.size() - 1; }
+    int64_t get_block_row_off(size_t i) const { return offs_[i]; }
+    const std::vector<int64_t> &get_offs() const { return offs_; }
+    const std::vector<char> &get_header_page() const { return header_; }
+};
+
+/* SpM_2d_storage (matrix/sparse_matrix_format.h:488-...): the image bytes
+ * (header page, then the block rows) of a .mat file. */
+class SpM_2d_storage {
+    std::vector<char> data_;
+
+public:
+    typedef std::shared_ptr<SpM_2d_storage> ptr;
+
+    static ptr create(std::vector<char> image)
+    {
+        ptr p(new SpM_2d_storage());
+        p->data_ = std::move(image);
+        return p;
+    }
+
+    /* SpM_2d_storage::load (matrix/sparse_matrix_format.cpp:296-306); the
+     * index argument is kept for the reference's signature. */
+    static ptr load(const std::string &mat_file, SpM_2d_index::ptr index = SpM_2d_index::ptr())
+    {
+        (void)index;
+        return create(detail::read_all(mat_file));
+    }
+
+    /* SpM_2d_storage::dump (matrix/sparse_matrix_format.cpp:417-420). */
+    void dump(const std::string &mat_file) const
+    {
+        detail::write_all(mat_file, data_.data(), data_.size(), nullptr, 0);
+    }
+
+    const std::vector<char> &get_data() const { return data_; }
+};
+
 /* fm::sparse_matrix (matrix/sparse_matrix.h:475-651) for a 2D-tiled image. */
 class sparse_matrix {
     std::shared_ptr<fmx_spm> spm_;
@@ -177,6 +291,35 @@ public:
                            "sparse_matrix::create"))
             return ptr();
         return ptr(new sparse_matrix(s, true));
+    }
+
+    /* sparse_matrix::create(SpM_2d_index::ptr, SpM_2d_storage::ptr)
+     * (matrix/sparse_matrix.h:522, matrix/test/test-2d_multiply.cpp:147-161).
+     * The index header page must agree with the image's. */
+    static ptr create(SpM_2d_index::ptr index, SpM_2d_storage::ptr storage)
+    {
+        if (!index || !storage) return ptr();
+        const std::vector<char> &h = index->get_header_page(), &img = storage->get_data();
+        if (h.size() < 64 || img.size() < 64 || memcmp(h.data(), img.data(), 64) != 0)
+            throw wrong_format("index and matrix headers differ");
+        return create(img, index->get_offs());
+    }
+
+    static ptr create(SpM_2d_index::ptr index, SpM_2d_storage::ptr storage, SpM_2d_index::ptr t_index,
+                      SpM_2d_storage::ptr t_storage)
+    {
+        ptr a = create(index, storage), t = create(t_index, t_storage);
+        if (!a || !t) return ptr();
+        a->symmetric_ = t->symmetric_ = false;
+        a->transpose_ = t;
+        return a;
+    }
+
+    /* Writes this matrix as a .mat / .mat_idx pair (what the reference's
+     * converters do with SpM_2d_storage::dump and SpM_2d_index::dump). */
+    bool dump(const std::string &mat_file, const std::string &idx_file) const
+    {
+        return detail::check(fmx_spm_dump(spm_.get(), mat_file.c_str(), idx_file.c_str()), "sparse_matrix::dump");
     }
 
     /* The asymmetric form create(index, storage, t_index, t_storage)

@@ -106,6 +106,18 @@ fmx_status fmx_spm_download(const fmx_spm *spm, void *image, size_t cap,
                             size_t *bytes, int64_t *offs);
 fmx_status fmx_spm_destroy(fmx_spm *spm);
 
+/* The .mat / .mat_idx pair on disk.  fmx_spm_load replaces
+ * SpM_2d_index::load + SpM_2d_storage::load + sparse_matrix::create
+ * (matrix/sparse_matrix_format.cpp:238-306, matrix/test/test-2d_multiply.cpp:
+ * 147-161): the matrix file is the image (header page, then the block rows)
+ * and the index file is the same header page followed by the nbr+1 int64
+ * block-row offsets.  Both are validated as fmx_spm_upload does and the
+ * headers must agree.  fmx_spm_dump writes the same two files
+ * (SpM_2d_storage::dump, SpM_2d_index::dump, sparse_matrix_format.cpp:
+ * 164-181,417-420).  Unreadable or unwritable files give FMX_EINVAL. */
+fmx_status fmx_spm_load(fmx_ctx *ctx, const char *mat_file, const char *idx_file, fmx_spm **out);
+fmx_status fmx_spm_dump(const fmx_spm *spm, const char *mat_file, const char *idx_file);
+
 /* ---- SpMM ------------------------------------------------------------------
  * Y = A * X, replacing sparse_matrix::multiply(in, out)
  * (matrix/sparse_matrix.h:636-644, matrix/sparse_matrix.cpp:709-831) with

@@ -520,10 +520,13 @@ int orc_verify_image(const uint8_t *image, uint64_t bytes,
         const uint8_t *p = image + offs[br], *end = image + offs[br + 1];
         long prev_bc = -1;
         uint64_t rows_here = inf.nrows - br * inf.H < inf.H ? inf.nrows - br * inf.H : inf.H;
+        uint64_t ntiles = 0, nempty = 0;
         while (p < end) {
             tile_t tl;
             if ((rc = parse_tile(p, end, inf.esz, &tl))) return rc;
             if (tl.br != br) return ORC_EFORMAT;
+            ntiles++;
+            if (!tl.nnz) nempty++;
             if (tl.nnz) {
                 if ((long)tl.bc <= prev_bc) return ORC_EFORMAT;
                 if ((uint64_t)tl.bc * inf.W >= inf.ncols) return ORC_EFORMAT;
@@ -564,6 +567,9 @@ int orc_verify_image(const uint8_t *image, uint64_t bytes,
             p += tl.size;
         }
         if (p != end) return ORC_EFORMAT;
+        /* the writer stores no empty tiles, except one header for an empty
+         * block row (fm_utils.cpp:317,340-350) */
+        if (ntiles == 0 || (nempty && ntiles > 1)) return ORC_EFORMAT;
     }
     return ORC_OK;
 }

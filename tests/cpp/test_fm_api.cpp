@@ -4,6 +4,9 @@
 // list, multiply by a dense matrix, check the known answer A*[0..n-1].
 // Prints "OK" on success; exit status 0/1.
 #include <cstdio>
+#include <cstdlib>
+#include <string>
+#include <unistd.h>
 #include <vector>
 
 #include "fm/sparse_matrix.h"
@@ -48,6 +51,31 @@ int main()
             printf("mismatch at %u: %ld/%ld %ld/%ld\n", i, y[i], want[i], yt[i], want_t[i]);
             return 1;
         }
+    // the .mat / .mat_idx pair: dump, then load the way the reference's
+    // drivers do (matrix/test/test-2d_multiply.cpp:147-161)
+    {
+        char dir[] = "/tmp/fmx_cpp_XXXXXX";
+        if (!mkdtemp(dir)) return 1;
+        const std::string mat = std::string(dir) + "/a.mat", idx = std::string(dir) + "/a.mat_idx";
+        if (!A->dump(mat, idx)) return 1;
+        fm::SpM_2d_index::ptr index = fm::SpM_2d_index::load(idx);
+        fm::sparse_matrix::ptr B = fm::sparse_matrix::create(index, fm::SpM_2d_storage::load(mat, index));
+        if (!B || B->get_nnz() != A->get_nnz() || index->get_num_block_rows() != (n + 255) / 256) return 1;
+        fm::dense_matrix::ptr Yb = B->multiply(X);
+        std::vector<long> yb(n);
+        Yb->copy_to_host(yb.data());
+        if (yb != y) return 1;
+        bool io_threw = false;
+        try {
+            fm::SpM_2d_index::load(std::string(dir) + "/missing.mat_idx");
+        } catch (const fm::io_exception &) {
+            io_threw = true;
+        }
+        if (!io_threw) return 1;
+        remove(mat.c_str());
+        remove(idx.c_str());
+        rmdir(dir);
+    }
     // error convention: a dimension mismatch is false + log, not an exception
     fm::dense_matrix::ptr bad = fm::dense_matrix::create(n + 1, 1, FMX_I64);
     if (A->multiply(*bad, *Y)) return 1;

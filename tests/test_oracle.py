@@ -169,6 +169,8 @@ def test_verify_rejects_corruption(orc):
     assert orc.verify(bad) == 4
     bad = orc.Image(img.data.copy(), np.array([4096, 4192, 4146, 4208]), img.n)
     assert orc.verify(bad) == 4
+    bad = orc.Image(np.frombuffer(tiny_kat.BAD_EMPTY_IMAGE, np.uint8).copy(), tiny_kat.BAD_EMPTY_OFFS, tiny_kat.N)
+    assert orc.verify(bad) == 4  # empty tile in a non-empty block row
 
 
 def test_normalize_errors(orc):

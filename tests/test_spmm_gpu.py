@@ -178,6 +178,9 @@ def test_errors(fmx, orc):
     with pytest.raises(fmx.FmxError) as e:
         fmx.SparseMatrix.upload(tiny_kat.IMAGE, tiny_kat.OFFS[:-1])
     assert e.value.status == fmx.EFORMAT
+    with pytest.raises(fmx.FmxError) as e:
+        fmx.SparseMatrix.upload(tiny_kat.BAD_EMPTY_IMAGE, tiny_kat.BAD_EMPTY_OFFS)
+    assert e.value.status == fmx.EFORMAT
 
 
 def test_rmat_generator_matches_oracle(fmx, orc):
