@@ -63,6 +63,7 @@ def main():
     ap.add_argument("--log2-w", type=int, default=15)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-reps", type=int, default=5)
+    ap.add_argument("--no-eigen", action="store_true", help="skip the eigensolver legs")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -158,7 +159,53 @@ def main():
             "sample": "full image and X, 1 warm-up + median of %d multiply() calls, "
                       "seconds per SpMM %.4f" % (args.cpu_reps, t_cpu)}
         out["check"] = {"max_rel_err_vs_cpu": rel, "zero_rows_exact": zeros_ok}
+    if not args.no_eigen:
+        del spm, X, Y
+        torch.cuda.empty_cache()
+        out["eigen"] = eigen_leg(F, torch, args.log2_h, args.log2_w)
+        out["learned_warm_start"] = learned_leg(F, torch)
     print(json.dumps(out), flush=True)
+
+
+def eigen_leg(F, torch, log2_h, log2_w):
+    """Top-16 eigenpairs of D^-1/2 A D^-1/2 for the symmetrised scale-22
+    R-MAT graph: block 16, FP32 SpMM, BF16-MFMA first Gram pass, FP64
+    reorthogonalisation / Rayleigh-Ritz / residuals, tolerance 1e-6."""
+    n = 1 << SCALE
+    src, dst = F.rmat_edges(SCALE, EDGE_FACTOR * n, SEED)
+    spm = F.SparseMatrix.build(src, dst, n=n, log2_h=log2_h, log2_w=log2_w,
+                               flags=F.BUILD_DEDUP | F.BUILD_SYMMETRIZE)
+    del src, dst
+    F.compute_eigen(spm, nev=16, block_size=16, tol=1e-2, precision=F.EIG_MIXED, seed=3)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ev, vecs, res, st = F.compute_eigen(spm, nev=16, block_size=16, tol=1e-6, precision=F.EIG_MIXED,
+                                        seed=1, max_restarts=200)
+    torch.cuda.synchronize()
+    sec = time.perf_counter() - t0
+    log("eigen: %.3f s, %d block SpMMs, max residual %.2e" % (sec, st["spmm_blocks"], res.max()))
+    return {"workload": "rmat22_ef16 symmetrised, top-16 of D^-1/2 A D^-1/2, block 16, tol 1e-6",
+            "precision": "mixed (fp32 spmm, bf16-mfma gram, fp64 reorth/rr/residual)",
+            "sec": round(sec, 4), "spmm_blocks": int(st["spmm_blocks"]), "restarts": int(st["restarts"]),
+            "converged": int(st["converged"]), "max_resid_fp64": float(res.max()),
+            "eigenvalues_top3": [round(float(x), 8) for x in ev[:3]], "nnz": int(spm.info.nnz)}
+
+
+def learned_leg(F, torch):
+    """One held-out DC-SBM graph (2^20 vertices): seconds to residual 1e-6
+    from a random start and from the trained network's start."""
+    from flashx_amd import learned as Lm
+    if not os.path.exists(Lm.DEFAULT_WEIGHTS):
+        return {"status": "no trained weights at flashx_amd/weights/warmstart.npy"}
+    import numpy as np
+    net = Lm.WarmStartNet(params=np.load(Lm.DEFAULT_WEIGHTS))
+    summary, rows = Lm.evaluate(net, count=1, n=1 << 20, seed0=900000, tol=1e-6, log=log)
+    r = rows[0]
+    return {"workload": "DC-SBM n=2^20, 32 blocks, ratio 20, gamma 2.5, mean degree 16 (held-out seed 900000)",
+            "random_sec": round(r["random"]["sec"], 4), "learned_sec": round(r["learned"]["sec"], 4),
+            "random_spmm_blocks": r["random"]["spmm_blocks"], "learned_spmm_blocks": r["learned"]["spmm_blocks"],
+            "speedup": round(r["random"]["sec"] / r["learned"]["sec"], 3),
+            "max_resid": max(r["random"]["max_resid"], r["learned"]["max_resid"])}
 
 
 if __name__ == "__main__":
