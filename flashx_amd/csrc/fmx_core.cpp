@@ -287,6 +287,32 @@ fmx_status get_plan(fmx_ctx *ctx, fmx_spm *s, uint32_t hs, Plan **out)
     // Longest items first, so the hardware dispatcher approximates LPT.
     std::stable_sort(plan->items.begin(), plan->items.end(),
                      [](const Item &a, const Item &b) { return a.nwords > b.nwords; });
+    // Column rotation: item i (in dispatch order) starts at the first of its
+    // tiles in block column >= (i * nbc / slots) mod nbc and wraps around.
+    // With `slots` items resident at a time and similar speeds, the items
+    // running together then gather from the same few tile columns of X,
+    // which keeps that part of X in the caches.  The start depends only on
+    // the item's index, so results stay deterministic.
+    const char *re = getenv("FMX_ROT_SLOTS");
+    const uint64_t slots = re ? strtoull(re, nullptr, 10) : (uint64_t)ctx->num_cus * 7;
+    for (uint32_t i = 0; i < plan->nitems; i++) {
+        Item &t = plan->items[i];
+        t.rot_begin = t.begin;
+        t.rot_dir = t.dir_begin;
+        if (slots == 0 || s->nbc < 2) continue;
+        const uint32_t col = (uint32_t)(((uint64_t)i * s->nbc / slots) % s->nbc);
+        // first tile after dir_begin with block column >= col, inside the item
+        uint32_t lo = t.dir_begin + 1, hi = t.dir_end;
+        while (lo < hi) {
+            const uint32_t mid = lo + (hi - lo) / 2;
+            if (s->dir[mid].y < col) lo = mid + 1;
+            else hi = mid;
+        }
+        if (lo < t.dir_end && s->dir[lo].x < t.begin + t.nwords && s->dir[lo].x > t.begin) {
+            t.rot_begin = s->dir[lo].x;
+            t.rot_dir = lo;
+        }
+    }
     plan->nitems = (uint32_t)plan->items.size();
     plan->nslots = nslots;
     plan->nsplits = (uint32_t)splits.size();

@@ -40,8 +40,10 @@ struct Item {
     uint32_t rows;        // rows in the band
     uint32_t carry;
     uint32_t slot;        // partial-band slot, kNone = write Y directly
+    uint32_t rot_begin;   // the item is processed from this word (a tile start, or begin) ...
+    uint32_t rot_dir;     // ... whose directory index this is, to its end, then from begin
 };
-static_assert(sizeof(Item) == 48, "Item layout");
+static_assert(sizeof(Item) == 56, "Item layout");
 
 // A band split over several items: Y rows [row0, row0+rows) are the ordered
 // sum of partial slots [slot_begin, slot_begin+nslots).

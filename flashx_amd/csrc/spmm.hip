@@ -413,7 +413,29 @@ struct ItemState {
     uint32_t dlast, slot, ct;
     uint64_t row0;
     int carry;
+    int32_t begin, end, rot;  // the whole item, and where processing starts
+    uint32_t dir_begin, rot_dir;
+    int item_carry;
 };
+
+// Segment 0 is [rot, end) and segment 1 is [begin, rot) (empty without a
+// rotation).  A rotated start is a tile start, where no row is carried in.
+__device__ __forceinline__ bool select_segment(ItemState &st, int seg)
+{
+    if (seg == 0) {
+        st.ic.begin = st.rot;
+        st.ic.pend = st.end;
+        st.ct = st.rot_dir;
+        st.carry = st.rot == st.begin ? st.item_carry : -1;
+        return true;
+    }
+    if (st.rot == st.begin) return false;
+    st.ic.begin = st.begin;
+    st.ic.pend = st.rot;
+    st.ct = st.dir_begin;
+    st.carry = st.item_carry;
+    return true;
+}
 
 __device__ __forceinline__ ItemState item_state(const SpmmArgs &a, const Item &it)
 {
@@ -430,6 +452,12 @@ __device__ __forceinline__ ItemState item_state(const SpmmArgs &a, const Item &i
     s.slot = it.slot;
     s.carry = (it.carry == kNone) ? -1 : (int)it.carry;
     s.ct = it.dir_begin;
+    s.begin = (int32_t)it.begin;
+    s.end = (int32_t)(it.begin + it.nwords);
+    s.rot = (int32_t)it.rot_begin;
+    s.dir_begin = it.dir_begin;
+    s.rot_dir = it.rot_dir;
+    s.item_carry = s.carry;
     return s;
 }
 
@@ -462,6 +490,8 @@ spmm_lane(SpmmArgs a)
     for (uint32_t i = lane; i < st.ic.rows * S; i += 64) band[i] = A(0);
     if (lane == 0) win[DWIN] = make_uint2(0xFFFFFFFFu, 0u);
 
+    for (int seg = 0; seg < 2; seg++) {
+    if (!select_segment(st, seg)) break;
     const uint64_t abs_end = st.ic.bw + (uint64_t)st.ic.pend;
     const uint64_t last8 = (abs_end - 1) & ~7ull;
     uint64_t c = (st.ic.bw + (uint64_t)st.ic.begin) & ~7ull;
@@ -495,6 +525,7 @@ spmm_lane(SpmmArgs a)
         }
         wave_sync();
         c = cn;
+    }
     }
     write_band<T, K, A>(a, band, S, st.row0, st.ic.rows, st.slot, lane);
 }
@@ -532,14 +563,10 @@ spmm_pipe(SpmmArgs a)
     for (uint32_t i = lane; i < st.ic.rows * S; i += 64) band[i] = A(0);
     if (lane == 0) win[DWIN] = make_uint2(0xFFFFFFFFu, 0u);
 
-    const uint64_t abs_end = st.ic.bw + (uint64_t)st.ic.pend;
-    const uint64_t last8 = (abs_end - 1) & ~7ull;
-    uint64_t c = (st.ic.bw + (uint64_t)st.ic.begin) & ~7ull;
+    uint64_t abs_end, last8, c;
     uint32_t wreg[R / 2];
     uint2 dreg[DPL];
     bool live[DPL];
-    load_words<R>(a.img, c, last8, lane, wreg);
-    load_window<DPL>(a.dir, st.ct, st.dlast, lane, dreg, live);
 
     // decodes chunk cc (its words and window are in registers) and requests
     // chunk cc + CW
@@ -555,6 +582,13 @@ spmm_pipe(SpmmArgs a)
         wave_sync();
         return n;
     };
+    for (int seg = 0; seg < 2; seg++) {
+    if (!select_segment(st, seg)) break;
+    abs_end = st.ic.bw + (uint64_t)st.ic.pend;
+    last8 = (abs_end - 1) & ~7ull;
+    c = (st.ic.bw + (uint64_t)st.ic.begin) & ~7ull;
+    load_words<R>(a.img, c, last8, lane, wreg);
+    load_window<DPL>(a.dir, st.ct, st.dlast, lane, dreg, live);
     uint32_t qn = decode_step(c);
     while (true) {
         Vec<T, C::E> v[P];
@@ -569,6 +603,7 @@ spmm_pipe(SpmmArgs a)
         wave_sync();
     }
     wave_sync();
+    }
     write_band<T, K, A>(a, band, S, st.row0, st.ic.rows, st.slot, lane);
 }
 
