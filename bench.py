@@ -59,8 +59,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--log2-h", type=int, default=int(os.environ.get("FMX_BENCH_LOG2H", "7")))
+    ap.add_argument("--log2-h", type=int, default=None,
+                    help="tile height (log2); default: autotuned over 64/128/256 rows")
     ap.add_argument("--log2-w", type=int, default=15)
+    ap.add_argument("--large", action="store_true",
+                    help="also run the R-MAT scale-27, k=32 SpMM with a sampled CPU check")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-reps", type=int, default=5)
     ap.add_argument("--no-eigen", action="store_true", help="skip the eigensolver legs")
@@ -77,11 +80,18 @@ def main():
     ctx = F.default_context(0)
     n = 1 << SCALE
     m = EDGE_FACTOR * n
-    t0 = time.perf_counter()
+    from flashx_amd.tune import autotune_tile
     src, dst = F.rmat_edges(SCALE, m, SEED)
     torch.cuda.synchronize()
+    tune = None
+    if args.log2_h is None:
+        (args.log2_h, args.log2_w), tt = autotune_tile(src, dst, n, K, torch.float32)
+        tune = {"%dx%d" % (1 << a, 1 << b): round(v, 4) for (a, b), v in tt.items()}
+        log("autotune (ms per SpMM): %s" % tune)
+    t0 = time.perf_counter()
     spm = F.SparseMatrix.build(src, dst, n=n, log2_h=args.log2_h, log2_w=args.log2_w,
                                flags=F.BUILD_DEDUP)
+    torch.cuda.synchronize()
     t_build = time.perf_counter() - t0
     del src, dst
     info = spm.info
@@ -134,7 +144,8 @@ def main():
         "dtype": "f32",
         "data": "synthetic: seeded R-MAT (a,b,c,d)=(.57,.19,.19,.05), X U[0,1) seeded",
         "config": {"workload": "spmm " + workload, "scale": SCALE, "edge_factor": EDGE_FACTOR,
-                   "k": K, "tile": tile, "nnz": int(info.nnz),
+                   "k": K, "tile": tile, "tile_autotune_ms": tune, "nnz": int(info.nnz),
+                   "build_s": round(t_build, 3),
                    "image_bytes": int(info.image_bytes), "alg_bytes_per_step": int(alg_bytes),
                    "pct_hbm_peak": round(100.0 * value / PEAK_HBM_GBS, 2)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS,
@@ -159,12 +170,74 @@ def main():
             "sample": "full image and X, 1 warm-up + median of %d multiply() calls, "
                       "seconds per SpMM %.4f" % (args.cpu_reps, t_cpu)}
         out["check"] = {"max_rel_err_vs_cpu": rel, "zero_rows_exact": zeros_ok}
-    if not args.no_eigen:
+    if not args.no_eigen or args.large:
         del spm, X, Y
         torch.cuda.empty_cache()
+    if args.large:
+        out["large"] = large_leg(F, torch)
+        torch.cuda.empty_cache()
+    if not args.no_eigen:
         out["eigen"] = eigen_leg(F, torch, args.log2_h, args.log2_w)
         out["learned_warm_start"] = learned_leg(F, torch)
     print(json.dumps(out), flush=True)
+
+
+def large_leg(F, torch, scale=27, k=32, steps=5):
+    """R-MAT scale 27 (2^27 vertices, 2^31 edges), X 2^27 x 32 FP32, all in
+    HBM; tile height autotuned; FP32 check of a seeded sample of >= 1M rows
+    (whole block rows) against the oracle's reference-order CPU SpMM."""
+    import numpy as np
+    from oracle import oracle as O
+    from flashx_amd.tune import autotune_tile
+    n = 1 << scale
+    g = torch.Generator(device="cuda")
+    g.manual_seed(scale)
+    src, dst = F.rmat_edges(scale, EDGE_FACTOR * n, scale)
+    X = torch.rand((n, k), dtype=torch.float32, device="cuda", generator=g)
+    (lh, lw), tt = autotune_tile(src, dst, n, k, torch.float32, candidates=((7, 15), (8, 15)), reps=2, X=X)
+    t0 = time.perf_counter()
+    spm = F.SparseMatrix.build(src, dst, n=n, log2_h=lh, log2_w=lw, flags=F.BUILD_DEDUP)
+    torch.cuda.synchronize()
+    t_build = time.perf_counter() - t0
+    del src, dst
+    Y = torch.empty((n, k), dtype=torch.float32, device="cuda")
+    for _ in range(2):
+        spm.multiply_async(X, Y)
+    torch.cuda.synchronize()
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(steps):
+        spm.multiply_async(X, Y)
+    b.record()
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / steps
+    alg = spm.alg_bytes(torch.float32, k)
+    H = 1 << lh
+    rng = np.random.default_rng(scale)
+    nb = (1 << 20) // H
+    br0 = int(rng.integers(0, spm.info.nbr - nb))
+    data, offs = spm.download()
+    img = O.Image(data, offs, n)
+    Xh = X.cpu().numpy()
+    t0 = time.perf_counter()
+    ref = O.spmm_range(img, Xh, br0, br0 + nb)
+    t_cpu = time.perf_counter() - t0
+    ref64 = O.spmm_range(img, Xh.astype(np.float64), br0, br0 + nb)
+    del Xh
+    got = Y[br0 * H:(br0 + nb) * H].cpu().numpy().astype(np.float64)
+    nzm = ref64 != 0
+    rel = float((np.abs(got - ref)[nzm] / np.abs(ref[nzm])).max())
+    rel64 = float((np.abs(got - ref64)[nzm] / np.abs(ref64[nzm])).max())
+    log("large: %.3f ms per SpMM, %.1f GB/s, sample rel err %.2e" % (ms, alg / ms / 1e6, rel))
+    return {"workload": "spmm rmat%d_ef%d_f32_k%d" % (scale, EDGE_FACTOR, k), "nnz": int(spm.info.nnz),
+            "image_bytes": int(spm.info.image_bytes), "tile": "%dx%d" % (H, 1 << lw),
+            "tile_autotune_ms": {"%dx%d" % (1 << p, 1 << q): round(v, 3) for (p, q), v in tt.items()},
+            "build_s": round(t_build, 2), "ms_per_step": round(ms, 4), "GBps": round(alg / ms / 1e6, 1),
+            "pct_hbm_peak": round(100 * alg / ms / 1e6 / PEAK_HBM_GBS, 2),
+            "sample": {"rows": nb * H, "first_row": br0 * H, "max_rel_err_vs_cpu_fp32": rel,
+                       "max_rel_err_vs_fp64": rel64,
+                       "cpu_sec_single_thread": round(t_cpu, 2)}}
 
 
 def eigen_leg(F, torch, log2_h, log2_w):

@@ -1,0 +1,43 @@
+"""Tile-shape autotuner for the SpMM path.
+
+The tile height H trades image size (taller tiles: fewer tile headers, more
+rows per row part) against the LDS band each wave holds (H rows x k
+accumulators) and hence the number of resident waves.  The tuner builds the
+matrix on the device for each candidate (log2 H, log2 W), times a few SpMMs
+with the caller's width and type, and returns the fastest.  Everything it
+runs is the product path (fmx_spm_build, fmx_spmm)."""
+import time
+
+from . import fmx as F
+
+DEFAULT_CANDIDATES = ((6, 15), (7, 15), (8, 15))
+
+
+def _time_spmm(spm, X, Y, reps):
+    torch = F._torch()
+    spm.multiply_async(X, Y)
+    torch.cuda.synchronize()
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        spm.multiply_async(X, Y)
+    b.record()
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / reps
+
+
+def autotune_tile(src, dst, n, k, dtype, flags=F.BUILD_DEDUP, candidates=DEFAULT_CANDIDATES, reps=3, X=None):
+    """Returns ((log2_h, log2_w), {candidate: ms per SpMM}) for the edge list
+    src, dst (device tensors) multiplied by an n x k operand of `dtype`."""
+    torch = F._torch()
+    if X is None:
+        X = torch.rand((n, k), device=src.device, dtype=torch.float32).to(dtype)
+    Y = torch.empty((n, k), device=src.device, dtype=dtype)
+    times = {}
+    for lh, lw in candidates:
+        spm = F.SparseMatrix.build(src, dst, n=n, log2_h=lh, log2_w=lw, flags=flags)
+        times[(lh, lw)] = _time_spmm(spm, X, Y, reps)
+        del spm
+    best = min(times, key=times.get)
+    return best, times

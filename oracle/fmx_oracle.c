@@ -843,6 +843,34 @@ static void *mt_worker(void *arg)
     return NULL;
 }
 
+/* orc_spmm restricted to block rows [br0, br1): Y's row 0 is global row
+ * br0*H.  Used to check a sample of a matrix too large for a full CPU pass. */
+int orc_spmm_range(const uint8_t *image, uint64_t bytes, const int64_t *offs,
+                   uint64_t nbr, uint64_t br0, uint64_t br1, int dtype, int k,
+                   const void *X, int64_t ldx, void *Y, int64_t ldy)
+{
+    img_info_t inf;
+    int rc = read_info(image, bytes, offs, nbr, &inf);
+    if (rc) return rc;
+    int es = elem_size(dtype);
+    if (!es) return ORC_ETYPE;
+    if (k <= 0 || ldx < k || ldy < k || br0 > br1 || br1 > nbr) return ORC_EDIM;
+    for (uint64_t br = br0; br < br1; br++) {
+        uint64_t r0 = br * inf.H;
+        uint64_t nr = inf.nrows - r0 < inf.H ? inf.nrows - r0 : inf.H;
+        uint8_t *band = (uint8_t *)Y + (r0 - br0 * inf.H) * ldy * es;
+        for (uint64_t r = 0; r < nr; r++) memset(band + r * ldy * es, 0, (size_t)k * es);
+        const uint8_t *p = image + offs[br], *end = image + offs[br + 1];
+        while (p < end) {
+            tile_t tl;
+            if ((rc = parse_tile(p, end, inf.esz, &tl))) return rc;
+            run_tile_any(dtype, &tl, &inf, k, X, ldx, band, ldy);
+            p += tl.size;
+        }
+    }
+    return ORC_OK;
+}
+
 int orc_spmm_mt(const uint8_t *image, uint64_t bytes, const int64_t *offs,
                 uint64_t nbr, int dtype, int k, const void *X, int64_t ldx,
                 void *Y, int64_t ldy, int nthreads, int copy_block_rows)

@@ -81,6 +81,9 @@ int orc_spmm(const uint8_t *image, uint64_t bytes, const int64_t *offs,
              uint64_t nbr, int dtype, int k, const void *X, int64_t ldx,
              void *Y, int64_t ldy);
 /* Multi-threaded restatement of the reference engine, timed as baseline. */
+int orc_spmm_range(const uint8_t *image, uint64_t bytes, const int64_t *offs,
+                   uint64_t nbr, uint64_t br0, uint64_t br1, int dtype, int k,
+                   const void *X, int64_t ldx, void *Y, int64_t ldy);
 int orc_spmm_mt(const uint8_t *image, uint64_t bytes, const int64_t *offs,
                 uint64_t nbr, int dtype, int k, const void *X, int64_t ldx,
                 void *Y, int64_t ldy, int nthreads, int copy_block_rows);

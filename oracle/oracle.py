@@ -207,6 +207,26 @@ def spmm(image, X, threads=0, copy=True):
     return Y
 
 
+def spmm_range(image, X, br0, br1):
+    """Rows of block rows [br0, br1) of A*X, in the reference order."""
+    X = np.ascontiguousarray(X)
+    dt = _NP2DT[X.dtype]
+    H = int(image.data[48:56].view(np.uint64)[0])
+    n_rows = int(image.data[24:32].view(np.uint64)[0])
+    rows = min(br1 * H, n_rows) - br0 * H
+    k = X.shape[1]
+    Y = np.empty((rows, k), X.dtype)
+    L = lib()
+    L.orc_spmm_range.restype = ctypes.c_int
+    L.orc_spmm_range.argtypes = [_u8p, ctypes.c_uint64, _i64p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64,
+                                 ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int64, _vp, ctypes.c_int64]
+    rc = L.orc_spmm_range(_p(image.data, _u8p), len(image.data), _p(image.offs, _i64p), image.nbr, br0, br1,
+                          dt, k, X.ctypes.data, k, Y.ctypes.data, k)
+    if rc:
+        raise ValueError("orc_spmm_range failed with status %d" % rc)
+    return Y
+
+
 def gram_f64(A, B):
     A = np.ascontiguousarray(A, np.float64)
     B = np.ascontiguousarray(B, np.float64)

@@ -239,3 +239,12 @@ def test_row_32767_header_collides_with_end_marker(orc):
     Y = orc.spmm(img, X)
     assert list(Y[32767]) == list(X[5] + X[9] + X[40000])
     assert list(Y[32766]) == list(X[3]) and list(Y[0]) == list(X[32767])
+
+
+def test_spmm_range_matches_full(orc):
+    s, d = orc.rmat_edges(12, 16 << 12, 3)
+    img = orc.build_2d(s, d, n=1 << 12, log2h=7, log2w=10)
+    X = np.random.default_rng(0).random((img.n, 8))
+    Y = orc.spmm(img, X)
+    for br0, br1 in ((0, 1), (5, 9), (30, 32)):
+        assert np.array_equal(orc.spmm_range(img, X, br0, br1), Y[br0 * 128:br1 * 128])
