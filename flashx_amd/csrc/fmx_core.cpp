@@ -295,7 +295,7 @@ fmx_status get_plan(fmx_ctx *ctx, fmx_spm *s, uint32_t hs, Plan **out)
     // the item's index, so results stay deterministic.
     const char *re = getenv("FMX_ROT_SLOTS");
     const uint64_t slots = re ? strtoull(re, nullptr, 10) : (uint64_t)ctx->num_cus * 7;
-    for (uint32_t i = 0; i < plan->nitems; i++) {
+    for (size_t i = 0; i < plan->items.size(); i++) {
         Item &t = plan->items[i];
         t.rot_begin = t.begin;
         t.rot_dir = t.dir_begin;
@@ -312,6 +312,15 @@ fmx_status get_plan(fmx_ctx *ctx, fmx_spm *s, uint32_t hs, Plan **out)
             t.rot_begin = s->dir[lo].x;
             t.rot_dir = lo;
         }
+    }
+    // The kernel trusts these invariants; check them before anything runs.
+    for (const Item &t : plan->items) {
+        const bool ok = t.nwords >= 1 && t.dir_begin < t.dir_end && t.dir_end <= s->dir.size() &&
+                        s->dir[t.dir_begin].x <= t.begin && t.rot_begin >= t.begin &&
+                        t.rot_begin < t.begin + t.nwords && t.rot_dir >= t.dir_begin && t.rot_dir < t.dir_end &&
+                        (t.rot_begin == t.begin ? t.rot_dir == t.dir_begin : s->dir[t.rot_dir].x == t.rot_begin) &&
+                        t.rows >= 1 && t.rows <= hs;
+        if (!ok) return fail(ctx, FMX_EINVAL, "internal error: inconsistent SpMM work item");
     }
     plan->nitems = (uint32_t)plan->items.size();
     plan->nslots = nslots;

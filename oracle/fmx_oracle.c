@@ -293,23 +293,24 @@ static uint64_t tile_bytes(const binfo_t *b, uint64_t esz)
     return TILE_HDR + 2ULL * b->nrow + 2ULL * b->nnz + 2 + esz * b->nnz;
 }
 
-int orc_build_2d(const uint32_t *src, const uint32_t *dst, uint64_t m,
-                 uint64_t n, int log2h, int log2w, unsigned flags,
-                 uint8_t **image_out, uint64_t *image_bytes,
-                 int64_t **offs_out, uint64_t *nbr_out, uint64_t *n_out,
-                 uint64_t *nnz_out)
+/* Writes the image of the sorted, normalised keys (src << 32 | dst); vals,
+ * when esz > 0, holds esz bytes per key in the same order and becomes each
+ * tile's values region (SCSR values in SCSR order, then COO values,
+ * sparse_matrix_format.h:233-241,294-299).  Takes ownership of keys/vals. */
+static int build_2d_impl(uint64_t *keys, uint64_t nk, uint8_t *vals, uint64_t esz, int prim,
+                         uint64_t n, int log2h, int log2w,
+                         uint8_t **image_out, uint64_t *image_bytes,
+                         int64_t **offs_out, uint64_t *nbr_out, uint64_t *n_out,
+                         uint64_t *nnz_out)
 {
-    if (log2h < 0 || log2h > 15 || log2w < 0 || log2w > 15) return ORC_EINVAL;
-    uint64_t *keys = NULL, nk = 0;
-    int rc = orc_normalize(src, dst, m, n, flags, &keys, &nk, &n);
-    if (rc) return rc;
+    int rc = ORC_OK;
     const uint64_t H = 1ULL << log2h, W = 1ULL << log2w;
     const uint64_t nbr = (n + H - 1) / H, nbc = (n + W - 1) / W;
 
     binfo_t *bi = (binfo_t *)calloc(nbc, sizeof(binfo_t));
     int64_t *offs = (int64_t *)malloc((nbr + 1) * sizeof(int64_t));
     uint64_t *brow_lo = (uint64_t *)malloc((nbr + 1) * sizeof(uint64_t));
-    if (!bi || !offs || !brow_lo) { free(bi); free(offs); free(brow_lo); free(keys); return ORC_ENOMEM; }
+    if (!bi || !offs || !brow_lo) { free(bi); free(offs); free(brow_lo); free(keys); free(vals); return ORC_ENOMEM; }
 
     /* Key range of each block row (keys are sorted by src). */
     {
@@ -341,7 +342,7 @@ int orc_build_2d(const uint32_t *src, const uint32_t *dst, uint64_t m,
         }
         uint64_t bytes = 0;
         for (uint64_t c = 0; c < nbc; c++)
-            if (bi[c].nnz) bytes += tile_bytes(&bi[c], 0);
+            if (bi[c].nnz) bytes += tile_bytes(&bi[c], esz);
         if (bytes == 0) bytes = TILE_HDR; /* empty block row, fm_utils.cpp:341-351 */
         offs[br] = (int64_t)total;
         total += bytes;
@@ -351,12 +352,18 @@ int orc_build_2d(const uint32_t *src, const uint32_t *dst, uint64_t m,
     uint8_t *img = (uint8_t *)malloc(total);
     if (!img) { rc = ORC_ENOMEM; goto fail; }
     write_header(img, n, n, H, W);
+    put_u64(img + 16, esz);
+    put_u32(img + 44, (uint32_t)prim);
 
     /* Pass 2: serialise tiles (run_blocks + add_nz + finalize). */
     {
         uint64_t *scsr_p = (uint64_t *)malloc(nbc * sizeof(uint64_t));
         uint64_t *coo_p = (uint64_t *)malloc(nbc * sizeof(uint64_t));
-        if (!scsr_p || !coo_p) { free(scsr_p); free(coo_p); free(img); rc = ORC_ENOMEM; goto fail; }
+        uint64_t *scsr_v = (uint64_t *)malloc(nbc * sizeof(uint64_t));
+        uint64_t *coo_v = (uint64_t *)malloc(nbc * sizeof(uint64_t));
+        if (!scsr_p || !coo_p || !scsr_v || !coo_v) {
+            free(scsr_p); free(coo_p); free(scsr_v); free(coo_v); free(img); rc = ORC_ENOMEM; goto fail;
+        }
         for (uint64_t br = 0; br < nbr; br++) {
             uint8_t *base = img + offs[br];
             uint64_t lo = brow_lo[br], hi = brow_lo[br + 1];
@@ -388,7 +395,9 @@ int orc_build_2d(const uint32_t *src, const uint32_t *dst, uint64_t m,
                 uint64_t rheader = 2ULL * (bi[c].nrow - bi[c].ncoo) + 2ULL * (bi[c].nnz - bi[c].ncoo) + 2;
                 scsr_p[c] = offs[br] + cur + TILE_HDR;
                 coo_p[c] = offs[br] + cur + TILE_HDR + rheader;
-                cur += tile_bytes(&bi[c], 0);
+                scsr_v[c] = offs[br] + cur + TILE_HDR + 2ULL * bi[c].nrow + 2ULL * bi[c].nnz + 2;
+                coo_v[c] = scsr_v[c] + esz * (bi[c].nnz - bi[c].ncoo);
+                cur += tile_bytes(&bi[c], esz);
             }
             i = lo;
             while (i < hi) {
@@ -400,12 +409,14 @@ int orc_build_2d(const uint32_t *src, const uint32_t *dst, uint64_t m,
                     put_u16(img + coo_p[bc], (uint16_t)(rrel | 0x8000));
                     put_u16(img + coo_p[bc] + 2, (uint16_t)((uint32_t)keys[i] & (W - 1)));
                     coo_p[bc] += 4;
+                    if (esz) { memcpy(img + coo_v[bc], vals + i * esz, esz); coo_v[bc] += esz; }
                 } else {
                     put_u16(img + scsr_p[bc], (uint16_t)(rrel | 0x8000));
                     scsr_p[bc] += 2;
                     for (uint64_t q = i; q < j; q++) {
                         put_u16(img + scsr_p[bc], (uint16_t)((uint32_t)keys[q] & (W - 1)));
                         scsr_p[bc] += 2;
+                        if (esz) { memcpy(img + scsr_v[bc], vals + q * esz, esz); scsr_v[bc] += esz; }
                     }
                 }
                 i = j;
@@ -415,11 +426,14 @@ int orc_build_2d(const uint32_t *src, const uint32_t *dst, uint64_t m,
         }
         free(scsr_p);
         free(coo_p);
+        free(scsr_v);
+        free(coo_v);
     }
 
     free(bi);
     free(brow_lo);
     free(keys);
+    free(vals);
     *image_out = img;
     *image_bytes = total;
     *offs_out = offs;
@@ -432,7 +446,90 @@ fail:
     free(offs);
     free(brow_lo);
     free(keys);
+    free(vals);
     return rc;
+}
+
+int orc_build_2d(const uint32_t *src, const uint32_t *dst, uint64_t m,
+                 uint64_t n, int log2h, int log2w, unsigned flags,
+                 uint8_t **image_out, uint64_t *image_bytes,
+                 int64_t **offs_out, uint64_t *nbr_out, uint64_t *n_out,
+                 uint64_t *nnz_out)
+{
+    if (log2h < 0 || log2h > 15 || log2w < 0 || log2w > 15) return ORC_EINVAL;
+    uint64_t *keys = NULL, nk = 0;
+    int rc = orc_normalize(src, dst, m, n, flags, &keys, &nk, &n);
+    if (rc) return rc;
+    return build_2d_impl(keys, nk, NULL, 0, 0, n, log2h, log2w, image_out, image_bytes, offs_out, nbr_out,
+                         n_out, nnz_out);
+}
+
+typedef struct { uint64_t key, idx; } kidx_t;
+
+static int kidx_cmp(const void *a, const void *b)
+{
+    const kidx_t *x = (const kidx_t *)a, *y = (const kidx_t *)b;
+    if (x->key != y->key) return x->key < y->key ? -1 : 1;
+    return x->idx < y->idx ? -1 : (x->idx > y->idx);
+}
+
+/* Weighted image (attr_apply_operate::run_row, fm_utils.cpp:484-535):
+ * per row, neighbours sorted ascending with their esz-byte values; self
+ * edges dropped unless KEEP_SELF; with DEDUP one value per (row, column).
+ * The reference sorts with std::sort, which leaves the order of equal
+ * columns (and so the value kept) unspecified; this restatement keeps
+ * input order among equal keys and keeps the first.  SYMMETRIZE adds
+ * (dst, src) with the same value right after (src, dst). */
+int orc_build_2d_w(const uint32_t *src, const uint32_t *dst, const void *vals, uint64_t esz, int prim,
+                   uint64_t m, uint64_t n, int log2h, int log2w, unsigned flags,
+                   uint8_t **image_out, uint64_t *image_bytes,
+                   int64_t **offs_out, uint64_t *nbr_out, uint64_t *n_out, uint64_t *nnz_out)
+{
+    if (log2h < 0 || log2h > 15 || log2w < 0 || log2w > 15) return ORC_EINVAL;
+    if (esz != 4 && esz != 8) return ORC_EINVAL;
+    uint64_t maxid = 0;
+    for (uint64_t i = 0; i < m; i++) {
+        if (src[i] > maxid) maxid = src[i];
+        if (dst[i] > maxid) maxid = dst[i];
+    }
+    if (n == 0) {
+        if (m == 0) return ORC_EINVAL;
+        n = maxid + 1;
+    } else if (m > 0 && maxid >= n) {
+        return ORC_EDIM;
+    }
+    const int sym = (flags & ORC_SYMMETRIZE) != 0, keep_self = (flags & ORC_KEEP_SELF) != 0;
+    const uint64_t cap = sym ? 2 * m : m;
+    kidx_t *kv = (kidx_t *)malloc((cap ? cap : 1) * sizeof(kidx_t));
+    if (!kv) return ORC_ENOMEM;
+    uint64_t c = 0;
+    for (uint64_t i = 0; i < m; i++) {
+        uint32_t s = src[i], d = dst[i];
+        if (s == d && !keep_self) continue;
+        kv[c].key = ((uint64_t)s << 32) | d;
+        kv[c].idx = sym ? 2 * i : i;
+        c++;
+        if (sym && s != d) {
+            kv[c].key = ((uint64_t)d << 32) | s;
+            kv[c].idx = 2 * i + 1;
+            c++;
+        }
+    }
+    qsort(kv, c, sizeof(kidx_t), kidx_cmp);
+    uint64_t *keys = (uint64_t *)malloc((c ? c : 1) * sizeof(uint64_t));
+    uint8_t *sv = (uint8_t *)malloc((c ? c : 1) * esz);
+    if (!keys || !sv) { free(kv); free(keys); free(sv); return ORC_ENOMEM; }
+    uint64_t o = 0;
+    for (uint64_t i = 0; i < c; i++) {
+        if ((flags & ORC_DEDUP) && o > 0 && keys[o - 1] == kv[i].key) continue;
+        keys[o] = kv[i].key;
+        const uint64_t e = sym ? kv[i].idx / 2 : kv[i].idx;
+        memcpy(sv + o * esz, (const uint8_t *)vals + e * esz, esz);
+        o++;
+    }
+    free(kv);
+    return build_2d_impl(keys, o, sv, esz, prim, n, log2h, log2w, image_out, image_bytes, offs_out, nbr_out,
+                         n_out, nnz_out);
 }
 
 /* ------------------------------------------------------------------------ */

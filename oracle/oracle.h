@@ -59,6 +59,12 @@ int orc_build_2d(const uint32_t *src, const uint32_t *dst, uint64_t m,
                  uint8_t **image, uint64_t *image_bytes,
                  int64_t **offs, uint64_t *nbr, uint64_t *n_out,
                  uint64_t *nnz_out);
+/* Weighted form: esz = 4 or 8 value bytes per edge, prim the header's
+ * prim_type (4 P_INTEGER, 6 P_LONG, 8 P_FLOAT, 9 P_DOUBLE). */
+int orc_build_2d_w(const uint32_t *src, const uint32_t *dst, const void *vals, uint64_t esz, int prim,
+                   uint64_t m, uint64_t n, int log2h, int log2w, unsigned flags,
+                   uint8_t **image_out, uint64_t *image_bytes,
+                   int64_t **offs_out, uint64_t *nbr_out, uint64_t *n_out, uint64_t *nnz_out);
 void orc_free(void *p);
 
 /* Sorted, normalised edge list (the rows fed to the tile writer). */

@@ -150,6 +150,39 @@ def build_2d(src, dst, n=0, log2h=15, log2w=15, flags=DEDUP):
     return Image(data, o, n_out.value, nnz.value)
 
 
+_PRIM = {np.dtype(np.int32): 4, np.dtype(np.int64): 6, np.dtype(np.float32): 8, np.dtype(np.float64): 9}
+
+
+def build_2d_w(src, dst, vals, n=0, log2h=15, log2w=15, flags=DEDUP):
+    """Weighted image: vals (int32/int64/float32/float64, one per edge)
+    become the tiles' values regions; duplicates keep the first value."""
+    src = np.ascontiguousarray(src, dtype=np.uint32)
+    dst = np.ascontiguousarray(dst, dtype=np.uint32)
+    vals = np.ascontiguousarray(vals)
+    prim = _PRIM[vals.dtype]
+    L = lib()
+    L.orc_build_2d_w.restype = ctypes.c_int
+    L.orc_build_2d_w.argtypes = [_u32p, _u32p, _vp, ctypes.c_uint64, ctypes.c_int, ctypes.c_uint64,
+                                 ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_uint,
+                                 ctypes.POINTER(_u8p), _u64p, ctypes.POINTER(_i64p), _u64p, _u64p, _u64p]
+    img = _u8p()
+    nbytes = ctypes.c_uint64()
+    offs = _i64p()
+    nbr = ctypes.c_uint64()
+    n_out = ctypes.c_uint64()
+    nnz = ctypes.c_uint64()
+    rc = L.orc_build_2d_w(_p(src, _u32p), _p(dst, _u32p), vals.ctypes.data, vals.itemsize, prim, len(src), n,
+                          log2h, log2w, flags, ctypes.byref(img), ctypes.byref(nbytes), ctypes.byref(offs),
+                          ctypes.byref(nbr), ctypes.byref(n_out), ctypes.byref(nnz))
+    if rc:
+        raise ValueError("orc_build_2d_w failed with status %d" % rc)
+    data = np.ctypeslib.as_array(img, shape=(nbytes.value,)).copy()
+    o = np.ctypeslib.as_array(offs, shape=(nbr.value + 1,)).copy()
+    L.orc_free(ctypes.cast(img, _vp))
+    L.orc_free(ctypes.cast(offs, _vp))
+    return Image(data, o, n_out.value, nnz.value)
+
+
 def normalize(src, dst, n=0, flags=DEDUP):
     src = np.ascontiguousarray(src, dtype=np.uint32)
     dst = np.ascontiguousarray(dst, dtype=np.uint32)

@@ -248,3 +248,42 @@ def test_spmm_range_matches_full(orc):
     Y = orc.spmm(img, X)
     for br0, br1 in ((0, 1), (5, 9), (30, 32)):
         assert np.array_equal(orc.spmm_range(img, X, br0, br1), Y[br0 * 128:br1 * 128])
+
+
+def test_weighted_tiny_kat(orc):
+    """The weighted writer reproduces the hand-written weighted image, and
+    the reader multiplies with its values."""
+    img = orc.build_2d_w(tiny_kat.SRC, tiny_kat.DST, np.array(tiny_kat.W_VALS, np.float32), n=tiny_kat.N,
+                         log2h=tiny_kat.LOG2H, log2w=tiny_kat.LOG2W, flags=orc.DEDUP)
+    assert bytes(img.data) == tiny_kat.W_IMAGE
+    assert list(img.offs) == tiny_kat.W_OFFS
+    assert orc.verify(img) == 0
+    Y = orc.spmm(img, np.array(tiny_kat.X, np.float32))
+    assert Y.tolist() == tiny_kat.W_Y
+
+
+@pytest.mark.parametrize("dt,lh,lw,flags", [(np.float32, 5, 6, 1), (np.float64, 7, 4, 1 | 4),
+                                            (np.int32, 3, 9, 0), (np.int64, 9, 9, 1 | 2)])
+def test_weighted_against_brute_force(orc, dt, lh, lw, flags):
+    """Weighted images of R-MAT graphs: SpMM equals a dense brute force built
+    straight from the edge list (first value of a duplicate kept, self edges
+    per KEEP_SELF, (dst, src) added with the same value for SYMMETRIZE)."""
+    rng = np.random.default_rng(lh)
+    s, d = orc.rmat_edges(10, 8 << 10, lh)
+    v = (rng.random(len(s)) * 100).astype(dt)
+    img = orc.build_2d_w(s, d, v, n=1 << 10, log2h=lh, log2w=lw, flags=flags)
+    assert orc.verify(img) == 0
+    A = np.zeros((1 << 10, 1 << 10))
+    seen = set()
+    for a, b, w in zip(s.tolist(), d.tolist(), v.astype(np.float64).tolist()):
+        pairs = [(a, b)] + ([(b, a)] if flags & 4 and a != b else [])
+        for p in pairs:
+            if p[0] == p[1] and not flags & 2:
+                continue
+            if flags & 1:
+                if p in seen:
+                    continue
+                seen.add(p)
+            A[p] += w
+    X = (rng.random((1 << 10, 3)) * 8).astype(np.float64)
+    assert np.allclose(orc.spmm(img, X), A @ X, rtol=1e-12, atol=1e-9)
