@@ -83,8 +83,14 @@ fmx_status analyse_image(fmx_ctx *ctx, fmx_spm *s)
         return fail(ctx, FMX_EFORMAT, "tile dimensions must be powers of two <= 32768");
     if (s->nrows > (1ULL << 32) || s->ncols > (1ULL << 32))
         return fail(ctx, FMX_EFORMAT, "more than 2^32 rows or columns");
-    if (s->esz != 0)
-        return fail(ctx, FMX_EUNSUPPORTED, "weighted tile images (entry_size > 0) are not supported by this build");
+    // binary, or values of the header's primitive type: P_INTEGER / P_FLOAT
+    // (4 bytes), P_LONG / P_DOUBLE (8 bytes) (generic_type.h:57-70)
+    if (s->esz != 0) {
+        const bool ok4 = s->esz == 4 && (s->prim == 4 || s->prim == 8);
+        const bool ok8 = s->esz == 8 && (s->prim == 6 || s->prim == 9);
+        if (!ok4 && !ok8)
+            return fail(ctx, FMX_EUNSUPPORTED, "weighted images need 4-byte int/float or 8-byte long/double values");
+    }
     const uint64_t nbr = (s->nrows + s->H - 1) / s->H;
     if (s->offs.size() != nbr + 1)
         return fail(ctx, FMX_EFORMAT, "there are an incorrect number of offsets");
@@ -97,6 +103,7 @@ fmx_status analyse_image(fmx_ctx *ctx, fmx_spm *s)
             return fail(ctx, FMX_EFORMAT, "block-row offsets must be even and strictly increasing");
 
     s->dir.clear();
+    s->vdir.clear();
     s->brow_dir.assign(nbr + 1, 0);
     uint64_t nnz = 0;
     for (uint64_t br = 0; br < nbr; br++) {
@@ -121,7 +128,7 @@ fmx_status analyse_image(fmx_ctx *ctx, fmx_spm *s)
                 prev_bc = tbc;
                 if (tncoo > tnrow || tncoo > tnnz || tnrow > rows_here)
                     return fail(ctx, FMX_EFORMAT, "inconsistent tile counts");
-                size = 16 + 2ULL * tnrow + 2ULL * tnnz + 2;
+                size = 16 + 2ULL * tnrow + 2ULL * tnnz + 2 + s->esz * tnnz;
                 if ((uint64_t)(end - p) < size) return fail(ctx, FMX_EFORMAT, "tile runs past its block row");
                 const uint64_t cols_here = std::min<uint64_t>(s->W, s->ncols - (uint64_t)tbc * s->W);
                 const uint8_t *w = p + 16;
@@ -163,6 +170,7 @@ fmx_status analyse_image(fmx_ctx *ctx, fmx_spm *s)
                 nnz += tnnz;
             }
             s->dir.push_back(make_uint2((uint32_t)((p - base) / 2), tbc));
+            if (s->esz) s->vdir.push_back((uint32_t)((p - base) / 2 + 8 + tnrow + tnnz + 1));
             p += size;
         }
         if (p != end) return fail(ctx, FMX_EFORMAT, "block row size disagrees with its tiles");
@@ -264,7 +272,19 @@ fmx_status get_plan(fmx_ctx *ctx, fmx_spm *s, uint32_t hs, Plan **out)
         for (uint32_t i = 1; i < pieces; i++) {
             uint32_t p = (uint32_t)((uint64_t)b.words * i / pieces);
             if (p <= cut[i - 1]) p = cut[i - 1] + 1;
-            carries[i] = carry_at(s, b.br, p, dirs[i]);
+            if (s->esz) {
+                // weighted: cut at the tile start at or before p, so no row
+                // and no value position is carried into a piece
+                const uint32_t lo = s->brow_dir[b.br], hi = s->brow_dir[b.br + 1];
+                uint32_t t = (uint32_t)(std::upper_bound(s->dir.begin() + lo, s->dir.begin() + hi, p,
+                                                         [](uint32_t v, const uint2 &d) { return v < d.x; }) -
+                                        s->dir.begin()) - 1;
+                p = std::max(s->dir[t].x, cut[i - 1]);
+                dirs[i] = t;
+                carries[i] = kNone;
+            } else {
+                carries[i] = carry_at(s, b.br, p, dirs[i]);
+            }
             cut[i] = p;
         }
         cut[pieces] = b.words;
@@ -340,15 +360,25 @@ fmx_status get_plan(fmx_ctx *ctx, fmx_spm *s, uint32_t hs, Plan **out)
 
 // A device-built image: bring the host copy, validate it like an uploaded
 // one and publish the tile directory.
+// Tile directory (and, for weighted images, the values directory) to HBM.
+fmx_status upload_dirs(fmx_ctx *ctx, fmx_spm *s)
+{
+    FMX_HIP(ctx, hipMalloc(&s->d_dir, std::max<size_t>(1, s->dir.size()) * sizeof(uint2)));
+    FMX_HIP(ctx, hipMemcpy(s->d_dir, s->dir.data(), s->dir.size() * sizeof(uint2), hipMemcpyHostToDevice));
+    if (s->esz) {
+        FMX_HIP(ctx, hipMalloc(&s->d_vdir, std::max<size_t>(1, s->vdir.size()) * sizeof(uint32_t)));
+        FMX_HIP(ctx, hipMemcpy(s->d_vdir, s->vdir.data(), s->vdir.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    }
+    return FMX_OK;
+}
+
 fmx_status finish_device_image(fmx_ctx *ctx, fmx_spm *s)
 {
     s->h_image.resize(s->bytes);
     FMX_HIP(ctx, hipMemcpy(s->h_image.data(), s->d_image, s->bytes, hipMemcpyDeviceToHost));
     fmx_status st = analyse_image(ctx, s);
     if (st) return st;
-    FMX_HIP(ctx, hipMalloc(&s->d_dir, std::max<size_t>(1, s->dir.size()) * sizeof(uint2)));
-    FMX_HIP(ctx, hipMemcpy(s->d_dir, s->dir.data(), s->dir.size() * sizeof(uint2), hipMemcpyHostToDevice));
-    return FMX_OK;
+    return upload_dirs(ctx, s);
 }
 
 }  // namespace fmx
@@ -357,6 +387,7 @@ fmx_spm::~fmx_spm()
 {
     if (d_image) (void)hipFree(d_image);
     if (d_dir) (void)hipFree(d_dir);
+    if (d_vdir) (void)hipFree(d_vdir);
 }
 
 using namespace fmx;
@@ -445,8 +476,8 @@ fmx_status fmx_spm_upload(fmx_ctx *ctx, const void *image, size_t bytes,
     FMX_HIP(ctx, hipMalloc(&s->d_image, padded));
     FMX_HIP(ctx, hipMemcpy(s->d_image, s->h_image.data(), s->bytes, hipMemcpyHostToDevice));
     if (padded > s->bytes) FMX_HIP(ctx, hipMemset(s->d_image + s->bytes, 0, padded - s->bytes));
-    FMX_HIP(ctx, hipMalloc(&s->d_dir, std::max<size_t>(1, s->dir.size()) * sizeof(uint2)));
-    FMX_HIP(ctx, hipMemcpy(s->d_dir, s->dir.data(), s->dir.size() * sizeof(uint2), hipMemcpyHostToDevice));
+    st = upload_dirs(ctx, s.get());
+    if (st) return st;
     *out = s.release();
     return FMX_OK;
 }
@@ -577,6 +608,12 @@ fmx_status fmx_spmm(fmx_ctx *ctx, const fmx_spm *cspm, fmx_dtype dtype, int k,
     if (spm->ctx != ctx) return fail(ctx, FMX_EINVAL, "matrix belongs to another context");
     if (dtype != FMX_I32 && dtype != FMX_I64 && dtype != FMX_F32 && dtype != FMX_F64)
         return fail(ctx, FMX_ETYPE, "wrong matrix type in spmm");
+    // a weighted matrix multiplies dense data of its own value type
+    // (sparse_matrix.cpp:754-766)
+    if (spm->esz) {
+        const int want = dtype == FMX_I32 ? 4 : dtype == FMX_I64 ? 6 : dtype == FMX_F32 ? 8 : 9;
+        if (spm->prim != want) return fail(ctx, FMX_ETYPE, "wrong matrix type in spmm");
+    }
     if (k <= 0) return fail(ctx, FMX_EDIM, "k must be positive");
     if (ldx < k || ldy < k) return fail(ctx, FMX_EDIM, "leading dimension smaller than k");
     const int es = elem_size(dtype);

@@ -92,9 +92,11 @@ struct fmx_spm {
     int log2h = 0, log2w = 0, prim = 0;
     uint8_t *d_image = nullptr;         // image in HBM (padded to 4 KiB)
     uint2 *d_dir = nullptr;             // tile directory
+    uint32_t *d_vdir = nullptr;         // weighted images: values-region word of each tile
     std::vector<uint8_t> h_image;       // host copy used to build plans
     std::vector<int64_t> offs;          // block-row byte offsets
     std::vector<uint2> dir;             // host directory
+    std::vector<uint32_t> vdir;         // weighted images: values-region word (block-row relative)
     std::vector<uint32_t> brow_dir;     // first directory index per block row (+1)
     std::map<uint32_t, std::unique_ptr<fmx::Plan>> plans;
     fmx::Scratch partial;               // partial bands of split items

@@ -54,6 +54,8 @@ struct SpmmArgs {
     int log2w;
     int k;
     uint64_t ncols;
+    const uint32_t *vdir;  // weighted images: values-region word of each tile
+    int vwords;            // weighted images: u16 words per value (2 or 4)
 };
 
 template <typename T, int E>
@@ -607,6 +609,320 @@ spmm_pipe(SpmmArgs a)
     write_band<T, K, A>(a, band, S, st.row0, st.ic.rows, st.slot, lane);
 }
 
+
+// ---------------------------------------------------------------------------
+// Weighted images (entry_size 4 or 8): each tile's values follow its COO
+// region, one per non-zero in stream order (SCSR values, then COO values,
+// sparse_matrix_format.h:233-241).  The decode is decode_chunk's plus two
+// facts per entry: words at or past the tile's values region are not
+// entries, and the entry's value is the tile's (o - wq[j])-th, where o is
+// the entry's index in the chunk and wq[j] the chunk index at which window
+// tile j starts (minus the columns of tile 0 seen in earlier chunks, for
+// j = 0).  Work items start at tile starts (rotation and splits), so no
+// value position is carried into an item.
+template <int R, int SEGP>
+__device__ __forceinline__ uint32_t decode_chunk_w(const ItemCtx &ic, uint64_t c, const uint32_t *wreg,
+                                                   const uint2 *win, const uint32_t *wvb, int32_t *wq, int c0,
+                                                   int vwords, int &carry, uint32_t *qcol, uint16_t *qrow,
+                                                   uint32_t *qval, int lane)
+{
+    using LC = LaneCfg<R>;
+    constexpr int NB = LC::NB, DWIN = LC::DWIN;
+    const int32_t p0 = (int32_t)((int64_t)c - (int64_t)ic.bw) + lane * R;
+    const int32_t q = p0 > ic.begin ? p0 : ic.begin;
+    uint32_t j = 0;
+#pragma unroll
+    for (int st = DWIN / 2; st >= 1; st >>= 1)
+        if (win[j + st].x <= (uint32_t)q) j += st;
+    uint32_t ts0[NB], tb0[NB], tv0[NB];
+#pragma unroll
+    for (int b = 0; b < NB; b++) {
+        const uint32_t jj = j + b < DWIN ? j + b : DWIN;
+        const uint2 e = win[jj];
+        ts0[b] = e.x;
+        tb0[b] = e.y;
+        tv0[b] = wvb[jj];
+    }
+    // pass 1: counts, last row head, and the column count at each tile
+    // start inside this lane's words (a tile starting exactly at the lane's
+    // first item word counts too, unless it is window tile 0)
+    int cnt = 0, lrow = 0;
+    bool hh = false;
+    int nx = 0;
+    int xcnt[NB];
+    uint32_t xj[NB];
+    {
+        uint32_t ts[NB], tv[NB];
+#pragma unroll
+        for (int b = 0; b < NB; b++) {
+            ts[b] = ts0[b];
+            tv[b] = tv0[b];
+            xcnt[b] = 0;
+            xj[b] = 0;
+        }
+        if (j > 0 && ts0[0] == (uint32_t)q && q < ic.pend) {
+            xcnt[0] = 0;
+            xj[0] = j;
+            nx = 1;
+        }
+        uint32_t jc = j;
+#pragma unroll
+        for (int i = 0; i < R; i++) {
+            const int32_t pos = p0 + i;
+            const uint32_t w = (wreg[i >> 1] >> ((i & 1) * 16)) & 0xFFFFu;
+            const bool valid = pos >= ic.begin && pos < ic.pend;
+            if (NB > 1 && valid && (uint32_t)pos >= ts[NB > 1 ? 1 : 0]) {
+#pragma unroll
+                for (int b = 0; b + 1 < NB; b++) {
+                    ts[b] = ts[b + 1];
+                    tv[b] = tv[b + 1];
+                }
+                ts[NB - 1] = 0xFFFFFFFFu;
+                tv[NB - 1] = 0xFFFFFFFFu;
+                jc++;
+                if (nx < NB) {
+                    xcnt[nx] = cnt;
+                    xj[nx] = jc;
+                    nx++;
+                }
+            }
+            const bool body = valid && ((uint32_t)pos - ts[0]) >= 8u && (uint32_t)pos < tv[0];
+            if (body && (w & 0x8000u)) {
+                lrow = (int)(w & 0x7FFFu);
+                hh = true;
+            }
+            cnt += (body && !(w & 0x8000u)) ? 1 : 0;
+        }
+    }
+    const uint64_t hm = __ballot(hh);
+    const uint64_t below = hm & mask_lt(lane);
+    const int src = below ? 63 - __clzll(below) : 0;
+    const int prow = __shfl(lrow, src);
+    const int row_in = below ? prow : carry;
+    if (hm) carry = (int)rdlane((uint32_t)lrow, 63 - __clzll(hm));
+    int incl = cnt;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int t = __shfl_up(incl, d);
+        if (lane >= d) incl += t;
+    }
+    const uint32_t qn = rdlane((uint32_t)incl, 63);
+    const int base = incl - cnt;
+    if (lane == 0) wq[0] = -c0;
+#pragma unroll
+    for (int b = 0; b < NB; b++)
+        if (b < nx) wq[xj[b]] = base + xcnt[b];
+    wave_sync();
+    // pass 2: entries with their value words
+    {
+        uint32_t ts[NB], tb[NB], tv[NB];
+#pragma unroll
+        for (int b = 0; b < NB; b++) {
+            ts[b] = ts0[b];
+            tb[b] = tb0[b];
+            tv[b] = tv0[b];
+        }
+        int qt = wq[j];
+        int r = row_in;
+        uint32_t o = (uint32_t)base;
+#pragma unroll
+        for (int i = 0; i < R; i++) {
+            const int32_t pos = p0 + i;
+            const uint32_t w = (wreg[i >> 1] >> ((i & 1) * 16)) & 0xFFFFu;
+            const bool valid = pos >= ic.begin && pos < ic.pend;
+            if (NB > 1 && valid && (uint32_t)pos >= ts[NB > 1 ? 1 : 0]) {
+#pragma unroll
+                for (int b = 0; b + 1 < NB; b++) {
+                    ts[b] = ts[b + 1];
+                    tb[b] = tb[b + 1];
+                    tv[b] = tv[b + 1];
+                }
+                ts[NB - 1] = 0xFFFFFFFFu;
+                tv[NB - 1] = 0xFFFFFFFFu;
+                qt = (int)o;
+            }
+            const bool body = valid && ((uint32_t)pos - ts[0]) >= 8u && (uint32_t)pos < tv[0];
+            if (body && (w & 0x8000u)) r = (int)(w & 0x7FFFu);
+            if (body && !(w & 0x8000u)) {
+                const uint32_t br = (uint32_t)(r - (int)ic.sb_base);
+                const uint32_t gcol = (tb[0] << ic.log2w) | w;
+                const bool keep = r >= 0 && br < ic.rows && gcol < ic.ncols;
+                const uint32_t sl = qslot<SEGP>(o);
+                qcol[sl] = keep ? gcol : 0u;
+                qrow[sl] = keep ? (uint16_t)br : (uint16_t)0xFFFFu;
+                qval[sl] = tv[0] + (uint32_t)((int)o - qt) * (uint32_t)vwords;
+                o++;
+            }
+        }
+    }
+    return qn;
+}
+
+// One value of type T from the u16-aligned values region (u16 loads, so no
+// alignment beyond 2 bytes is assumed).
+template <typename T>
+__device__ __forceinline__ T load_value(const uint16_t *p)
+{
+    if constexpr (sizeof(T) == 4) {
+        const uint32_t u = (uint32_t)p[0] | ((uint32_t)p[1] << 16);
+        return __builtin_bit_cast(T, u);
+    } else {
+        const uint64_t u = (uint64_t)p[0] | ((uint64_t)p[1] << 16) | ((uint64_t)p[2] << 32) |
+                           ((uint64_t)p[3] << 48);
+        return __builtin_bit_cast(T, u);
+    }
+}
+
+template <typename T, int K, int R, int P>
+__global__ void __launch_bounds__(64 * kWPB) __attribute__((amdgpu_waves_per_eu(1, 2)))
+spmm_lane_w(SpmmArgs a)
+{
+    using LC = LaneCfg<R>;
+    constexpr int CW = LC::CW, DWIN = LC::DWIN, DPL = LC::DPL;
+    constexpr int SEGP = K ? P : (1 << 30);
+    constexpr int QS = CW + CW / (K ? P : CW) + 1;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int lane = threadIdx.x & 63;
+    const uint32_t wv = uni(threadIdx.x >> 6);
+    const uint32_t item_id = blockIdx.x * kWPB + wv;
+    if (item_id >= a.nitems) return;
+
+    unsigned char *wbase = smem + wv * a.wave_bytes;
+    using A = typename AccOf<T>::type;
+    A *band = (A *)wbase;
+    uint32_t *qcol = (uint32_t *)(wbase + a.band_bytes);
+    uint32_t *qval = qcol + QS;
+    uint16_t *qrow = (uint16_t *)(qval + QS);
+    const uint32_t win_off = (a.band_bytes + QS * 10 + 7) & ~7u;
+    uint2 *win = (uint2 *)(wbase + win_off);
+    uint32_t *wvb = (uint32_t *)(wbase + win_off + (DWIN + 1) * 8);
+    int32_t *wq = (int32_t *)(wbase + win_off + (DWIN + 1) * 12);
+
+    ItemState st = item_state(a, a.items[item_id]);
+    const uint32_t S = a.band_stride;
+    for (uint32_t i = lane; i < st.ic.rows * S; i += 64) band[i] = A(0);
+    if (lane == 0) {
+        win[DWIN] = make_uint2(0xFFFFFFFFu, 0u);
+        wvb[DWIN] = 0xFFFFFFFFu;
+    }
+    const T *X = (const T *)a.X;
+    const uint16_t *vimg = a.img + st.ic.bw;
+
+    for (int seg = 0; seg < 2; seg++) {
+    if (!select_segment(st, seg)) break;
+    const uint64_t abs_end = st.ic.bw + (uint64_t)st.ic.pend;
+    const uint64_t last8 = (abs_end - 1) & ~7ull;
+    uint64_t c = (st.ic.bw + (uint64_t)st.ic.begin) & ~7ull;
+    int c0 = 0;  // columns of window tile 0 decoded in earlier chunks
+    uint32_t wreg[R / 2];
+    uint2 dreg[DPL];
+    uint32_t vreg[DPL];
+    bool live[DPL];
+    load_words<R>(a.img, c, last8, lane, wreg);
+    load_window<DPL>(a.dir, st.ct, st.dlast, lane, dreg, live);
+#pragma unroll
+    for (int h = 0; h < DPL; h++) {
+        const uint32_t t = st.ct + h * 64 + lane;
+        vreg[h] = a.vdir[t < st.dlast ? t : st.dlast];
+    }
+
+    while (c < abs_end) {
+        store_window<DPL>(win, dreg, live, lane);
+#pragma unroll
+        for (int h = 0; h < DPL; h++) wvb[h * 64 + lane] = live[h] ? vreg[h] : 0xFFFFFFFFu;
+        wave_sync();
+        const uint32_t qn = decode_chunk_w<R, SEGP>(st.ic, c, wreg, win, wvb, wq, c0, a.vwords, st.carry,
+                                                    qcol, qrow, qval, lane);
+        const uint64_t cn = c + CW;
+        if (cn < abs_end) {
+            const uint32_t adv = window_advance<DPL>(dreg, live, (uint32_t)((int64_t)cn - (int64_t)st.ic.bw));
+            c0 = (int)qn - wq[adv - 1];
+            st.ct += adv - 1;
+            load_words<R>(a.img, cn, last8, lane, wreg);
+            load_window<DPL>(a.dir, st.ct, st.dlast, lane, dreg, live);
+#pragma unroll
+            for (int h = 0; h < DPL; h++) {
+                const uint32_t t = st.ct + h * 64 + lane;
+                vreg[h] = a.vdir[t < st.dlast ? t : st.dlast];
+            }
+        }
+        wave_sync();
+        if constexpr (K > 0) {
+            using C = Lanes<T, K>;
+            const int q = lane % C::L;
+            const int g = lane / C::L;
+            for (uint32_t base = 0; base < qn; base += C::NZP * P) {
+                Vec<T, C::E> v[P];
+                uint32_t rr[P], gcs[P];
+                T vals[P];
+#pragma unroll
+                for (int pp = 0; pp < P; pp++) {
+                    const uint32_t e = base + g * P + pp;
+                    const uint32_t sl = qslot<P>(e);
+                    const bool ok = e < qn;
+                    const uint32_t qr = qrow[sl];
+                    gcs[pp] = ok ? qcol[sl] : 0u;
+                    rr[pp] = (ok && qr != 0xFFFFu) ? qr : 0xFFFFFFFFu;
+                    const uint32_t vo = ok ? qval[sl] : 0u;  // a safe address when unused
+                    vals[pp] = load_value<T>(vimg + vo);
+                }
+#pragma unroll
+                for (int pp = 0; pp < P; pp++)
+                    v[pp] = *(const Vec<T, C::E> *)(X + (uint64_t)gcs[pp] * a.ldx + q * C::E);
+                A acc[C::E];
+                uint32_t cur = 0xFFFFFFFFu;
+#pragma unroll
+                for (int i = 0; i < C::E; i++) acc[i] = A(0);
+#pragma unroll
+                for (int pp = 0; pp < P; pp++) {
+                    if (rr[pp] != cur) {
+                        if (cur != 0xFFFFFFFFu) {
+                            A *dst = band + cur * S + q;
+#pragma unroll
+                            for (int i = 0; i < C::E; i++) lds_add(dst + i * C::L, acc[i]);
+                        }
+                        cur = rr[pp];
+#pragma unroll
+                        for (int i = 0; i < C::E; i++) acc[i] = (A)v[pp].v[i] * (A)vals[pp];
+                    } else {
+#pragma unroll
+                        for (int i = 0; i < C::E; i++) acc[i] += (A)v[pp].v[i] * (A)vals[pp];
+                    }
+                }
+                if (cur != 0xFFFFFFFFu) {
+                    A *dst = band + cur * S + q;
+#pragma unroll
+                    for (int i = 0; i < C::E; i++) lds_add(dst + i * C::L, acc[i]);
+                }
+            }
+        } else {
+            const uint32_t k = (uint32_t)a.k;
+            const uint32_t tot = qn * k;
+            for (uint32_t base = 0; base < tot; base += 64 * P) {
+                T xv[P], vv[P];
+                uint32_t dst[P];
+#pragma unroll
+                for (int pp = 0; pp < P; pp++) {
+                    const uint32_t f = base + pp * 64 + lane;
+                    const uint32_t fc = f < tot ? f : tot - 1;
+                    const uint32_t e = fc / k, jj = fc - e * k;
+                    const uint32_t qr = qrow[e];
+                    xv[pp] = X[(uint64_t)qcol[e] * a.ldx + jj];
+                    vv[pp] = load_value<T>(vimg + qval[e]);
+                    dst[pp] = (f < tot && qr != 0xFFFFu) ? qr * S + jj : 0xFFFFFFFFu;
+                }
+#pragma unroll
+                for (int pp = 0; pp < P; pp++)
+                    if (dst[pp] != 0xFFFFFFFFu) lds_add(band + dst[pp], (A)xv[pp] * (A)vv[pp]);
+            }
+        }
+        wave_sync();
+        c = cn;
+    }
+    }
+    write_band<T, K, A>(a, band, S, st.row0, st.ic.rows, st.slot, lane);
+}
+
 // Y rows of a split band = partial slots summed in stream order.
 template <typename T, typename A>
 __global__ void __launch_bounds__(256) spmm_reduce(const Split *splits, const A *partial,
@@ -662,8 +978,35 @@ uint32_t wave_bytes_for(uint32_t band_bytes)
 }
 
 template <typename T, int K>
+fmx_status launch_kw(fmx_ctx *ctx, SpmmArgs a, const Plan *plan, hipStream_t stream)
+{
+    constexpr int P = passes_for<T, K>();
+    constexpr int R = kLaneR;
+    constexpr int CW = 64 * R;
+    constexpr int QS = CW + CW / (K ? P : CW) + 1;
+    auto kern = spmm_lane_w<T, K, R, P>;
+    const uint32_t win_off = (a.band_bytes + QS * 10 + 7) & ~7u;
+    a.wave_bytes = (win_off + (LaneCfg<R>::DWIN + 1) * 16 + 15) & ~15u;
+    const uint32_t lds = a.wave_bytes * kWPB;
+    if (lds > ctx->lds_per_block) return fail(ctx, FMX_EUNSUPPORTED, "band does not fit in LDS");
+    static bool attr_set = false;
+    if (!attr_set) {
+        FMX_HIP(ctx, hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         160 * 1024));
+        attr_set = true;
+    }
+    const uint32_t blocks = (plan->nitems + kWPB - 1) / kWPB;
+    if (blocks) {
+        hipLaunchKernelGGL(kern, dim3(blocks), dim3(64 * kWPB), lds, stream, a);
+        FMX_HIP(ctx, hipGetLastError());
+    }
+    return FMX_OK;
+}
+
+template <typename T, int K>
 fmx_status launch_k(fmx_ctx *ctx, SpmmArgs a, const Plan *plan, hipStream_t stream)
 {
+    if (a.vdir) return launch_kw<T, K>(ctx, a, plan, stream);
     constexpr int P = passes_for<T, K>();
     void (*kern)(SpmmArgs);
     if constexpr (use_pipe<T, K>()) kern = spmm_pipe<T, K, P>;
@@ -709,6 +1052,8 @@ fmx_status launch_t(fmx_ctx *ctx, const fmx_spm *spm, const Plan *plan, int k,
     a.log2w = spm->log2w;
     a.k = k;
     a.ncols = spm->ncols;
+    a.vdir = spm->esz ? spm->d_vdir : nullptr;
+    a.vwords = (int)(spm->esz / 2);
     // 16-byte vector gathers need aligned rows; otherwise the runtime-width path.
     const int vb = (k * (int)sizeof(T) < 16) ? k * (int)sizeof(T) : 16;
     const bool aligned = ((uintptr_t)X % vb == 0) && ((ldx * (int64_t)sizeof(T)) % vb == 0);

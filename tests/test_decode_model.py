@@ -133,3 +133,150 @@ def test_lane_decode_model(orc, scale, lh, lw, R):
             ref, _ = seq_decode(words[bs:be], d, a0, a1, carry, lw)
             got = lane_decode(words, bs, d, a0, a1, carry, lw, R)
             assert got == ref, (br, a0, a1)
+
+
+# ---------------------------------------------------------------------------
+# Weighted images: decode_chunk_w (values region skipped; value index of an
+# entry = its chunk index minus the chunk index at which its tile starts).
+
+def tiles_of_w(img_words, bstart, bend, vw):
+    """(start_rel, bc, values_rel) per tile of one weighted block row."""
+    d = []
+    p = bstart
+    while p < bend:
+        bc = img_words[p + 2] | (img_words[p + 3] << 16)
+        nnz = img_words[p + 4] | (img_words[p + 5] << 16)
+        nrow = img_words[p + 6]
+        if nnz == 0:
+            d.append((p - bstart, bc, p - bstart + 8))
+            p += 8
+            continue
+        vb = p - bstart + 8 + nrow + nnz + 1
+        d.append((p - bstart, bc, vb))
+        p += 8 + nrow + nnz + 1 + vw * nnz
+    assert p == bend
+    return d
+
+
+def seq_decode_w(words, d, begin, pend, log2w, vw):
+    out = []
+    row = -1
+    ti = 0
+    nz_in_tile = 0
+    for pos in range(begin, pend):
+        while ti + 1 < len(d) and d[ti + 1][0] <= pos:
+            ti += 1
+            nz_in_tile = 0
+        ts, bc, vb = d[ti]
+        if pos - ts < 8 or pos >= vb:
+            continue
+        w = words[pos]
+        if w & 0x8000:
+            row = w & 0x7FFF
+        else:
+            out.append((row, (bc << log2w) | w, vb + nz_in_tile * vw))
+            nz_in_tile += 1
+    return out
+
+
+def lane_decode_w(words_abs, bw, d, begin, pend, log2w, R, vw):
+    CW = 64 * R
+    NB = 1 + (R + 10) // 11
+    DWIN = ((CW - 1) // 11 + 2 + 63) // 64 * 64
+    SENT = (0xFFFFFFFF, 0, 0xFFFFFFFF)
+    ct = max(i for i, e in enumerate(d) if e[0] <= begin)
+    dlast = len(d) - 1
+    out = []
+    carry, c0 = -1, 0
+    c = (bw + begin) & ~7
+    while c < bw + pend:
+        win = [d[ct + j] if ct + j <= dlast else SENT for j in range(DWIN)] + [SENT]
+        lanes = []
+        for l in range(64):
+            p0 = (c - bw) + l * R
+            q = max(p0, begin)
+            j = 0
+            st = DWIN // 2
+            while st >= 1:
+                if win[j + st][0] <= q:
+                    j += st
+                st //= 2
+            lanes.append((p0, q, j, [win[min(j + b, DWIN)] for b in range(NB)]))
+        wq = {0: -c0}
+        cnt, lrow, hh, xs = [], [], [], []
+        for l, (p0, q, j, t0) in enumerate(lanes):
+            t = list(t0)
+            n, lr, h, jc = 0, 0, False, j
+            x = [(0, j)] if (j > 0 and t0[0][0] == q and q < pend) else []
+            for i in range(R):
+                pos, a = p0 + i, c + l * R + i
+                w = words_abs[a] if a < len(words_abs) else 0
+                valid = begin <= pos < pend
+                if NB > 1 and valid and pos >= t[1][0]:
+                    t = t[1:] + [SENT]
+                    jc += 1
+                    x.append((n, jc))
+                body = valid and pos - t[0][0] >= 8 and pos < t[0][2]
+                if body and w & 0x8000:
+                    lr, h = w & 0x7FFF, True
+                if body and not w & 0x8000:
+                    n += 1
+            cnt.append(n); lrow.append(lr); hh.append(h); xs.append(x)
+        base = np.concatenate([[0], np.cumsum(cnt)[:-1]]).astype(int)
+        for l in range(64):
+            for n_before, jj in xs[l]:
+                wq[jj] = int(base[l]) + n_before
+        rows_in = []
+        for l in range(64):
+            below = [m for m in range(l) if hh[m]]
+            rows_in.append(lrow[below[-1]] if below else carry)
+        anyh = [m for m in range(64) if hh[m]]
+        if anyh:
+            carry = lrow[anyh[-1]]
+        for l, (p0, q, j, t0) in enumerate(lanes):
+            t = list(t0)
+            r, o, qt = rows_in[l], int(base[l]), wq.get(j)  # unused by lanes without entries
+            for i in range(R):
+                pos, a = p0 + i, c + l * R + i
+                w = words_abs[a] if a < len(words_abs) else 0
+                valid = begin <= pos < pend
+                if NB > 1 and valid and pos >= t[1][0]:
+                    t = t[1:] + [SENT]
+                    qt = o
+                body = valid and pos - t[0][0] >= 8 and pos < t[0][2]
+                if body and w & 0x8000:
+                    r = w & 0x7FFF
+                if body and not w & 0x8000:
+                    assert qt is not None, "value base of this tile was never published"
+                    out.append((r, (t[0][1] << log2w) | w, t[0][2] + (o - qt) * vw))
+                    o += 1
+        qn = int(sum(cnt))
+        cn = c + CW
+        adv = sum(1 for e in win[:DWIN] if e[0] <= cn - bw)
+        if cn < bw + pend:
+            c0 = qn - wq[adv - 1]
+            ct += adv - 1
+        c = cn
+    return out
+
+
+@pytest.mark.parametrize("dt,lh,lw,R", [(np.float32, 5, 6, 16), (np.float64, 7, 10, 16), (np.int32, 4, 4, 16),
+                                        (np.int64, 9, 9, 16)])
+def test_weighted_lane_decode_model(orc, dt, lh, lw, R):
+    rng = np.random.default_rng(lh * 7 + lw)
+    s, dd = orc.rmat_edges(10, 16 << 10, lh)
+    v = (rng.random(len(s)) * 10).astype(dt)
+    img = orc.build_2d_w(s, dd, v, n=1 << 10, log2h=lh, log2w=lw, flags=orc.DEDUP)
+    vw = np.dtype(dt).itemsize // 2
+    words = np.frombuffer(img.data, dtype=np.uint16)
+    offs = img.offs
+    for br in range(0, len(offs) - 1, max(1, (len(offs) - 1) // 16)):
+        bs, be = offs[br] // 2, offs[br + 1] // 2
+        d = tiles_of_w(words, bs, be, vw)
+        # items start at tile starts (weighted splits and rotations)
+        starts = [e[0] for e in d]
+        cuts = sorted(set([0, be - bs] + [starts[int(x)] for x in rng.integers(0, len(starts), size=3)]))
+        for a0, a1 in zip(cuts[:-1], cuts[1:]):
+            ref = seq_decode_w(words[bs:be], d, a0, a1, lw, vw)
+            got = lane_decode_w(words, bs, d, a0, a1, lw, R, vw)
+            assert got == ref, (br, a0, a1)
