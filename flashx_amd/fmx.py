@@ -69,6 +69,8 @@ SIGNATURES = {
     "fmx_spm_download": (ctypes.c_int, [_vp, _vp, ctypes.c_size_t,
                                         ctypes.POINTER(ctypes.c_size_t), _vp]),
     "fmx_spm_destroy": (ctypes.c_int, [_vp]),
+    "fmx_pagerank": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_float, _vp,
+                                    ctypes.POINTER(ctypes.c_int), _vp]),
     "fmx_spm_load": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(_vp)]),
     "fmx_spm_dump": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_char_p]),
     "fmx_spmm": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int, _vp, _i64, _vp, _i64, _vp]),
@@ -417,6 +419,19 @@ def compute_eigen(spm, nev=16, block_size=16, num_blocks=6, tol=1e-6, max_restar
               "fmx_eigen_normalized")
     stats = {f: getattr(st, f) for f, _ in fmx_eigen_stats._fields_}
     return evals, vecs, resid, stats
+
+
+def pagerank(A, At=None, max_iters=30, damping=0.85):
+    """PageRank (matrix/libmatrix-algs/PageRank.cpp): returns (pr float32
+    device tensor, iterations).  At is A's transpose; None for symmetric A."""
+    torch = _torch()
+    ctx = A.ctx
+    pr = torch.empty(A.get_num_rows(), dtype=torch.float32, device="cuda:%d" % ctx.device)
+    it = ctypes.c_int()
+    ctx.check(lib().fmx_pagerank(ctx.handle, A.handle, At.handle if At is not None else None, max_iters,
+                                 damping, pr.data_ptr(), ctypes.byref(it), ctx.stream_ptr()), "fmx_pagerank")
+    torch.cuda.synchronize()
+    return pr, it.value
 
 
 def dcsbm_params(n, nblocks=32, gamma=2.5, size_decay=1.06):

@@ -409,6 +409,25 @@ public:
 
 };
 
+/* fm::PageRank (matrix/libmatrix-algs/PageRank.cpp:78-137): returns the
+ * n x 1 FP32 rank vector, or null.  num_in_mem is kept for the reference's
+ * signature; everything is resident in HBM. */
+inline dense_matrix::ptr PageRank(sparse_matrix::ptr mat, size_t max_niters, float damping_factor,
+                                  size_t num_in_mem = 1)
+{
+    (void)num_in_mem;
+    if (!mat || mat->get_num_rows() != mat->get_num_cols()) return dense_matrix::ptr();
+    dense_matrix::ptr pr = dense_matrix::create(mat->get_num_rows(), 1, FMX_F32);
+    sparse_matrix::ptr t = mat->transpose();
+    int iters = 0;
+    if (!pr || !t ||
+        !detail::check(fmx_pagerank(detail::global_ctx(), mat->handle(), t->handle(), (int)max_niters,
+                                    damping_factor, (float *)pr->get_raw_arr(), &iters, nullptr),
+                       "PageRank"))
+        return dense_matrix::ptr();
+    return pr;
+}
+
 }  // namespace fm
 
 #endif

@@ -207,6 +207,15 @@ fmx_status fmx_eigen_normalized(fmx_ctx *ctx, const fmx_spm *spm, const fmx_eige
                                 double *evals, double *evecs, int64_t ldv, double *resid,
                                 fmx_eigen_stats *stats);
 
+/* ---- SpMV callers ------------------------------------------------------------
+ * PageRank as matrix/libmatrix-algs/PageRank.cpp:49-137 computes it (FP32):
+ * out_deg = max(A*1, 1), pr = 1/n; each iteration in = pr/out_deg,
+ * out = At*in, pr = damping*out + (1-damping)/n, until every vertex moved by
+ * less than 0.001/n or max_iters.  At is the transpose of A (NULL: A is
+ * symmetric).  pr is a device array of n floats; *iters the iterations run. */
+fmx_status fmx_pagerank(fmx_ctx *ctx, const fmx_spm *A, const fmx_spm *At, int max_iters, float damping,
+                        float *pr, int *iters, void *stream);
+
 /* ---- seeded generators (device output) ------------------------------------
  * R-MAT edges with the project's SplitMix64 specification (the reference's
  * generator, flash-graph/tools/rmat-gen.cpp:43-48, is a time-seeded Boost

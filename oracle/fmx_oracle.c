@@ -1294,3 +1294,57 @@ int orc_gnn_loss_grad_ex(uint64_t n, const int64_t *rp, const int64_t *ci, const
     free(Y); free(dY); free(dH); free(dZ); free(T); free(T2);
     return rc;
 }
+
+/* ------------------------------------------------------------------------ */
+/* PageRank (matrix/libmatrix-algs/PageRank.cpp:49-137), FP32 throughout as  */
+/* the reference computes it: out_deg = max(A*1, 1) (:49-60, :83); pr = 1/n  */
+/* (:31-43); each iteration in = pr / out_deg, pr1 = in * out_deg (:101-105),*/
+/* out = A^T * in (:110), pr2 = d*out + (1-d)/n (:118-119), converged =      */
+/* #{|pr1 - pr2| < 0.001/n} (:97,120-124), pr = pr2; until every vertex has  */
+/* converged or max_iters.  A and At are images of the matrix and of its     */
+/* transpose (the same image for a symmetric matrix).                        */
+/* ------------------------------------------------------------------------ */
+int orc_pagerank(const uint8_t *a_img, uint64_t a_bytes, const int64_t *a_offs, uint64_t a_nbr,
+                 const uint8_t *t_img, uint64_t t_bytes, const int64_t *t_offs, uint64_t t_nbr,
+                 uint64_t n, int max_iters, float damping, float *pr, int *iters)
+{
+    float *deg = (float *)malloc(n * sizeof(float));
+    float *ones = (float *)malloc(n * sizeof(float));
+    float *in = (float *)malloc(n * sizeof(float));
+    float *out = (float *)malloc(n * sizeof(float));
+    float *pr1 = (float *)malloc(n * sizeof(float));
+    int rc = ORC_OK;
+    if (!deg || !ones || !in || !out || !pr1) { rc = ORC_ENOMEM; goto done; }
+    for (uint64_t i = 0; i < n; i++) ones[i] = 1.0f;
+    rc = orc_spmm(a_img, a_bytes, a_offs, a_nbr, ORC_F32, 1, ones, 1, deg, 1);
+    if (rc) goto done;
+    for (uint64_t i = 0; i < n; i++) {
+        deg[i] = deg[i] > 1.0f ? deg[i] : 1.0f;
+        pr[i] = (float)(1.0 / (double)n);
+    }
+    {
+        const float eps = (float)(0.001 / (double)n);
+        const float add = (float)((1.0 - (double)damping) / (double)n);
+        uint64_t conv = 0;
+        int it = 0;
+        while (conv < n && it < max_iters) {
+            for (uint64_t i = 0; i < n; i++) {
+                in[i] = pr[i] / deg[i];
+                pr1[i] = in[i] * deg[i];
+            }
+            rc = orc_spmm(t_img, t_bytes, t_offs, t_nbr, ORC_F32, 1, in, 1, out, 1);
+            if (rc) goto done;
+            conv = 0;
+            for (uint64_t i = 0; i < n; i++) {
+                const float p2 = out[i] * damping + add;
+                conv += fabsf(pr1[i] - p2) < eps;
+                pr[i] = p2;
+            }
+            it++;
+        }
+        *iters = it;
+    }
+done:
+    free(deg); free(ones); free(in); free(out); free(pr1);
+    return rc;
+}

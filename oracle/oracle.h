@@ -103,6 +103,11 @@ void orc_gemm_tall_f64(const double *A, int64_t lda, uint64_t n, int ka,
 void orc_colnorm_f64(const double *A, int64_t lda, uint64_t n, int k,
                      double *norms);
 
+/* ---- PageRank (matrix/libmatrix-algs/PageRank.cpp:49-137) ------------- */
+int orc_pagerank(const uint8_t *a_img, uint64_t a_bytes, const int64_t *a_offs, uint64_t a_nbr,
+                 const uint8_t *t_img, uint64_t t_bytes, const int64_t *t_offs, uint64_t t_nbr,
+                 uint64_t n, int max_iters, float damping, float *pr, int *iters);
+
 /* ---- learned warm start: network and loss (FP64 checker) ----------------- */
 uint64_t orc_gnn_param_count(void);
 int orc_subspace_loss(uint64_t n, int k, const double *Y, const double *U, double *loss,

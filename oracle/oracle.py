@@ -260,6 +260,24 @@ def spmm_range(image, X, br0, br1):
     return Y
 
 
+def pagerank(A, At, max_iters=30, damping=0.85):
+    """(pr float32 array, iterations) as matrix/libmatrix-algs/PageRank.cpp computes them."""
+    L = lib()
+    L.orc_pagerank.restype = ctypes.c_int
+    L.orc_pagerank.argtypes = [_u8p, ctypes.c_uint64, _i64p, ctypes.c_uint64, _u8p, ctypes.c_uint64, _i64p,
+                               ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, ctypes.c_float, _vp,
+                               ctypes.POINTER(ctypes.c_int)]
+    n = int(A.data[24:32].view(np.uint64)[0])
+    pr = np.empty(n, np.float32)
+    it = ctypes.c_int()
+    rc = L.orc_pagerank(_p(A.data, _u8p), len(A.data), _p(A.offs, _i64p), A.nbr, _p(At.data, _u8p),
+                        len(At.data), _p(At.offs, _i64p), At.nbr, n, max_iters, damping, pr.ctypes.data,
+                        ctypes.byref(it))
+    if rc:
+        raise ValueError("orc_pagerank failed with status %d" % rc)
+    return pr, it.value
+
+
 def gram_f64(A, B):
     A = np.ascontiguousarray(A, np.float64)
     B = np.ascontiguousarray(B, np.float64)

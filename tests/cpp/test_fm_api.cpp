@@ -4,6 +4,8 @@
 // list, multiply by a dense matrix, check the known answer A*[0..n-1].
 // Prints "OK" on success; exit status 0/1.
 #include <cstdio>
+#include <algorithm>
+#include <cmath>
 #include <cstdlib>
 #include <string>
 #include <unistd.h>
@@ -75,6 +77,34 @@ int main()
         remove(mat.c_str());
         remove(idx.c_str());
         rmdir(dir);
+    }
+    // PageRank over A and its transpose (matrix/libmatrix-algs/PageRank.cpp),
+    // against a host power iteration of the same recurrence
+    {
+        fm::dense_matrix::ptr pr = fm::PageRank(A, 100, 0.85f);
+        if (!pr) return 1;
+        std::vector<float> g(n);
+        pr->copy_to_host(g.data());
+        std::vector<double> outdeg(n, 0), p(n, 1.0 / n), in(n), out(n);
+        std::vector<std::pair<uint32_t, uint32_t>> edges;
+        for (size_t e = 0; e < src.size(); e++) {
+            if (src[e] == dst[e]) continue;
+            bool dup = false;
+            for (size_t f = 0; f < e && !dup; f++) dup = (src[f] == src[e] && dst[f] == dst[e]);
+            if (!dup) edges.push_back({src[e], dst[e]});
+        }
+        for (auto &e : edges) outdeg[e.first] += 1;
+        for (int it = 0; it < 100; it++) {
+            for (uint32_t i = 0; i < n; i++) in[i] = p[i] / std::max(1.0, outdeg[i]);
+            std::fill(out.begin(), out.end(), 0.0);
+            for (auto &e : edges) out[e.second] += in[e.first];
+            for (uint32_t i = 0; i < n; i++) p[i] = 0.85 * out[i] + 0.15 / n;
+        }
+        for (uint32_t i = 0; i < n; i++)
+            if (std::abs(g[i] - p[i]) > 1e-2 / n) {  // stopped once every step < 0.001/n
+                printf("pagerank mismatch at %u: %g vs %g\n", i, g[i], p[i]);
+                return 1;
+            }
     }
     // error convention: a dimension mismatch is false + log, not an exception
     fm::dense_matrix::ptr bad = fm::dense_matrix::create(n + 1, 1, FMX_I64);

@@ -287,3 +287,35 @@ def test_weighted_against_brute_force(orc, dt, lh, lw, flags):
             A[p] += w
     X = (rng.random((1 << 10, 3)) * 8).astype(np.float64)
     assert np.allclose(orc.spmm(img, X), A @ X, rtol=1e-12, atol=1e-9)
+
+
+def test_pagerank_closed_forms(orc):
+    """The oracle's PageRank (PageRank.cpp restated) on graphs whose ranks
+    are known: a directed cycle and an undirected regular graph (every vertex
+    has the same in and out degree) keep the uniform start 1/n and stop after
+    one iteration; on a star with a hub the hub ranks above the leaves by the
+    ratio the fixed point fixes."""
+    n = 1000
+    s = np.arange(n, dtype=np.uint32)
+    d = (s + 1) % n
+    A = orc.build_2d(s, d, n=n, log2h=6, log2w=8)
+    At = orc.build_2d(d, s, n=n, log2h=6, log2w=8)
+    pr, it = orc.pagerank(A, At, 50, 0.85)
+    assert it == 1 and np.all(pr == np.float32(1.0 / n))
+    # 4-regular undirected ring (i <-> i+1, i <-> i+2)
+    s2 = np.concatenate([s, s]).astype(np.uint32)
+    d2 = np.concatenate([(s + 1) % n, (s + 2) % n]).astype(np.uint32)
+    R = orc.build_2d(s2, d2, n=n, log2h=7, log2w=7, flags=orc.DEDUP | orc.SYMMETRIZE)
+    pr, it = orc.pagerank(R, R, 50, 0.85)
+    assert it == 1 and np.allclose(pr, 1.0 / n, rtol=1e-6)
+    # star: leaves 1..m point at hub 0 and the hub at every leaf (undirected).
+    # Fixed point: h = d*m*l/1 + c, l = d*h/m + c with c = (1-d)/(m+1).
+    m = 99
+    leaves = np.arange(1, m + 1, dtype=np.uint32)
+    S = orc.build_2d(leaves, np.zeros(m, np.uint32), n=m + 1, log2h=4, log2w=4,
+                     flags=orc.DEDUP | orc.SYMMETRIZE)
+    pr, it = orc.pagerank(S, S, 200, 0.85)
+    dmp, c = 0.85, 0.15 / (m + 1)
+    h = (c + dmp * m * c) / (1 - dmp * dmp)
+    l = dmp * h / m + c
+    assert abs(pr[0] - h) < 1e-5 and np.allclose(pr[1:], l, rtol=1e-4)
