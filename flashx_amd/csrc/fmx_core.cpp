@@ -307,14 +307,16 @@ fmx_status get_plan(fmx_ctx *ctx, fmx_spm *s, uint32_t hs, Plan **out)
     // Longest items first, so the hardware dispatcher approximates LPT.
     std::stable_sort(plan->items.begin(), plan->items.end(),
                      [](const Item &a, const Item &b) { return a.nwords > b.nwords; });
-    // Column rotation: item i (in dispatch order) starts at the first of its
-    // tiles in block column >= (i * nbc / slots) mod nbc and wraps around.
-    // With `slots` items resident at a time and similar speeds, the items
-    // running together then gather from the same few tile columns of X,
-    // which keeps that part of X in the caches.  The start depends only on
-    // the item's index, so results stay deterministic.
+    // Column rotation (off unless FMX_ROT_SLOTS is set): item i (in dispatch
+    // order) starts at the first of its tiles in block column
+    // >= (i * nbc / slots) mod nbc and wraps around, so that items resident
+    // together gather from the same tile columns of X.  The start depends
+    // only on the item's index, so results stay deterministic.  Measured on
+    // R-MAT scale 22 it does not pay: the gathers are not bound by X's
+    // cache footprint (time per non-zero is the same with 1/8 of X), and
+    // every rotation rate tried was 5-10% slower than none.
     const char *re = getenv("FMX_ROT_SLOTS");
-    const uint64_t slots = re ? strtoull(re, nullptr, 10) : (uint64_t)ctx->num_cus * 7;
+    const uint64_t slots = re ? strtoull(re, nullptr, 10) : 0;
     for (size_t i = 0; i < plan->items.size(); i++) {
         Item &t = plan->items[i];
         t.rot_begin = t.begin;
