@@ -159,17 +159,22 @@ def main():
         data, offs = spm.download()
         Xh = X.cpu().numpy()
         t_cpu, Ycpu = cpu_baseline(data, offs, n, Xh, threads, args.cpu_reps)
-        Yg = Y.cpu().numpy()
+        Yg = Y.cpu().numpy().astype(np.float64)
         ref = Ycpu.astype(np.float64)
-        nz = ref != 0
-        rel = float(np.max(np.abs(Yg.astype(np.float64) - ref)[nz] / np.abs(ref[nz])))
-        zeros_ok = bool(np.all(Yg[~nz] == 0))
+        from oracle import oracle as O
+        exact = O.spmm(O.Image(data, offs, n), Xh.astype(np.float64), threads=threads)
+        nz = exact != 0
+        rel = float(np.max(np.abs(Yg - ref)[nz] / np.abs(ref[nz])))
+        rel64 = float(np.max(np.abs(Yg - exact)[nz] / np.abs(exact[nz])))
+        cpu64 = float(np.max(np.abs(ref - exact)[nz] / np.abs(exact[nz])))
+        zeros_ok = bool(np.all(Yg[~nz] == 0) and np.all(ref[~nz] == 0))
         out["cpu_baseline"] = {
             "value": round(alg_bytes / t_cpu / 1e9, 3), "unit": "GB/s", "cores": threads,
             "kind": "port",
             "sample": "full image and X, 1 warm-up + median of %d multiply() calls, "
                       "seconds per SpMM %.4f" % (args.cpu_reps, t_cpu)}
-        out["check"] = {"max_rel_err_vs_cpu": rel, "zero_rows_exact": zeros_ok}
+        out["check"] = {"max_rel_err_vs_cpu": rel, "max_rel_err_vs_fp64_exact": rel64,
+                        "cpu_fp32_max_rel_err_vs_fp64_exact": cpu64, "zero_rows_exact": zeros_ok}
     if not args.no_eigen or args.large:
         del spm, X, Y
         torch.cuda.empty_cache()

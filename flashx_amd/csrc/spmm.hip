@@ -36,10 +36,6 @@ namespace fmx {
 namespace {
 
 constexpr int kWPB = 1;  // waves per workgroup: one item, so LDS packs per wave
-#ifndef FMX_PIPE_R
-#define FMX_PIPE_R 16
-#endif
-constexpr int kPipeR = FMX_PIPE_R;  // words per lane per chunk in the pipelined kernel
 
 struct SpmmArgs {
     const uint16_t *img;
@@ -303,29 +299,23 @@ __device__ __forceinline__ uint32_t window_advance(const uint2 *dreg, const bool
 }
 
 // ---------------------------------------------------------------------------
-// Drains (compile-time width K).  A batch holds up to NZP*P queue entries;
-// lane group g takes `seg` consecutive ones (seg = ceil(entries / NZP) <= P,
-// the same for every group), so a run of one row is summed in registers and
-// reaches the band with one LDS add per element at each row change, and
-// passes past `seg` cost no accumulation.  The P gathers are always issued
-// (unused ones read X row 0), so every batch issues the same loads.
+// Drains (compile-time width K): gathers of P consecutive entries per lane
+// group, summed in registers per row run, added to the band at row changes.
 template <typename T, int K, int P, typename A>
-__device__ __forceinline__ uint32_t gather_batch(const SpmmArgs &a, const uint32_t *qcol, const uint16_t *qrow,
-                                                 uint32_t base, uint32_t qn, int lane,
-                                                 Vec<T, Lanes<T, K>::E> *v, uint32_t *rr)
+__device__ __forceinline__ void gather_batch(const SpmmArgs &a, const uint32_t *qcol, const uint16_t *qrow,
+                                             uint32_t base, uint32_t qn, int lane, Vec<T, Lanes<T, K>::E> *v,
+                                             uint32_t *rr)
 {
     using C = Lanes<T, K>;
     const T *X = (const T *)a.X;
     const int q = lane % C::L;
     const int g = lane / C::L;
-    const uint32_t nb = min(qn - base, (uint32_t)(C::NZP * P));
-    const uint32_t seg = (nb + C::NZP - 1) / C::NZP;
     uint32_t gcs[P];
 #pragma unroll
     for (int pp = 0; pp < P; pp++) {
-        const uint32_t e = base + g * seg + pp;
-        const bool ok = (uint32_t)pp < seg && e < base + nb;
-        const uint32_t sl = qslot<P>(ok ? e : base);
+        const uint32_t e = base + g * P + pp;
+        const uint32_t sl = qslot<P>(e);
+        const bool ok = e < qn;
         const uint32_t qr = qrow[sl];
         gcs[pp] = ok ? qcol[sl] : 0u;
         rr[pp] = (ok && qr != 0xFFFFu) ? qr : 0xFFFFFFFFu;
@@ -335,12 +325,11 @@ __device__ __forceinline__ uint32_t gather_batch(const SpmmArgs &a, const uint32
 #pragma unroll
     for (int pp = 0; pp < P; pp++) v[pp] = *(const Vec<T, C::E> *)(X + (uint64_t)gcs[pp] * a.ldx + q * C::E);
     __builtin_amdgcn_sched_barrier(0);
-    return seg;
 }
 
 template <typename T, int K, int P, typename A>
 __device__ __forceinline__ void accumulate_batch(A *band, uint32_t S, int lane, const Vec<T, Lanes<T, K>::E> *v,
-                                                 const uint32_t *rr, uint32_t seg)
+                                                 const uint32_t *rr)
 {
     using C = Lanes<T, K>;
     const int q = lane % C::L;
@@ -350,7 +339,6 @@ __device__ __forceinline__ void accumulate_batch(A *band, uint32_t S, int lane, 
     for (int i = 0; i < C::E; i++) acc[i] = (A)v[0].v[i];
 #pragma unroll
     for (int pp = 1; pp < P; pp++) {
-        if ((uint32_t)pp >= seg) break;  // uniform
         if (rr[pp] != cur) {
             if (cur != 0xFFFFFFFFu) {
                 A *dst = band + cur * S + q;
@@ -531,8 +519,8 @@ spmm_lane(SpmmArgs a)
             for (uint32_t base = 0; base < qn; base += C::NZP * P) {
                 Vec<T, C::E> v[P];
                 uint32_t rr[P];
-                const uint32_t seg = gather_batch<T, K, P, A>(a, qcol, qrow, base, qn, lane, v, rr);
-                accumulate_batch<T, K, P, A>(band, S, lane, v, rr, seg);
+                gather_batch<T, K, P, A>(a, qcol, qrow, base, qn, lane, v, rr);
+                accumulate_batch<T, K, P, A>(band, S, lane, v, rr);
             }
         } else {
             if (qn) drain_generic<T, P, A>(a, band, S, qcol, qrow, qn, lane);
@@ -555,7 +543,7 @@ __global__ void __launch_bounds__(64 * kWPB) __attribute__((amdgpu_waves_per_eu(
 spmm_pipe(SpmmArgs a)
 {
     using C = Lanes<T, K>;
-    constexpr int R = kPipeR;
+    constexpr int R = C::NZP * P / 64;
     using LC = LaneCfg<R>;
     constexpr int CW = LC::CW, DWIN = LC::DWIN, DPL = LC::DPL;
     constexpr int QS = CW + CW / P + 1;
@@ -607,21 +595,11 @@ spmm_pipe(SpmmArgs a)
     while (true) {
         Vec<T, C::E> v[P];
         uint32_t rr[P];
-        const uint32_t seg = gather_batch<T, K, P, A>(a, qcol, qrow, 0, qn, lane, v, rr);
-        // a chunk with more entries than one batch (long row parts) drains
-        // its remaining batches before the queue is reused
-        const bool extra = qn > (uint32_t)(C::NZP * P);
-        if (extra) {
-            accumulate_batch<T, K, P, A>(band, S, lane, v, rr, seg);
-            for (uint32_t base = C::NZP * P; base < qn; base += C::NZP * P) {
-                const uint32_t sg = gather_batch<T, K, P, A>(a, qcol, qrow, base, qn, lane, v, rr);
-                accumulate_batch<T, K, P, A>(band, S, lane, v, rr, sg);
-            }
-        }
+        gather_batch<T, K, P, typename AccOf<T>::type>(a, qcol, qrow, 0, qn, lane, v, rr);
         c += CW;
         const bool more = c < abs_end;
         const uint32_t qn_next = decode_step(c);  // no entries past the item's end
-        if (!extra) accumulate_batch<T, K, P, A>(band, S, lane, v, rr, seg);
+        accumulate_batch<T, K, P, A>(band, S, lane, v, rr);
         if (!more) break;
         qn = qn_next;
         wave_sync();
@@ -978,11 +956,12 @@ constexpr int passes_for()
     }
 }
 
-// Compile-time widths run pipelined.
+// The pipelined kernel needs a drain batch of a whole 512-word chunk.
 template <typename T, int K>
 constexpr bool use_pipe()
 {
-    return K > 0;
+    if constexpr (K == 0) return false;
+    else return Lanes<T, K>::NZP * passes_for<T, K>() == 512;
 }
 
 constexpr int kLaneR = 16;  // words per lane in the non-pipelined kernel
@@ -991,7 +970,7 @@ template <typename T, int K>
 uint32_t wave_bytes_for(uint32_t band_bytes)
 {
     constexpr int P = passes_for<T, K>();
-    constexpr int R = use_pipe<T, K>() ? kPipeR : kLaneR;
+    constexpr int R = use_pipe<T, K>() ? 8 : kLaneR;
     constexpr int CW = 64 * R;
     constexpr int QS = CW + CW / (K ? P : CW) + 1;
     const uint32_t win_off = (band_bytes + QS * 6 + 7) & ~7u;

@@ -54,7 +54,9 @@ def test_weighted_rmat(fmx, orc, edges, dt, lh, lw, flags):
     if np.issubdtype(dt, np.integer):
         v = rng.integers(-9, 10, len(s)).astype(dt)
     else:
-        v = (rng.random(len(s)) * 4 - 1).astype(dt)
+        # positive weights: the relative bounds are for sums without
+        # cancellation, as with the U[0,1) operands
+        v = (rng.random(len(s)) * 4 + 0.25).astype(dt)
     img = orc.build_2d_w(s, d, v, n=n, log2h=lh, log2w=lw, flags=flags)
     spm = fmx.SparseMatrix.upload(img.data, img.offs)
     for k in (1, 3, 8, 16):
