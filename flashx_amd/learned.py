@@ -166,7 +166,7 @@ def build_training_set(count, seed0=1000, log=print):
     return graphs
 
 
-def train(graphs, epochs=12, lr=2e-3, seed=0, log=print):
+def train(graphs, epochs=40, lr=2e-3, seed=0, log=print, halve_every=10):
     net = WarmStartNet(seed=seed)
     rng = np.random.default_rng(seed)
     hist = []
@@ -174,7 +174,7 @@ def train(graphs, epochs=12, lr=2e-3, seed=0, log=print):
         order = rng.permutation(len(graphs))
         tot = 0.0
         t0 = time.time()
-        cur_lr = lr * (0.5 ** (ep // 4))
+        cur_lr = lr * (0.5 ** (ep // halve_every))
         for i in order:
             tot += net.train_step(graphs[i], cur_lr)
         hist.append(tot / len(graphs))
@@ -225,7 +225,7 @@ def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("cmd", choices=["train", "eval"])
     ap.add_argument("--graphs", type=int, default=200)
-    ap.add_argument("--epochs", type=int, default=12)
+    ap.add_argument("--epochs", type=int, default=40)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--weights", default=DEFAULT_WEIGHTS)
     ap.add_argument("--out", default=DEFAULT_WEIGHTS)
