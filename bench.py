@@ -67,6 +67,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-reps", type=int, default=5)
     ap.add_argument("--no-eigen", action="store_true", help="skip the eigensolver legs")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run three 1 GiB device copies (the known-size kernel that scales "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE into bytes)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -75,6 +78,14 @@ def main():
 
     import numpy as np
     import torch
+    if args.calibrate_copy:
+        a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log("copied 3 x %d bytes" % (a.numel() * 4))
+        return
     import flashx_amd.fmx as F
 
     ctx = F.default_context(0)
