@@ -120,6 +120,47 @@ __global__ void __launch_bounds__(kB) compact_unique(const uint64_t *k, uint64_t
         if (i == 0 || k[i] != k[i - 1]) out[ex[i]] = k[i];
 }
 
+// Weighted builds keep input order among equal keys (the first value of a
+// duplicate is kept), so keys are emitted in edge order: cnt[i] = keys of
+// edge i, then after an exclusive scan each edge writes its keys at its
+// offset with the edge index as payload.
+__global__ void __launch_bounds__(kB) count_keys_w(const uint32_t *src, const uint32_t *dst, uint64_t m,
+                                                   int keep_self, int sym, uint32_t *cnt)
+{
+    for (uint64_t i = gtid(); i < m; i += gstride()) {
+        const uint32_t a = src[i], b = dst[i];
+        const bool ok = keep_self || a != b;
+        cnt[i] = ok ? ((sym && a != b) ? 2u : 1u) : 0u;
+    }
+}
+
+__global__ void __launch_bounds__(kB) emit_keys_w(const uint32_t *src, const uint32_t *dst, uint64_t m,
+                                                  KeyFmt f, int keep_self, int sym, const uint32_t *pos,
+                                                  uint64_t *keys, uint32_t *eidx)
+{
+    for (uint64_t i = gtid(); i < m; i += gstride()) {
+        const uint32_t a = src[i], b = dst[i];
+        if (!keep_self && a == b) continue;
+        const uint32_t p = pos[i];
+        keys[p] = f.key(a, b);
+        eidx[p] = (uint32_t)i;
+        if (sym && a != b) {
+            keys[p + 1] = f.key(b, a);
+            eidx[p + 1] = (uint32_t)i;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kB) compact_unique_kv(const uint64_t *k, const uint32_t *v, uint64_t n,
+                                                        const uint32_t *ex, uint64_t *out, uint32_t *vout)
+{
+    for (uint64_t i = gtid(); i < n; i += gstride())
+        if (i == 0 || k[i] != k[i - 1]) {
+            out[ex[i]] = k[i];
+            vout[ex[i]] = v[i];
+        }
+}
+
 // run_start[r] = first key of run r; run_start[R] = nk
 __global__ void __launch_bounds__(kB) fill_run_start(const uint64_t *k, uint64_t nk, int lw,
                                                      const uint32_t *ex, uint32_t *run_start,
@@ -165,13 +206,13 @@ __global__ void __launch_bounds__(kB) fill_tiles(const uint64_t *k, const uint32
 __global__ void __launch_bounds__(kB) tile_props(const uint64_t *k, KeyFmt f, const uint32_t *run_start,
                                                  const uint32_t *tile_first_run, uint32_t T,
                                                  const uint64_t *scsr_ex, const uint32_t *coo_ex,
-                                                 uint64_t *tsize, unsigned long long *brsize)
+                                                 uint64_t *tsize, unsigned long long *brsize, uint32_t esz)
 {
     for (uint64_t t = gtid(); t < T; t += gstride()) {
         const uint32_t fr = tile_first_run[t], lr = tile_first_run[t + 1];
         const uint64_t nnz = run_start[lr] - run_start[fr];
         const uint64_t nrow = lr - fr;
-        const uint64_t size = 16 + 2 * (nrow + nnz + 1);
+        const uint64_t size = 16 + 2 * (nrow + nnz + 1) + (uint64_t)esz * nnz;
         tsize[t] = size;
         atomicAdd(&brsize[f.br(k[run_start[fr]])], (unsigned long long)size);
     }
@@ -251,7 +292,8 @@ __global__ void __launch_bounds__(kB) write_nz(const uint64_t *k, uint64_t nk, K
                                                const uint32_t *tile_of_run,
                                                const uint32_t *tile_first_run,
                                                const uint64_t *scsr_ex, const uint32_t *coo_ex,
-                                               const uint64_t *tile_off, uint8_t *img)
+                                               const uint64_t *tile_off, uint8_t *img, const uint32_t *eidx,
+                                               const uint8_t *vals, uint32_t esz)
 {
     for (uint64_t i = gtid(); i < nk; i += gstride()) {
         const uint64_t key = k[i];
@@ -261,15 +303,27 @@ __global__ void __launch_bounds__(kB) write_nz(const uint64_t *k, uint64_t nk, K
         const uint32_t fr = tile_first_run[t], lr = tile_first_run[t + 1];
         const uint32_t s0 = run_start[r], len = run_start[r + 1] - s0;
         const uint64_t off = tile_off[t];
+        const uint64_t sw = scsr_ex[lr] - scsr_ex[fr];  // SCSR words (heads and columns) of the tile
+        uint64_t vidx;                                   // the non-zero's index in the values region
         if (len == 1) {
-            const uint64_t sw = scsr_ex[lr] - scsr_ex[fr];
             const uint64_t pos = off + 16 + 2 * (sw + 1) + 4ull * (coo_ex[r] - coo_ex[fr]);
             put16(img, pos, 0x8000u | f.rrel(key));
             put16(img, pos + 2, f.crel(key));
+            const uint64_t scsr_runs = (uint64_t)(lr - fr) - (coo_ex[lr] - coo_ex[fr]);
+            vidx = (sw - scsr_runs) + (coo_ex[r] - coo_ex[fr]);
         } else {
             const uint64_t sb = off + 16 + 2 * (scsr_ex[r] - scsr_ex[fr]);
             if (i == s0) put16(img, sb, 0x8000u | f.rrel(key));
             put16(img, sb + 2 * (1 + (i - s0)), f.crel(key));
+            const uint64_t runs_before = (uint64_t)(r - fr) - (coo_ex[r] - coo_ex[fr]);
+            vidx = (scsr_ex[r] - scsr_ex[fr]) - runs_before + (i - s0);
+        }
+        if (esz) {
+            // values follow the COO region: SCSR values in order, then COO values
+            const uint64_t ncoo = coo_ex[lr] - coo_ex[fr];
+            const uint64_t vb = off + 16 + 2 * (sw + 1) + 4 * ncoo + vidx * esz;
+            const uint8_t *v = vals + (uint64_t)eidx[i] * esz;
+            for (uint32_t b = 0; b < esz; b += 2) put16(img, vb + b, (uint32_t)v[b] | ((uint32_t)v[b + 1] << 8));
         }
     }
 }
@@ -297,9 +351,13 @@ fmx_status finish_device_image(fmx_ctx *ctx, fmx_spm *s);
 
 using namespace fmx;
 
-extern "C" fmx_status fmx_spm_build(fmx_ctx *ctx, const uint32_t *src, const uint32_t *dst,
-                                    size_t m, uint64_t n, uint32_t log2_h, uint32_t log2_w,
-                                    uint32_t flags, fmx_spm **out)
+namespace fmx {
+namespace {
+
+// vals (device, esz bytes per edge) != nullptr builds a weighted image.
+fmx_status build_impl(fmx_ctx *ctx, const uint32_t *src, const uint32_t *dst, size_t m, uint64_t n,
+                      uint32_t log2_h, uint32_t log2_w, uint32_t flags, const void *vals, uint32_t esz,
+                      int32_t prim_type, fmx_spm **out)
 {
     if (!ctx || !out) return fail(ctx, FMX_EINVAL, "null argument");
     *out = nullptr;
@@ -334,28 +392,54 @@ extern "C" fmx_status fmx_spm_build(fmx_ctx *ctx, const uint32_t *src, const uin
     const int sym = (flags & FMX_BUILD_SYMMETRIZE) != 0;
     const uint64_t cap = sym ? 2 * (uint64_t)m : (uint64_t)m;
 
-    // 1. keys
+    // 1. keys (weighted: in edge order, with the edge index as payload)
+    const int keep_self = (flags & FMX_BUILD_KEEP_SELF) ? 1 : 0;
     DevBuf<uint64_t> keys, tmp;
+    DevBuf<uint32_t> eidx, etmp;
     DevBuf<unsigned long long> counter;
     FMX_HIP(ctx, keys.alloc(cap));
     FMX_HIP(ctx, tmp.alloc(cap));
     FMX_HIP(ctx, counter.alloc(1));
     FMX_HIP(ctx, hipMemsetAsync(counter.p, 0, sizeof(unsigned long long), s));
-    if (m) {
-        hipLaunchKernelGGL(make_keys, dim3(grid_for(ctx, m)), dim3(kB), 0, s, src, dst, (uint64_t)m, f,
-                           (flags & FMX_BUILD_KEEP_SELF) ? 1 : 0, sym, keys.p, counter.p);
-        FMX_HIP(ctx, hipGetLastError());
-    }
     unsigned long long nk64 = 0;
-    FMX_HIP(ctx, hipMemcpyAsync(&nk64, counter.p, sizeof(nk64), hipMemcpyDeviceToHost, s));
-    FMX_HIP(ctx, hipStreamSynchronize(s));
+    fmx_status st = FMX_OK;
+    if (vals) {
+        if (m >= (1ull << 31)) return fail(ctx, FMX_EUNSUPPORTED, "weighted builds take fewer than 2^31 edges");
+        FMX_HIP(ctx, eidx.alloc(cap));
+        FMX_HIP(ctx, etmp.alloc(cap));
+        DevBuf<uint32_t> pos;
+        FMX_HIP(ctx, pos.alloc((uint64_t)m + 1));
+        uint32_t total = 0;
+        if (m) {
+            hipLaunchKernelGGL(count_keys_w, dim3(grid_for(ctx, m)), dim3(kB), 0, s, src, dst, (uint64_t)m,
+                               keep_self, sym, pos.p);
+            FMX_HIP(ctx, hipGetLastError());
+            st = scan_exclusive<uint32_t>(ctx, pos.p, pos.p, m, &total, s);
+            if (st) return st;
+            hipLaunchKernelGGL(emit_keys_w, dim3(grid_for(ctx, m)), dim3(kB), 0, s, src, dst, (uint64_t)m, f,
+                               keep_self, sym, pos.p, keys.p, eidx.p);
+            FMX_HIP(ctx, hipGetLastError());
+        }
+        nk64 = total;
+    } else {
+        if (m) {
+            hipLaunchKernelGGL(make_keys, dim3(grid_for(ctx, m)), dim3(kB), 0, s, src, dst, (uint64_t)m, f,
+                               keep_self, sym, keys.p, counter.p);
+            FMX_HIP(ctx, hipGetLastError());
+        }
+        FMX_HIP(ctx, hipMemcpyAsync(&nk64, counter.p, sizeof(nk64), hipMemcpyDeviceToHost, s));
+        FMX_HIP(ctx, hipStreamSynchronize(s));
+    }
     if (nk64 >= (1ull << 32)) return fail(ctx, FMX_EUNSUPPORTED, "more than 2^32 - 1 non-zeros");
 
-    // 2. sort + unique
+    // 2. sort + unique (a stable sort: equal keys keep edge order)
     uint64_t *K = keys.p;
-    fmx_status st = radix_sort_u64(ctx, keys.p, tmp.p, nk64, keybits, &K, s);
+    uint32_t *V = eidx.p;
+    st = radix_sort_u64_kv(ctx, keys.p, tmp.p, vals ? eidx.p : nullptr, vals ? etmp.p : nullptr, nk64, keybits,
+                           &K, &V, s);
     if (st) return st;
     uint64_t *other = (K == keys.p) ? tmp.p : keys.p;
+    uint32_t *vother = (V == eidx.p) ? etmp.p : eidx.p;
     uint64_t nk = nk64;
     DevBuf<uint32_t> flag;
     FMX_HIP(ctx, flag.alloc(nk + 1));
@@ -365,7 +449,13 @@ extern "C" fmx_status fmx_spm_build(fmx_ctx *ctx, const uint32_t *src, const uin
         uint32_t nu = 0;
         st = scan_exclusive<uint32_t>(ctx, flag.p, flag.p, nk, &nu, s);
         if (st) return st;
-        hipLaunchKernelGGL(compact_unique, dim3(grid_for(ctx, nk)), dim3(kB), 0, s, K, nk, flag.p, other);
+        if (vals) {
+            hipLaunchKernelGGL(compact_unique_kv, dim3(grid_for(ctx, nk)), dim3(kB), 0, s, K, V, nk, flag.p,
+                               other, vother);
+            V = vother;
+        } else {
+            hipLaunchKernelGGL(compact_unique, dim3(grid_for(ctx, nk)), dim3(kB), 0, s, K, nk, flag.p, other);
+        }
         FMX_HIP(ctx, hipGetLastError());
         K = other;
         nk = nu;
@@ -421,7 +511,7 @@ extern "C" fmx_status fmx_spm_build(fmx_ctx *ctx, const uint32_t *src, const uin
                            tflag.p, tile_first_run.p, tile_of_run.p, T);
         FMX_HIP(ctx, hipGetLastError());
         hipLaunchKernelGGL(tile_props, dim3(grid_for(ctx, T)), dim3(kB), 0, s, K, f, run_start.p,
-                           tile_first_run.p, T, scsr_ex.p, coo_ex.p, tsize.p, brsize.p);
+                           tile_first_run.p, T, scsr_ex.p, coo_ex.p, tsize.p, brsize.p, vals ? esz : 0u);
         FMX_HIP(ctx, hipGetLastError());
         hipLaunchKernelGGL(first_tile_of_row, dim3(grid_for(ctx, T)), dim3(kB), 0, s, K, f, run_start.p,
                            tile_first_run.p, T, br_first.p);
@@ -448,12 +538,12 @@ extern "C" fmx_status fmx_spm_build(fmx_ctx *ctx, const uint32_t *src, const uin
         uint8_t hdr[4096];
         memset(hdr, 0, sizeof(hdr));
         const int64_t magic = 0x0123456789FFFFFFLL;
-        const int32_t ver = 2, type = 2, layout = 2, prim = 0;
-        const uint64_t esz = 0;
+        const int32_t ver = 2, type = 2, layout = 2, prim = vals ? prim_type : 0;
+        const uint64_t hesz = vals ? esz : 0;
         memcpy(hdr + 0, &magic, 8);
         memcpy(hdr + 8, &ver, 4);
         memcpy(hdr + 12, &type, 4);
-        memcpy(hdr + 16, &esz, 8);
+        memcpy(hdr + 16, &hesz, 8);
         memcpy(hdr + 24, &n, 8);
         memcpy(hdr + 32, &n, 8);
         memcpy(hdr + 40, &layout, 4);
@@ -470,7 +560,7 @@ extern "C" fmx_status fmx_spm_build(fmx_ctx *ctx, const uint32_t *src, const uin
         FMX_HIP(ctx, hipGetLastError());
         hipLaunchKernelGGL(write_nz, dim3(grid_for(ctx, nk)), dim3(kB), 0, s, K, nk, f, flag.p,
                            run_start.p, tile_of_run.p, tile_first_run.p, scsr_ex.p, coo_ex.p,
-                           tile_off.p, spm->d_image);
+                           tile_off.p, spm->d_image, V, (const uint8_t *)vals, vals ? esz : 0u);
         FMX_HIP(ctx, hipGetLastError());
     }
     hipLaunchKernelGGL(write_stubs, dim3(grid_for(ctx, nbr)), dim3(kB), 0, s, had.p, nbr, offs_ex.p,
@@ -486,4 +576,34 @@ extern "C" fmx_status fmx_spm_build(fmx_ctx *ctx, const uint32_t *src, const uin
     if (st) return st;
     *out = spm.release();
     return FMX_OK;
+}
+
+}  // namespace
+}  // namespace fmx
+
+extern "C" fmx_status fmx_spm_build(fmx_ctx *ctx, const uint32_t *src, const uint32_t *dst,
+                                    size_t m, uint64_t n, uint32_t log2_h, uint32_t log2_w,
+                                    uint32_t flags, fmx_spm **out)
+{
+    return fmx::build_impl(ctx, src, dst, m, n, log2_h, log2_w, flags, nullptr, 0, 0, out);
+}
+
+extern "C" fmx_status fmx_spm_build_w(fmx_ctx *ctx, const uint32_t *src, const uint32_t *dst, const void *vals,
+                                      fmx_dtype dtype, size_t m, uint64_t n, uint32_t log2_h, uint32_t log2_w,
+                                      uint32_t flags, fmx_spm **out)
+{
+    if (!ctx) return FMX_EINVAL;
+    if (m && !vals) return fmx::fail(ctx, FMX_EINVAL, "null values");
+    // prim_type of the header (generic_type.h:57-70): P_INTEGER, P_LONG, P_FLOAT, P_DOUBLE
+    int32_t prim;
+    uint32_t esz;
+    switch (dtype) {
+    case FMX_I32: prim = 4; esz = 4; break;
+    case FMX_I64: prim = 6; esz = 8; break;
+    case FMX_F32: prim = 8; esz = 4; break;
+    case FMX_F64: prim = 9; esz = 8; break;
+    default: return fmx::fail(ctx, FMX_ETYPE, "values must be int32, int64, float32 or float64");
+    }
+    static const uint8_t none = 0;
+    return fmx::build_impl(ctx, src, dst, m, n, log2_h, log2_w, flags, vals ? vals : &none, esz, prim, out);
 }

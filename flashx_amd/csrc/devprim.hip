@@ -140,8 +140,11 @@ __global__ void __launch_bounds__(kT) rs_count(const uint64_t *keys, uint64_t n,
     hist[(uint64_t)threadIdx.x * nb + blockIdx.x] = cnt[threadIdx.x];
 }
 
-__global__ void __launch_bounds__(kT) rs_scatter(const uint64_t *in, uint64_t *out, uint64_t n,
-                                                 int shift, const uint64_t *offs, uint32_t nb)
+// Scatter of one pass; the optional u32 payload (vin -> vout) moves with its key.
+template <bool KV>
+__global__ void __launch_bounds__(kT) rs_scatter_t(const uint64_t *in, uint64_t *out, const uint32_t *vin,
+                                                   uint32_t *vout, uint64_t n, int shift, const uint64_t *offs,
+                                                   uint32_t nb)
 {
     __shared__ uint32_t wcnt[kWaves][256];
     __shared__ uint64_t wbase[kWaves][256];
@@ -151,10 +154,12 @@ __global__ void __launch_bounds__(kT) rs_scatter(const uint64_t *in, uint64_t *o
     // wave w owns keys [w*1024, (w+1)*1024) of the tile, 64 at a time, in order
     const uint64_t base = (uint64_t)blockIdx.x * kTile + (uint64_t)w * (kChunks * 64);
     uint64_t k[kChunks];
+    uint32_t pv[KV ? kChunks : 1];
 #pragma unroll
     for (int c = 0; c < kChunks; c++) {
         uint64_t idx = base + (uint64_t)c * 64 + lane;
         k[c] = idx < n ? in[idx] : 0;
+        if constexpr (KV) pv[c] = idx < n ? vin[idx] : 0u;
         if (idx < n) atomicAdd(&wcnt[w][(k[c] >> shift) & 0xFF], 1u);
     }
     __syncthreads();
@@ -182,7 +187,10 @@ __global__ void __launch_bounds__(kT) rs_scatter(const uint64_t *in, uint64_t *o
             peers &= bit ? bm : ~bm;
         }
         const uint64_t pos = wbase[w][d] + __popcll(peers & lt);
-        if (valid) out[pos] = k[c];
+        if (valid) {
+            out[pos] = k[c];
+            if constexpr (KV) vout[pos] = pv[c];
+        }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         if (valid && (peers & lt) == 0) wbase[w][d] += __popcll(peers);
@@ -235,7 +243,15 @@ fmx_status scan_u64_exclusive(fmx_ctx *ctx, const uint64_t *in, uint64_t *out, u
 fmx_status radix_sort_u64(fmx_ctx *ctx, uint64_t *keys, uint64_t *tmp, uint64_t n, int bits,
                           uint64_t **sorted, hipStream_t s)
 {
+    uint32_t *dummy = nullptr;
+    return radix_sort_u64_kv(ctx, keys, tmp, nullptr, nullptr, n, bits, sorted, &dummy, s);
+}
+
+fmx_status radix_sort_u64_kv(fmx_ctx *ctx, uint64_t *keys, uint64_t *tmp, uint32_t *vals, uint32_t *vtmp,
+                             uint64_t n, int bits, uint64_t **sorted, uint32_t **sorted_vals, hipStream_t s)
+{
     *sorted = keys;
+    *sorted_vals = vals;
     if (n < 2 || bits <= 0) return FMX_OK;
     const uint64_t nb64 = (n + kTile - 1) / kTile;
     if (nb64 >= (1ull << 32) / 256) return fail(ctx, FMX_EUNSUPPORTED, "too many keys to sort");
@@ -243,19 +259,28 @@ fmx_status radix_sort_u64(fmx_ctx *ctx, uint64_t *keys, uint64_t *tmp, uint64_t 
     DevBuf<uint64_t> hist;
     FMX_HIP(ctx, hist.alloc((uint64_t)nb * 256));
     uint64_t *a = keys, *b = tmp;
+    uint32_t *va = vals, *vb = vtmp;
     for (int shift = 0; shift < bits; shift += 8) {
         hipLaunchKernelGGL(rs_count, dim3(nb), dim3(kT), 0, s, a, n, shift, hist.p, nb);
         FMX_HIP(ctx, hipGetLastError());
         fmx_status st = scan_exclusive<uint64_t>(ctx, hist.p, hist.p, (uint64_t)nb * 256, nullptr, s);
         if (st) return st;
-        hipLaunchKernelGGL(rs_scatter, dim3(nb), dim3(kT), 0, s, a, b, n, shift, hist.p, nb);
+        if (vals)
+            hipLaunchKernelGGL(rs_scatter_t<true>, dim3(nb), dim3(kT), 0, s, a, b, va, vb, n, shift, hist.p, nb);
+        else
+            hipLaunchKernelGGL(rs_scatter_t<false>, dim3(nb), dim3(kT), 0, s, a, b, nullptr, nullptr, n, shift,
+                               hist.p, nb);
         FMX_HIP(ctx, hipGetLastError());
         uint64_t *t = a;
         a = b;
         b = t;
+        uint32_t *vt = va;
+        va = vb;
+        vb = vt;
     }
     FMX_HIP(ctx, hipStreamSynchronize(s));
     *sorted = a;
+    *sorted_vals = va;
     return FMX_OK;
 }
 

@@ -71,6 +71,8 @@ SIGNATURES = {
     "fmx_spm_destroy": (ctypes.c_int, [_vp]),
     "fmx_pagerank": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_float, _vp,
                                     ctypes.POINTER(ctypes.c_int), _vp]),
+    "fmx_spm_build_w": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_size_t, ctypes.c_uint64,
+                                       ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(_vp)]),
     "fmx_spm_load": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(_vp)]),
     "fmx_spm_dump": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_char_p]),
     "fmx_spmm": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int, _vp, _i64, _vp, _i64, _vp]),
@@ -236,6 +238,20 @@ class SparseMatrix:
         h = _vp()
         ctx.check(lib().fmx_spm_build(ctx.handle, src.data_ptr(), dst.data_ptr(), src.numel(), n,
                                       log2_h, log2_w, flags, ctypes.byref(h)), "fmx_spm_build")
+        return cls(ctx, h)
+
+    @classmethod
+    def build_weighted(cls, src, dst, vals, n=0, log2_h=15, log2_w=15, flags=BUILD_DEDUP, ctx=None):
+        """Weighted image from device tensors src, dst (32-bit ids) and vals
+        (int32 / int64 / float32 / float64, one per edge)."""
+        ctx = ctx or default_context()
+        if vals.device.type != "cuda" or vals.numel() != src.numel() or src.numel() != dst.numel():
+            raise FmxError(EINVAL, "src, dst and vals must be device tensors of equal length")
+        src, dst, vals = src.contiguous(), dst.contiguous(), vals.contiguous()
+        h = _vp()
+        ctx.check(lib().fmx_spm_build_w(ctx.handle, src.data_ptr(), dst.data_ptr(), vals.data_ptr(),
+                                        dtype_code(vals.dtype), src.numel(), n, log2_h, log2_w, flags,
+                                        ctypes.byref(h)), "fmx_spm_build_w")
         return cls(ctx, h)
 
     # -- reference accessors -------------------------------------------------

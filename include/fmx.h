@@ -97,6 +97,18 @@ fmx_status fmx_spm_build(fmx_ctx *ctx, const uint32_t *src, const uint32_t *dst,
                          size_t m, uint64_t n, uint32_t log2_h, uint32_t log2_w,
                          uint32_t flags, fmx_spm **out);
 
+/* Weighted form of fmx_spm_build (fm::create_2d_matrix with an attribute
+ * column; attr_apply_operate, matrix/fm_utils.cpp:484-535): vals is a DEVICE
+ * array of m values of dtype (I32, I64, F32, F64), written to each tile's
+ * values region (entry size 4 or 8, prim_type P_INTEGER / P_LONG / P_FLOAT /
+ * P_DOUBLE).  Among equal (row, column) edges input order is kept and
+ * FMX_BUILD_DEDUP keeps the first one's value (the reference's std::sort
+ * leaves that choice unspecified); SYMMETRIZE gives (dst, src) the same value.
+ * Bit-exact with the oracle's weighted writer.  m < 2^31. */
+fmx_status fmx_spm_build_w(fmx_ctx *ctx, const uint32_t *src, const uint32_t *dst, const void *vals,
+                           fmx_dtype dtype, size_t m, uint64_t n, uint32_t log2_h, uint32_t log2_w,
+                           uint32_t flags, fmx_spm **out);
+
 fmx_status fmx_spm_info_get(const fmx_spm *spm, fmx_spm_info *info);
 
 /* Copies the image (cap >= image_bytes) and the nbr+1 offsets to the host;

@@ -95,3 +95,21 @@ def test_weighted_file_round_trip(fmx, orc, edges, tmp_path):
     back = fmx.SparseMatrix.load(mat, idx)
     X = np.random.default_rng(2).integers(-5, 6, (1 << 14, 4)).astype(np.int32)
     assert np.array_equal(back.multiply(to_dev(X)).cpu().numpy(), orc.spmm(img, X))
+
+
+@pytest.mark.parametrize("dt,lh,lw,flags", [(np.float32, 7, 12, 1), (np.float64, 5, 9, 1 | 4),
+                                            (np.int32, 10, 10, 0), (np.int64, 6, 15, 1 | 2 | 4)])
+def test_weighted_device_build_bit_exact(fmx, orc, dt, lh, lw, flags):
+    """fmx_spm_build_w writes the same bytes as the oracle's weighted writer
+    (duplicates keep their first value, symmetrised edges share it)."""
+    import torch
+    s, d = orc.rmat_edges(13, 16 << 13, lh)
+    rng = np.random.default_rng(lw)
+    v = (rng.random(len(s)) * 100).astype(dt) if not np.issubdtype(dt, np.integer) else \
+        rng.integers(-1000, 1000, len(s)).astype(dt)
+    img = orc.build_2d_w(s, d, v, n=1 << 13, log2h=lh, log2w=lw, flags=flags)
+    spm = fmx.SparseMatrix.build_weighted(to_dev(s.view(np.int32)), to_dev(d.view(np.int32)), to_dev(v),
+                                          n=1 << 13, log2_h=lh, log2_w=lw, flags=flags)
+    data, offs = spm.download()
+    assert np.array_equal(offs, img.offs)
+    assert np.array_equal(data, img.data)
