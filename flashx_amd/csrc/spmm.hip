@@ -63,6 +63,8 @@ struct alignas(sizeof(T) * E) Vec {
     T v[E];
 };
 
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
 __device__ __forceinline__ uint64_t mask_lt(int lane) { return (1ull << lane) - 1ull; }
 
 __device__ __forceinline__ uint32_t rdlane(uint32_t v, int l)
@@ -258,7 +260,8 @@ __device__ __forceinline__ void load_words(const uint16_t *img, uint64_t c, uint
     for (int h = 0; h < R / 8; h++) {
         uint64_t g = c + (uint64_t)lane * R + h * 8;
         g = g < last8 ? g : last8;
-        const uint4 v = *(const uint4 *)(img + g);
+        // streamed once: non-temporal, so the image does not displace X in L2
+        const u32x4 v = __builtin_nontemporal_load((const u32x4 *)(img + g));
         wreg[h * 4 + 0] = v.x;
         wreg[h * 4 + 1] = v.y;
         wreg[h * 4 + 2] = v.z;
@@ -399,7 +402,8 @@ __device__ __forceinline__ void write_band(const SpmmArgs &a, const A *band, uin
         T *dst = (T *)a.Y + row0 * (uint64_t)a.ldy;
         for (uint32_t i = lane; i < rows * k; i += 64) {
             const uint32_t r = i / k, cc = i - r * k;
-            dst[r * (uint64_t)a.ldy + cc] = (T)band[r * S + bcol(cc)];
+            // written once and not read here: non-temporal
+            __builtin_nontemporal_store((T)band[r * S + bcol(cc)], &dst[r * (uint64_t)a.ldy + cc]);
         }
     } else {
         A *dst = (A *)a.partial + (uint64_t)slot * a.hs * k;
