@@ -65,6 +65,9 @@ def main():
     ap.add_argument("--large", action="store_true",
                     help="also run the R-MAT scale-27, k=32 SpMM with a sampled CPU check")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-stream", action="store_true", help="skip the out-of-core (streamed image) leg")
+    ap.add_argument("--stream-chunk", type=int, default=64 << 20,
+                    help="block-row chunk bytes of the streamed leg")
     ap.add_argument("--cpu-reps", type=int, default=5)
     ap.add_argument("--no-eigen", action="store_true", help="skip the eigensolver legs")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -165,6 +168,9 @@ def main():
         "cpu_baseline": None,
     }
 
+    if not args.no_stream:
+        out["streamed"] = streamed_leg(F, torch, spm, X, Y, args.stream_chunk)
+
     if not args.no_cpu:
         threads = len(os.sched_getaffinity(0))
         data, offs = spm.download()
@@ -198,24 +204,59 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def large_leg(F, torch, scale=27, k=32, steps=5):
+def streamed_leg(F, torch, spm, X, Y_ref, chunk_bytes, steps=10):
+    """The same SpMM with the image kept in pinned host memory and streamed
+    through HBM in block-row chunks (out of core): time per SpMM with the
+    host-to-device copies inside it, and bit-equality with the resident
+    result."""
+    data, offs = spm.download()
+    st = F.SparseMatrix.upload_streamed(data, offs, chunk_bytes=chunk_bytes)
+    del data
+    Y = torch.empty_like(Y_ref)
+    for _ in range(2):
+        st.multiply_async(X, Y)
+    torch.cuda.synchronize()
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(steps):
+        st.multiply_async(X, Y)
+    b.record()
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / steps
+    # the resident result of the same X (Y_ref holds the last timed product)
+    same = bool(torch.equal(Y, Y_ref))
+    tile_bytes = st.info.tile_bytes
+    log("streamed: %.3f ms per SpMM in %d chunks, image %.1f GB/s over the host link, same as resident: %s"
+        % (ms, st.info.stream_chunks, tile_bytes / ms / 1e6, same))
+    out = {"chunk_bytes": chunk_bytes, "chunks": int(st.info.stream_chunks), "ms_per_step": round(ms, 4),
+           "image_GBps_host_to_hbm": round(tile_bytes / ms / 1e6, 2), "equal_to_resident": same}
+    st.close()
+    return out
+
+
+def large_leg(F, torch, scale=27, k=32, steps=5, lh=7, lw=15):
     """R-MAT scale 27 (2^27 vertices, 2^31 edges), X 2^27 x 32 FP32, all in
-    HBM; tile height autotuned; FP32 check of a seeded sample of >= 1M rows
-    (whole block rows) against the oracle's reference-order CPU SpMM."""
+    HBM; 128 x 32768 tiles (the band of a k = 32 wave then leaves room for
+    four waves per CU); FP32 check of a seeded sample of >= 1M rows (whole
+    block rows) against the oracle's reference-order CPU SpMM."""
     import numpy as np
     from oracle import oracle as O
-    from flashx_amd.tune import autotune_tile
     n = 1 << scale
     g = torch.Generator(device="cuda")
     g.manual_seed(scale)
+    log("large: generating %d edges" % (EDGE_FACTOR * n))
     src, dst = F.rmat_edges(scale, EDGE_FACTOR * n, scale)
     X = torch.rand((n, k), dtype=torch.float32, device="cuda", generator=g)
-    (lh, lw), tt = autotune_tile(src, dst, n, k, torch.float32, candidates=((7, 15), (8, 15)), reps=2, X=X)
+    torch.cuda.synchronize()
+    log("large: building %dx%d tiles" % (1 << lh, 1 << lw))
     t0 = time.perf_counter()
     spm = F.SparseMatrix.build(src, dst, n=n, log2_h=lh, log2_w=lw, flags=F.BUILD_DEDUP)
     torch.cuda.synchronize()
     t_build = time.perf_counter() - t0
+    log("large: built in %.1f s, nnz %d, image %.2f GB" % (t_build, spm.info.nnz, spm.info.image_bytes / 1e9))
     del src, dst
+    torch.cuda.empty_cache()
     Y = torch.empty((n, k), dtype=torch.float32, device="cuda")
     for _ in range(2):
         spm.multiply_async(X, Y)
@@ -229,6 +270,7 @@ def large_leg(F, torch, scale=27, k=32, steps=5):
     torch.cuda.synchronize()
     ms = a.elapsed_time(b) / steps
     alg = spm.alg_bytes(torch.float32, k)
+    log("large: %.3f ms per SpMM; checking a sample on the CPU" % ms)
     H = 1 << lh
     rng = np.random.default_rng(scale)
     nb = (1 << 20) // H
@@ -248,7 +290,6 @@ def large_leg(F, torch, scale=27, k=32, steps=5):
     log("large: %.3f ms per SpMM, %.1f GB/s, sample rel err %.2e" % (ms, alg / ms / 1e6, rel))
     return {"workload": "spmm rmat%d_ef%d_f32_k%d" % (scale, EDGE_FACTOR, k), "nnz": int(spm.info.nnz),
             "image_bytes": int(spm.info.image_bytes), "tile": "%dx%d" % (H, 1 << lw),
-            "tile_autotune_ms": {"%dx%d" % (1 << p, 1 << q): round(v, 3) for (p, q), v in tt.items()},
             "build_s": round(t_build, 2), "ms_per_step": round(ms, 4), "GBps": round(alg / ms / 1e6, 1),
             "pct_hbm_peak": round(100 * alg / ms / 1e6 / PEAK_HBM_GBS, 2),
             "sample": {"rows": nb * H, "first_row": br0 * H, "max_rel_err_vs_cpu_fp32": rel,

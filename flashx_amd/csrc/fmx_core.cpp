@@ -64,8 +64,8 @@ static int log2_exact(uint64_t v)
 // further checks.
 fmx_status analyse_image(fmx_ctx *ctx, fmx_spm *s)
 {
-    const uint8_t *img = s->h_image.data();
-    const uint64_t bytes = s->h_image.size();
+    const uint8_t *img = s->host_image();
+    const uint64_t bytes = s->host_bytes();
     if (bytes < kHeaderBytes) return fail(ctx, FMX_EFORMAT, "image shorter than its 4096-byte header");
     if (rd64(img) != 0x0123456789FFFFFFULL) return fail(ctx, FMX_EFORMAT, "wrong magic number");
     if (rd32(img + 8) != 2) return fail(ctx, FMX_EFORMAT, "wrong version number");
@@ -203,7 +203,7 @@ uint32_t band_rows(const fmx_spm *spm, int k, int s)
 // is a row-part or COO header); `p` is first moved out of a tile header.
 static uint32_t carry_at(const fmx_spm *s, uint64_t br, uint32_t &p, uint32_t &dir_idx)
 {
-    const uint16_t *w = reinterpret_cast<const uint16_t *>(s->h_image.data() + s->offs[br]);
+    const uint16_t *w = reinterpret_cast<const uint16_t *>(s->host_image() + s->offs[br]);
     uint32_t lo = s->brow_dir[br], hi = s->brow_dir[br + 1];
     // last tile whose start <= p
     uint32_t t = (uint32_t)(std::upper_bound(s->dir.begin() + lo, s->dir.begin() + hi, p,
@@ -344,6 +344,29 @@ fmx_status get_plan(fmx_ctx *ctx, fmx_spm *s, uint32_t hs, Plan **out)
                         t.rows >= 1 && t.rows <= hs;
         if (!ok) return fail(ctx, FMX_EINVAL, "internal error: inconsistent SpMM work item");
     }
+    if (s->streamed()) {
+        // streamed: the items of each chunk together, longest first within it
+        const std::vector<uint64_t> &cb = s->chunk_br;
+        auto chunk_of = [&](const Item &t) {
+            const uint64_t br = (uint64_t)(((uint64_t)t.row0_hi << 32 | t.row0_lo) / s->H);
+            return (size_t)(std::upper_bound(cb.begin(), cb.end(), br) - cb.begin()) - 1;
+        };
+        std::stable_sort(plan->items.begin(), plan->items.end(),
+                         [&](const Item &a, const Item &b) { return chunk_of(a) < chunk_of(b); });
+        const size_t nch = cb.size() - 1;
+        plan->chunk_items.assign(nch + 1, 0);
+        for (const Item &t : plan->items) plan->chunk_items[chunk_of(t) + 1]++;
+        for (size_t c = 0; c < nch; c++) plan->chunk_items[c + 1] += plan->chunk_items[c];
+        // a chunk's items stay inside the bytes staged for it
+        for (size_t c = 0; c < nch; c++)
+            for (uint32_t i = plan->chunk_items[c]; i < plan->chunk_items[c + 1]; i++) {
+                const Item &t = plan->items[i];
+                const uint64_t lo = 2 * ((t.brow_word + t.begin) & ~7ull);
+                const uint64_t hi = 2 * (((t.brow_word + t.begin + t.nwords - 1) | 7ull) + 1);
+                if (lo < s->chunk_lo[c] || hi > s->chunk_hi[c])
+                    return fail(ctx, FMX_EINVAL, "internal error: work item outside its chunk");
+            }
+    }
     plan->nitems = (uint32_t)plan->items.size();
     plan->nslots = nslots;
     plan->nsplits = (uint32_t)splits.size();
@@ -383,13 +406,107 @@ fmx_status finish_device_image(fmx_ctx *ctx, fmx_spm *s)
     return upload_dirs(ctx, s);
 }
 
+// Out-of-core SpMM (the SAFS-streamed multiply of sparse_matrix.cpp:796-830
+// and matrix_io.cpp:188-236, here host memory -> HBM): chunk c is copied into
+// staging buffer c % 2 on the matrix's copy stream while the kernel of chunk
+// c - 1 runs on the caller's stream.  A buffer is refilled only after the
+// kernel that last read it has finished (ev_free, also across calls and
+// streams), and the copies wait for the caller's earlier work on its stream.
+// Split bands are reduced once, after the last chunk.
+fmx_status spmm_streamed(fmx_ctx *ctx, fmx_spm *spm, Plan *plan, fmx_dtype dtype, int k, const void *X,
+                         int64_t ldx, void *Y, int64_t ldy, hipStream_t s)
+{
+    const size_t nch = spm->chunk_br.size() - 1;
+    FMX_HIP(ctx, hipEventRecord(spm->ev_start, s));
+    FMX_HIP(ctx, hipStreamWaitEvent(spm->copy_stream, spm->ev_start, 0));
+    for (size_t c = 0; c < nch; c++) {
+        const int b = (int)(c & 1);
+        const uint64_t lo = spm->chunk_lo[c], hi = spm->chunk_hi[c];
+        // the kernel that last read buffer b (this call's, or an earlier
+        // call's on any stream) has finished
+        if (spm->free_recorded[b]) FMX_HIP(ctx, hipStreamWaitEvent(spm->copy_stream, spm->ev_free[b], 0));
+        FMX_HIP(ctx, hipMemcpyAsync(spm->d_stage[b], spm->h_pinned + lo, hi - lo, hipMemcpyHostToDevice,
+                                    spm->copy_stream));
+        FMX_HIP(ctx, hipEventRecord(spm->ev_copied[b], spm->copy_stream));
+        FMX_HIP(ctx, hipStreamWaitEvent(s, spm->ev_copied[b], 0));
+        // absolute word w of the image is at staging byte 2w - lo
+        const uint16_t *img = (const uint16_t *)(spm->d_stage[b] - lo);
+        fmx_status st = launch_spmm(ctx, spm, plan, dtype, k, X, ldx, Y, ldy, spm->partial.ptr, s, img,
+                                    plan->chunk_items[c], plan->chunk_items[c + 1], c + 1 == nch);
+        if (st) return st;
+        FMX_HIP(ctx, hipEventRecord(spm->ev_free[b], s));
+        spm->free_recorded[b] = true;
+    }
+    return FMX_OK;
+}
+
+// Pinned host image (already holding h_pinned_len bytes) -> validated,
+// chunked streamed matrix.
+fmx_status finish_streamed(fmx_ctx *ctx, fmx_spm *s, uint64_t chunk_bytes, bool whole_file)
+{
+    fmx_status st = analyse_image(ctx, s);
+    if (st) return st;
+    if (whole_file && s->bytes != s->h_pinned_len)
+        return fail(ctx, FMX_EFORMAT, "matrix file is longer than its block rows");
+    if (chunk_bytes < 4096) return fail(ctx, FMX_EINVAL, "chunk_bytes must be at least 4096");
+    // chunks of whole block rows, each at most chunk_bytes unless one block
+    // row alone is larger
+    s->chunk_br.clear();
+    s->chunk_lo.clear();
+    s->chunk_hi.clear();
+    uint64_t br = 0;
+    while (br < s->nbr) {
+        const uint64_t lo = (uint64_t)s->offs[br] & ~15ull;
+        uint64_t e = br + 1;
+        while (e < s->nbr && ((((uint64_t)s->offs[e + 1] + 15) & ~15ull) - lo) <= chunk_bytes) e++;
+        s->chunk_br.push_back(br);
+        s->chunk_lo.push_back(lo);
+        s->chunk_hi.push_back(((uint64_t)s->offs[e] + 15) & ~15ull);
+        s->stage_bytes = std::max(s->stage_bytes, s->chunk_hi.back() - lo);
+        br = e;
+    }
+    s->chunk_br.push_back(s->nbr);
+    FMX_HIP(ctx, hipSetDevice(ctx->device));
+    for (int b = 0; b < 2; b++) {
+        FMX_HIP(ctx, hipMalloc(&s->d_stage[b], std::max<uint64_t>(s->stage_bytes, 16)));
+        FMX_HIP(ctx, hipEventCreateWithFlags(&s->ev_copied[b], hipEventDisableTiming));
+        FMX_HIP(ctx, hipEventCreateWithFlags(&s->ev_free[b], hipEventDisableTiming));
+    }
+    FMX_HIP(ctx, hipEventCreateWithFlags(&s->ev_start, hipEventDisableTiming));
+    FMX_HIP(ctx, hipStreamCreateWithFlags(&s->copy_stream, hipStreamNonBlocking));
+    return upload_dirs(ctx, s);
+}
+
+// Pinned buffer of `len` bytes padded with zeros to 4 KiB.
+fmx_status alloc_pinned(fmx_ctx *ctx, fmx_spm *s, uint64_t len)
+{
+    const uint64_t padded = std::max<uint64_t>((len + 4095) & ~4095ull, 4096);
+    FMX_HIP(ctx, hipSetDevice(ctx->device));
+    FMX_HIP(ctx, hipHostMalloc((void **)&s->h_pinned, padded, hipHostMallocDefault));
+    memset(s->h_pinned + len, 0, padded - len);
+    s->h_pinned_len = len;
+    return FMX_OK;
+}
+
 }  // namespace fmx
 
 fmx_spm::~fmx_spm()
 {
+    // no copy or kernel may still use the staging buffers
+    if (copy_stream) (void)hipStreamSynchronize(copy_stream);
+    for (int b = 0; b < 2; b++)
+        if (free_recorded[b]) (void)hipEventSynchronize(ev_free[b]);
     if (d_image) (void)hipFree(d_image);
     if (d_dir) (void)hipFree(d_dir);
     if (d_vdir) (void)hipFree(d_vdir);
+    for (int b = 0; b < 2; b++) {
+        if (d_stage[b]) (void)hipFree(d_stage[b]);
+        if (ev_copied[b]) (void)hipEventDestroy(ev_copied[b]);
+        if (ev_free[b]) (void)hipEventDestroy(ev_free[b]);
+    }
+    if (ev_start) (void)hipEventDestroy(ev_start);
+    if (copy_stream) (void)hipStreamDestroy(copy_stream);
+    if (h_pinned) (void)hipHostFree(h_pinned);
 }
 
 using namespace fmx;
@@ -497,6 +614,7 @@ fmx_status fmx_spm_info_get(const fmx_spm *spm, fmx_spm_info *info)
     info->tile_bytes = spm->bytes - kHeaderBytes;
     info->entry_size = spm->esz;
     info->ntiles = spm->ntiles;
+    info->stream_chunks = spm->streamed() ? spm->chunk_br.size() - 1 : 0;
     return FMX_OK;
 }
 
@@ -508,8 +626,12 @@ fmx_status fmx_spm_download(const fmx_spm *spm, void *image, size_t cap,
     if (bytes) *bytes = spm->bytes;
     if (image) {
         if (cap < spm->bytes) return fail(ctx, FMX_EINVAL, "image buffer too small");
-        FMX_HIP(ctx, hipSetDevice(ctx->device));
-        FMX_HIP(ctx, hipMemcpy(image, spm->d_image, spm->bytes, hipMemcpyDeviceToHost));
+        if (spm->streamed()) {
+            memcpy(image, spm->h_pinned, spm->bytes);
+        } else {
+            FMX_HIP(ctx, hipSetDevice(ctx->device));
+            FMX_HIP(ctx, hipMemcpy(image, spm->d_image, spm->bytes, hipMemcpyDeviceToHost));
+        }
     }
     if (offs) memcpy(offs, spm->offs.data(), spm->offs.size() * sizeof(int64_t));
     return FMX_OK;
@@ -568,6 +690,71 @@ fmx_status fmx_spm_load(fmx_ctx *ctx, const char *mat_file, const char *idx_file
         return fail(ctx, FMX_EFORMAT, "matrix file is longer than its block rows");
     }
     return FMX_OK;
+}
+
+fmx_status fmx_spm_upload_streamed(fmx_ctx *ctx, const void *image, size_t bytes, const int64_t *offs, size_t nbr,
+                                   uint64_t chunk_bytes, fmx_spm **out)
+{
+    if (!ctx || !image || !offs || !out) return fail(ctx, FMX_EINVAL, "null argument");
+    *out = nullptr;
+    std::unique_ptr<fmx_spm> s(new fmx_spm());
+    s->ctx = ctx;
+    fmx_status st = alloc_pinned(ctx, s.get(), bytes);
+    if (st) return st;
+    memcpy(s->h_pinned, image, bytes);
+    s->offs.assign(offs, offs + nbr + 1);
+    st = finish_streamed(ctx, s.get(), chunk_bytes, false);
+    if (st) return st;
+    *out = s.release();
+    return FMX_OK;
+}
+
+fmx_status fmx_spm_open_index(fmx_ctx *ctx, const char *mat_file, const void *header, const int64_t *offs,
+                              size_t nbr, uint64_t chunk_bytes, fmx_spm **out)
+{
+    if (!ctx || !mat_file || !header || !offs || !out) return fail(ctx, FMX_EINVAL, "null argument");
+    *out = nullptr;
+    FILE *f = fopen(mat_file, "rb");
+    if (!f) return fail(ctx, FMX_EINVAL, std::string("can't read ") + mat_file);
+    long n = -1;
+    if (fseek(f, 0, SEEK_END) == 0) n = ftell(f);
+    if (n < 0 || fseek(f, 0, SEEK_SET) != 0) {
+        fclose(f);
+        return fail(ctx, FMX_EINVAL, std::string("can't read ") + mat_file);
+    }
+    std::unique_ptr<fmx_spm> s(new fmx_spm());
+    s->ctx = ctx;
+    fmx_status st = alloc_pinned(ctx, s.get(), (uint64_t)n);
+    if (st) {
+        fclose(f);
+        return st;
+    }
+    const bool ok = n == 0 || fread(s->h_pinned, (size_t)n, 1, f) == 1;
+    fclose(f);
+    if (!ok) return fail(ctx, FMX_EINVAL, std::string("can't read ") + mat_file);
+    if ((uint64_t)n < kHeaderBytes) return fail(ctx, FMX_EFORMAT, "matrix file shorter than its header");
+    // both describe the same matrix: type, entry size, shape, layout, tiles
+    if (memcmp(header, s->h_pinned, 64) != 0) return fail(ctx, FMX_EFORMAT, "index and matrix headers differ");
+    s->offs.assign(offs, offs + nbr + 1);
+    st = finish_streamed(ctx, s.get(), chunk_bytes, true);
+    if (st) return st;
+    *out = s.release();
+    return FMX_OK;
+}
+
+fmx_status fmx_spm_open(fmx_ctx *ctx, const char *mat_file, const char *idx_file, uint64_t chunk_bytes,
+                        fmx_spm **out)
+{
+    if (!ctx || !mat_file || !idx_file || !out) return fail(ctx, FMX_EINVAL, "null argument");
+    *out = nullptr;
+    std::vector<uint8_t> idx;
+    if (!read_file(idx_file, idx)) return fail(ctx, FMX_EINVAL, std::string("can't read ") + idx_file);
+    if (idx.size() < kHeaderBytes + 8 || (idx.size() - kHeaderBytes) % 8)
+        return fail(ctx, FMX_EFORMAT, "index file is not a header plus 8-byte offsets");
+    const size_t nent = (idx.size() - kHeaderBytes) / 8;
+    std::vector<int64_t> offs(nent);
+    memcpy(offs.data(), idx.data() + kHeaderBytes, nent * 8);
+    return fmx_spm_open_index(ctx, mat_file, idx.data(), offs.data(), nent - 1, chunk_bytes, out);
 }
 
 fmx_status fmx_spm_dump(const fmx_spm *spm, const char *mat_file, const char *idx_file)
@@ -636,7 +823,10 @@ fmx_status fmx_spmm(fmx_ctx *ctx, const fmx_spm *cspm, fmx_dtype dtype, int k,
         spm->partial.bytes = need;
     }
     hipStream_t s = (hipStream_t)stream;
-    return launch_spmm(ctx, spm, plan, dtype, k, X, ldx, Y, ldy, spm->partial.ptr, s);
+    if (!spm->streamed())
+        return launch_spmm(ctx, spm, plan, dtype, k, X, ldx, Y, ldy, spm->partial.ptr, s,
+                           (const uint16_t *)spm->d_image, 0, plan->nitems, true);
+    return spmm_streamed(ctx, spm, plan, dtype, k, X, ldx, Y, ldy, s);
 }
 
 }  // extern "C"

@@ -64,6 +64,8 @@ struct Plan {
     Item *d_items = nullptr;
     Split *d_splits = nullptr;
     std::vector<Item> items;    // host copy (tests / diagnostics)
+    // streamed matrices: the items of chunk c are [chunk_items[c], chunk_items[c+1])
+    std::vector<uint32_t> chunk_items;
     ~Plan();
 };
 
@@ -93,13 +95,30 @@ struct fmx_spm {
     uint8_t *d_image = nullptr;         // image in HBM (padded to 4 KiB)
     uint2 *d_dir = nullptr;             // tile directory
     uint32_t *d_vdir = nullptr;         // weighted images: values-region word of each tile
-    std::vector<uint8_t> h_image;       // host copy used to build plans
+    std::vector<uint8_t> h_image;       // host copy used to build plans (resident matrices)
     std::vector<int64_t> offs;          // block-row byte offsets
     std::vector<uint2> dir;             // host directory
     std::vector<uint32_t> vdir;         // weighted images: values-region word (block-row relative)
     std::vector<uint32_t> brow_dir;     // first directory index per block row (+1)
     std::map<uint32_t, std::unique_ptr<fmx::Plan>> plans;
     fmx::Scratch partial;               // partial bands of split items
+    // Out-of-core form (d_image == nullptr): the image stays in pinned host
+    // memory and every SpMM streams it through two HBM staging buffers in
+    // chunks of whole block rows; directory and plans stay in HBM.
+    uint8_t *h_pinned = nullptr;        // image, padded with zeros to 4 KiB
+    uint64_t h_pinned_len = 0;          // bytes given (file or buffer length)
+    std::vector<uint64_t> chunk_br;     // first block row of each chunk, then nbr
+    std::vector<uint64_t> chunk_lo, chunk_hi;  // 16-byte aligned byte span of each chunk
+    uint8_t *d_stage[2] = {nullptr, nullptr};
+    uint64_t stage_bytes = 0;
+    hipStream_t copy_stream = nullptr;
+    hipEvent_t ev_copied[2] = {nullptr, nullptr};
+    hipEvent_t ev_free[2] = {nullptr, nullptr};   // last kernel reading each buffer
+    bool free_recorded[2] = {false, false};
+    hipEvent_t ev_start = nullptr;
+    bool streamed() const { return h_pinned != nullptr; }
+    const uint8_t *host_image() const { return h_pinned ? h_pinned : h_image.data(); }
+    uint64_t host_bytes() const { return h_pinned ? h_pinned_len : h_image.size(); }
     ~fmx_spm();
 };
 
@@ -117,10 +136,14 @@ fmx_status get_plan(fmx_ctx *ctx, fmx_spm *spm, uint32_t hs, Plan **out);
 // Band height for element size s and width k.
 uint32_t band_rows(const fmx_spm *spm, int k, int s);
 
+// Launches the SpMM kernel over plan items [item_begin, item_end) with the
+// image at img (absolute u16 word w of the image is img[w]) and, if reduce,
+// the in-order reduction of the plan's split bands.
 fmx_status launch_spmm(fmx_ctx *ctx, const fmx_spm *spm, const Plan *plan,
                        fmx_dtype dt, int k, const void *X, int64_t ldx,
                        void *Y, int64_t ldy, void *partial,
-                       hipStream_t stream);
+                       hipStream_t stream, const uint16_t *img,
+                       uint32_t item_begin, uint32_t item_end, bool reduce);
 
 }  // namespace fmx
 

@@ -250,17 +250,19 @@ __device__ __forceinline__ uint32_t decode_chunk(const ItemCtx &ic, uint64_t c, 
 }
 
 // This lane's words of chunk c, clamped to the aligned 8-word group holding
-// the item's last word (always inside the image allocation).
+// the item's last word lastw (always inside the image allocation, whose size
+// is a multiple of 4096 bytes).  The image is streamed once: non-temporal
+// loads, so it does not displace X in L2.
 template <int R>
-__device__ __forceinline__ void load_words(const uint16_t *img, uint64_t c, uint64_t last8, int lane,
+__device__ __forceinline__ void load_words(const uint16_t *img, uint64_t c, uint64_t lastw, int lane,
                                            uint32_t *wreg)
 {
     static_assert(R % 8 == 0, "lane word count");
+    const uint64_t last8 = lastw & ~7ull;
 #pragma unroll
     for (int h = 0; h < R / 8; h++) {
         uint64_t g = c + (uint64_t)lane * R + h * 8;
         g = g < last8 ? g : last8;
-        // streamed once: non-temporal, so the image does not displace X in L2
         const u32x4 v = __builtin_nontemporal_load((const u32x4 *)(img + g));
         wreg[h * 4 + 0] = v.x;
         wreg[h * 4 + 1] = v.y;
@@ -336,6 +338,21 @@ __device__ __forceinline__ void accumulate_batch(A *band, uint32_t S, int lane, 
 {
     using C = Lanes<T, K>;
     const int q = lane % C::L;
+    if constexpr (sizeof(T) == 8) {
+        // 8-byte data: each entry adds straight into the band.  Run sums
+        // save few LDS adds (rows average 2.4 entries per tile on R-MAT)
+        // and their divergent branches cost more than that for E <= 2
+        // (FP64 k = 16: 1.17 ms against 1.34 ms with run sums)
+#pragma unroll
+        for (int pp = 0; pp < P; pp++) {
+            if (rr[pp] != 0xFFFFFFFFu) {
+                A *dst = band + rr[pp] * S + q;
+#pragma unroll
+                for (int i = 0; i < C::E; i++) lds_add(dst + i * C::L, (A)v[pp].v[i]);
+            }
+        }
+        return;
+    }
     A acc[C::E];
     uint32_t cur = rr[0];
 #pragma unroll
@@ -471,7 +488,7 @@ __device__ __forceinline__ ItemState item_state(const SpmmArgs &a, const Item &i
 // Non-pipelined kernel: runtime widths (K = 0) and widths whose drain batch
 // is smaller than a chunk.  Chunks of 64*R words, drained batch by batch.
 template <typename T, int K, int R, int P>
-__global__ void __launch_bounds__(64 * kWPB) __attribute__((amdgpu_waves_per_eu(1, 2)))
+__global__ void __launch_bounds__(64 * kWPB) __attribute__((amdgpu_waves_per_eu(2, 2)))
 spmm_lane(SpmmArgs a)
 {
     using LC = LaneCfg<R>;
@@ -499,12 +516,12 @@ spmm_lane(SpmmArgs a)
     for (int seg = 0; seg < 2; seg++) {
     if (!select_segment(st, seg)) break;
     const uint64_t abs_end = st.ic.bw + (uint64_t)st.ic.pend;
-    const uint64_t last8 = (abs_end - 1) & ~7ull;
+    const uint64_t lastw = abs_end - 1;
     uint64_t c = (st.ic.bw + (uint64_t)st.ic.begin) & ~7ull;
     uint32_t wreg[R / 2];
     uint2 dreg[DPL];
     bool live[DPL];
-    load_words<R>(a.img, c, last8, lane, wreg);
+    load_words<R>(a.img, c, lastw, lane, wreg);
     load_window<DPL>(a.dir, st.ct, st.dlast, lane, dreg, live);
 
     while (c < abs_end) {
@@ -514,7 +531,7 @@ spmm_lane(SpmmArgs a)
         const uint64_t cn = c + CW;
         if (cn < abs_end) {
             st.ct += window_advance<DPL>(dreg, live, (uint32_t)((int64_t)cn - (int64_t)st.ic.bw)) - 1;
-            load_words<R>(a.img, cn, last8, lane, wreg);
+            load_words<R>(a.img, cn, lastw, lane, wreg);
             load_window<DPL>(a.dir, st.ct, st.dlast, lane, dreg, live);
         }
         wave_sync();
@@ -543,7 +560,7 @@ spmm_lane(SpmmArgs a)
 // in the same order (past the item's end they are clamped and decode to no
 // entries), so the waits before the decode cover only the earlier requests.
 template <typename T, int K, int P>
-__global__ void __launch_bounds__(64 * kWPB) __attribute__((amdgpu_waves_per_eu(1, 2)))
+__global__ void __launch_bounds__(64 * kWPB) __attribute__((amdgpu_waves_per_eu(2, 2)))
 spmm_pipe(SpmmArgs a)
 {
     using C = Lanes<T, K>;
@@ -569,7 +586,7 @@ spmm_pipe(SpmmArgs a)
     for (uint32_t i = lane; i < st.ic.rows * S; i += 64) band[i] = A(0);
     if (lane == 0) win[DWIN] = make_uint2(0xFFFFFFFFu, 0u);
 
-    uint64_t abs_end, last8, c;
+    uint64_t abs_end, lastw, c;
     uint32_t wreg[R / 2];
     uint2 dreg[DPL];
     bool live[DPL];
@@ -583,7 +600,7 @@ spmm_pipe(SpmmArgs a)
         const uint64_t cn = cc + CW;
         const uint32_t adv = window_advance<DPL>(dreg, live, (uint32_t)((int64_t)cn - (int64_t)st.ic.bw));
         if (cn < abs_end) st.ct += adv - 1;
-        load_words<R>(a.img, cn, last8, lane, wreg);
+        load_words<R>(a.img, cn, lastw, lane, wreg);
         load_window<DPL>(a.dir, st.ct, st.dlast, lane, dreg, live);
         wave_sync();
         return n;
@@ -591,9 +608,9 @@ spmm_pipe(SpmmArgs a)
     for (int seg = 0; seg < 2; seg++) {
     if (!select_segment(st, seg)) break;
     abs_end = st.ic.bw + (uint64_t)st.ic.pend;
-    last8 = (abs_end - 1) & ~7ull;
+    lastw = abs_end - 1;
     c = (st.ic.bw + (uint64_t)st.ic.begin) & ~7ull;
-    load_words<R>(a.img, c, last8, lane, wreg);
+    load_words<R>(a.img, c, lastw, lane, wreg);
     load_window<DPL>(a.dir, st.ct, st.dlast, lane, dreg, live);
     uint32_t qn = decode_step(c);
     while (true) {
@@ -815,14 +832,14 @@ spmm_lane_w(SpmmArgs a)
     for (int seg = 0; seg < 2; seg++) {
     if (!select_segment(st, seg)) break;
     const uint64_t abs_end = st.ic.bw + (uint64_t)st.ic.pend;
-    const uint64_t last8 = (abs_end - 1) & ~7ull;
+    const uint64_t lastw = abs_end - 1;
     uint64_t c = (st.ic.bw + (uint64_t)st.ic.begin) & ~7ull;
     int c0 = 0;  // columns of window tile 0 decoded in earlier chunks
     uint32_t wreg[R / 2];
     uint2 dreg[DPL];
     uint32_t vreg[DPL];
     bool live[DPL];
-    load_words<R>(a.img, c, last8, lane, wreg);
+    load_words<R>(a.img, c, lastw, lane, wreg);
     load_window<DPL>(a.dir, st.ct, st.dlast, lane, dreg, live);
 #pragma unroll
     for (int h = 0; h < DPL; h++) {
@@ -842,7 +859,7 @@ spmm_lane_w(SpmmArgs a)
             const uint32_t adv = window_advance<DPL>(dreg, live, (uint32_t)((int64_t)cn - (int64_t)st.ic.bw));
             c0 = (int)qn - wq[adv - 1];
             st.ct += adv - 1;
-            load_words<R>(a.img, cn, last8, lane, wreg);
+            load_words<R>(a.img, cn, lastw, lane, wreg);
             load_window<DPL>(a.dir, st.ct, st.dlast, lane, dreg, live);
 #pragma unroll
             for (int h = 0; h < DPL; h++) {
@@ -960,13 +977,20 @@ constexpr int passes_for()
     }
 }
 
-// The pipelined kernel needs a drain batch of a whole 512-word chunk.
+// The pipelined kernel drains a whole 512-word chunk (8 words per lane) in
+// one batch.  Narrower batches would need smaller chunks, whose fixed decode
+// cost outweighs the overlap on images with few entries per word (a 256-word
+// chunk took 340 ms against 192 ms for the 1024-word chunks of spmm_lane on
+// R-MAT scale 27 with k = 32 FP32), so those widths use spmm_lane.
 template <typename T, int K>
-constexpr bool use_pipe()
+constexpr int pipe_words()
 {
-    if constexpr (K == 0) return false;
-    else return Lanes<T, K>::NZP * passes_for<T, K>() == 512;
+    if constexpr (K == 0) return 0;
+    else return Lanes<T, K>::NZP * passes_for<T, K>() == 512 ? 512 : 0;
 }
+
+template <typename T, int K>
+constexpr bool use_pipe() { return pipe_words<T, K>() != 0; }
 
 constexpr int kLaneR = 16;  // words per lane in the non-pipelined kernel
 
@@ -974,7 +998,7 @@ template <typename T, int K>
 uint32_t wave_bytes_for(uint32_t band_bytes)
 {
     constexpr int P = passes_for<T, K>();
-    constexpr int R = use_pipe<T, K>() ? 8 : kLaneR;
+    constexpr int R = use_pipe<T, K>() ? pipe_words<T, K>() / 64 : kLaneR;
     constexpr int CW = 64 * R;
     constexpr int QS = CW + CW / (K ? P : CW) + 1;
     const uint32_t win_off = (band_bytes + QS * 6 + 7) & ~7u;
@@ -999,7 +1023,7 @@ fmx_status launch_kw(fmx_ctx *ctx, SpmmArgs a, const Plan *plan, hipStream_t str
                                          160 * 1024));
         attr_set = true;
     }
-    const uint32_t blocks = (plan->nitems + kWPB - 1) / kWPB;
+    const uint32_t blocks = (a.nitems + kWPB - 1) / kWPB;
     if (blocks) {
         hipLaunchKernelGGL(kern, dim3(blocks), dim3(64 * kWPB), lds, stream, a);
         FMX_HIP(ctx, hipGetLastError());
@@ -1024,7 +1048,7 @@ fmx_status launch_k(fmx_ctx *ctx, SpmmArgs a, const Plan *plan, hipStream_t stre
                                          160 * 1024));
         attr_set = true;
     }
-    const uint32_t blocks = (plan->nitems + kWPB - 1) / kWPB;
+    const uint32_t blocks = (a.nitems + kWPB - 1) / kWPB;
     if (blocks) {
         hipLaunchKernelGGL(kern, dim3(blocks), dim3(64 * kWPB), lds, stream, a);
         FMX_HIP(ctx, hipGetLastError());
@@ -1035,18 +1059,19 @@ fmx_status launch_k(fmx_ctx *ctx, SpmmArgs a, const Plan *plan, hipStream_t stre
 template <typename T>
 fmx_status launch_t(fmx_ctx *ctx, const fmx_spm *spm, const Plan *plan, int k,
                     const void *X, int64_t ldx, void *Y, int64_t ldy, void *partial,
-                    hipStream_t stream)
+                    hipStream_t stream, const uint16_t *img, uint32_t item_begin, uint32_t item_end,
+                    bool reduce)
 {
     SpmmArgs a;
-    a.img = (const uint16_t *)spm->d_image;
+    a.img = img;
     a.dir = spm->d_dir;
-    a.items = plan->d_items;
+    a.items = plan->d_items + item_begin;
     a.X = X;
     a.Y = Y;
     a.partial = partial;
     a.ldx = ldx;
     a.ldy = ldy;
-    a.nitems = plan->nitems;
+    a.nitems = item_end - item_begin;
     a.hs = plan->hs;
     // pad band rows by one element to spread the LDS adds over the banks
     a.band_stride = (uint32_t)k + ((k >= 4 && getenv("FMX_NO_PAD") == nullptr) ? 1u : 0u);
@@ -1073,7 +1098,7 @@ fmx_status launch_t(fmx_ctx *ctx, const fmx_spm *spm, const Plan *plan, int k,
     default: st = launch_k<T, 0>(ctx, a, plan, stream); break;
     }
     if (st) return st;
-    if (plan->nsplits) {
+    if (reduce && plan->nsplits) {
         hipLaunchKernelGGL((spmm_reduce<T, A>), dim3(plan->nsplits), dim3(256), 0, stream,
                            plan->d_splits, (const A *)partial, plan->hs, (uint32_t)k, (T *)Y, ldy);
         FMX_HIP(ctx, hipGetLastError());
@@ -1085,13 +1110,14 @@ fmx_status launch_t(fmx_ctx *ctx, const fmx_spm *spm, const Plan *plan, int k,
 
 fmx_status launch_spmm(fmx_ctx *ctx, const fmx_spm *spm, const Plan *plan, fmx_dtype dt,
                        int k, const void *X, int64_t ldx, void *Y, int64_t ldy,
-                       void *partial, hipStream_t stream)
+                       void *partial, hipStream_t stream, const uint16_t *img,
+                       uint32_t ib, uint32_t ie, bool reduce)
 {
     switch (dt) {
-    case FMX_I32: return launch_t<int32_t>(ctx, spm, plan, k, X, ldx, Y, ldy, partial, stream);
-    case FMX_I64: return launch_t<int64_t>(ctx, spm, plan, k, X, ldx, Y, ldy, partial, stream);
-    case FMX_F32: return launch_t<float>(ctx, spm, plan, k, X, ldx, Y, ldy, partial, stream);
-    case FMX_F64: return launch_t<double>(ctx, spm, plan, k, X, ldx, Y, ldy, partial, stream);
+    case FMX_I32: return launch_t<int32_t>(ctx, spm, plan, k, X, ldx, Y, ldy, partial, stream, img, ib, ie, reduce);
+    case FMX_I64: return launch_t<int64_t>(ctx, spm, plan, k, X, ldx, Y, ldy, partial, stream, img, ib, ie, reduce);
+    case FMX_F32: return launch_t<float>(ctx, spm, plan, k, X, ldx, Y, ldy, partial, stream, img, ib, ie, reduce);
+    case FMX_F64: return launch_t<double>(ctx, spm, plan, k, X, ldx, Y, ldy, partial, stream, img, ib, ie, reduce);
     default: return fail(ctx, FMX_ETYPE, "wrong matrix type in spmm");
     }
 }

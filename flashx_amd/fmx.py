@@ -38,7 +38,8 @@ class FmxError(RuntimeError):
 class fmx_spm_info(ctypes.Structure):
     _fields_ = [("nrows", _u64), ("ncols", _u64), ("nnz", _u64), ("block_h", _u64),
                 ("block_w", _u64), ("nbr", _u64), ("image_bytes", _u64),
-                ("tile_bytes", _u64), ("entry_size", _u64), ("ntiles", _u64)]
+                ("tile_bytes", _u64), ("entry_size", _u64), ("ntiles", _u64),
+                ("stream_chunks", _u64)]
 
 
 class fmx_eigen_opts(ctypes.Structure):
@@ -75,6 +76,11 @@ SIGNATURES = {
                                        ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(_vp)]),
     "fmx_spm_load": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(_vp)]),
     "fmx_spm_dump": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_char_p]),
+    "fmx_spm_open": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_char_p, _u64, ctypes.POINTER(_vp)]),
+    "fmx_spm_upload_streamed": (ctypes.c_int, [_vp, _vp, ctypes.c_size_t, _vp, ctypes.c_size_t, _u64,
+                                               ctypes.POINTER(_vp)]),
+    "fmx_spm_open_index": (ctypes.c_int, [_vp, ctypes.c_char_p, _vp, _vp, ctypes.c_size_t, _u64,
+                                          ctypes.POINTER(_vp)]),
     "fmx_spmm": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int, _vp, _i64, _vp, _i64, _vp]),
     "fmx_spmm_alg_bytes": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int,
                                           ctypes.POINTER(_u64)]),
@@ -187,7 +193,8 @@ def default_context(device=0):
 
 
 class SparseMatrix:
-    """fm::sparse_matrix over a 2D tile image resident in HBM."""
+    """fm::sparse_matrix over a 2D tile image, resident in HBM or (open,
+    upload_streamed) kept in host memory and streamed through HBM."""
 
     def __init__(self, ctx, handle):
         self.ctx = ctx
@@ -218,6 +225,34 @@ class SparseMatrix:
         ctx.check(lib().fmx_spm_load(ctx.handle, os.fsencode(mat_file), os.fsencode(idx_file), ctypes.byref(h)),
                   "fmx_spm_load")
         return cls(ctx, h)
+
+    @classmethod
+    def open(cls, mat_file, idx_file, chunk_bytes=1 << 30, ctx=None):
+        """Out-of-core: the image of a .mat / .mat_idx pair stays in pinned
+        host memory and each multiply streams it through HBM in block-row
+        chunks of at most chunk_bytes (the SAFS-backed sparse_matrix::create)."""
+        ctx = ctx or default_context()
+        h = _vp()
+        ctx.check(lib().fmx_spm_open(ctx.handle, os.fsencode(mat_file), os.fsencode(idx_file), int(chunk_bytes),
+                                     ctypes.byref(h)), "fmx_spm_open")
+        return cls(ctx, h)
+
+    @classmethod
+    def upload_streamed(cls, image, offs, chunk_bytes=1 << 30, ctx=None):
+        """Out-of-core form of upload(): the image is kept in pinned host memory."""
+        ctx = ctx or default_context()
+        img = np.ascontiguousarray(np.frombuffer(image, np.uint8) if isinstance(image, (bytes, bytearray))
+                                   else image, dtype=np.uint8)
+        offs = np.ascontiguousarray(offs, dtype=np.int64)
+        h = _vp()
+        ctx.check(lib().fmx_spm_upload_streamed(ctx.handle, img.ctypes.data, img.nbytes, offs.ctypes.data,
+                                                len(offs) - 1, int(chunk_bytes), ctypes.byref(h)),
+                  "fmx_spm_upload_streamed")
+        return cls(ctx, h)
+
+    @property
+    def streamed(self):
+        return self.info.stream_chunks > 0
 
     def dump(self, mat_file, idx_file):
         """Writes the .mat / .mat_idx pair (SpM_2d_storage::dump, SpM_2d_index::dump)."""

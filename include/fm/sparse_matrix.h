@@ -31,6 +31,40 @@
 
 #include "../fmx.h"
 
+/* safs::create_io_factory (libsafs/io_interface.h:518-525) names the file a
+ * matrix is streamed from.  Here the file is a regular host file: it is read
+ * once into pinned host memory and streamed through HBM in block-row chunks
+ * of at most chunk_bytes on every multiply.  The access method is accepted
+ * for the reference's signature and does not change the streaming. */
+namespace safs {
+enum { REMOTE_ACCESS = 3, GLOBAL_CACHE_ACCESS = 4, PART_GLOBAL_ACCESS = 5 };
+
+class file_io_factory {
+    std::string name_;
+    size_t chunk_bytes_;
+
+public:
+    typedef std::shared_ptr<file_io_factory> shared_ptr;
+    file_io_factory(const std::string &name, size_t chunk_bytes) : name_(name), chunk_bytes_(chunk_bytes) {}
+    const std::string &get_name() const { return name_; }
+    size_t get_chunk_bytes() const { return chunk_bytes_; }
+};
+
+inline file_io_factory::shared_ptr create_io_factory(const std::string &file_name, const int access_option,
+                                                     size_t chunk_bytes = size_t(1) << 30)
+{
+    (void)access_option;
+    return file_io_factory::shared_ptr(new file_io_factory(file_name, chunk_bytes));
+}
+
+inline bool exist_safs_file(const std::string &file_name)
+{
+    FILE *f = fopen(file_name.c_str(), "rb");
+    if (f) fclose(f);
+    return f != nullptr;
+}
+}  // namespace safs
+
 namespace fm {
 
 class wrong_format : public std::runtime_error {
@@ -314,6 +348,37 @@ public:
         a->transpose_ = t;
         return a;
     }
+
+    /* sparse_matrix::create(SpM_2d_index::ptr, file_io_factory) (matrix/
+     * sparse_matrix.h:534-535, .cpp:693-698; matrix/test/test-2d_multiply.cpp:
+     * 156-158): out of core.  The image stays in host memory and every
+     * multiply streams it through HBM (fmx_spm_open_index). */
+    static ptr create(SpM_2d_index::ptr index, safs::file_io_factory::shared_ptr mat_io_fac)
+    {
+        if (!index || !mat_io_fac || !init_flash_matrix()) return ptr();
+        const std::vector<char> &h = index->get_header_page();
+        if (h.size() < 64) throw wrong_format("index without a header page");
+        fmx_spm *s = nullptr;
+        const fmx_status st = fmx_spm_open_index(detail::global_ctx(), mat_io_fac->get_name().c_str(), h.data(),
+                                                 index->get_offs().data(), index->get_num_block_rows(),
+                                                 mat_io_fac->get_chunk_bytes(), &s);
+        if (st == FMX_EFORMAT) throw wrong_format(fmx_ctx_last_error(detail::global_ctx()));
+        if (!detail::check(st, "sparse_matrix::create")) return ptr();
+        return ptr(new sparse_matrix(s, true));
+    }
+
+    static ptr create(SpM_2d_index::ptr index, safs::file_io_factory::shared_ptr mat_io_fac,
+                      SpM_2d_index::ptr t_index, safs::file_io_factory::shared_ptr t_mat_io_fac)
+    {
+        ptr a = create(index, mat_io_fac), t = create(t_index, t_mat_io_fac);
+        if (!a || !t) return ptr();
+        a->symmetric_ = t->symmetric_ = false;
+        a->transpose_ = t;
+        return a;
+    }
+
+    /* Whether the image is resident in HBM (false for the streamed form). */
+    bool is_in_mem() const { return info_.stream_chunks == 0; }
 
     /* Writes this matrix as a .mat / .mat_idx pair (what the reference's
      * converters do with SpM_2d_storage::dump and SpM_2d_index::dump). */

@@ -62,6 +62,8 @@ typedef struct {
     uint64_t tile_bytes;       /* image_bytes - 4096                           */
     uint64_t entry_size;       /* 0 = binary                                   */
     uint64_t ntiles;           /* stored tiles (including empty stubs)         */
+    uint64_t stream_chunks;    /* 0: image in HBM; else block-row chunks the   */
+                               /* host-resident image is streamed in per SpMM  */
 } fmx_spm_info;
 
 /* ---- context -------------------------------------------------------------
@@ -128,6 +130,31 @@ fmx_status fmx_spm_destroy(fmx_spm *spm);
  * (SpM_2d_storage::dump, SpM_2d_index::dump, sparse_matrix_format.cpp:
  * 164-181,417-420).  Unreadable or unwritable files give FMX_EINVAL. */
 fmx_status fmx_spm_load(fmx_ctx *ctx, const char *mat_file, const char *idx_file, fmx_spm **out);
+
+/* Out-of-core matrices: the image stays in host memory and every SpMM
+ * streams it through HBM.  fmx_spm_open replaces the SAFS-backed
+ * sparse_matrix::create(SpM_2d_index::ptr, file_io_factory) (matrix/
+ * sparse_matrix.h:534-535, .cpp:693-698) and its block-row I/O (matrix/
+ * matrix_io.cpp:188-236, sparse_matrix.cpp:796-830): the .mat file is read
+ * into pinned host memory (the index file as for fmx_spm_load), validated,
+ * and cut into chunks of whole block rows of at most chunk_bytes (>= 4096;
+ * a larger block row is a chunk of its own).  The tile directory and work
+ * plans live in HBM, the image does not: fmx_spmm (and every caller of it)
+ * then copies chunk c + 1 into one of two HBM staging buffers on the
+ * matrix's own copy stream while the kernel of chunk c runs on the caller's
+ * stream.  Results are bit-identical to the resident matrix's.  Calls on
+ * one streamed matrix from several host threads at once are not supported
+ * (they share the staging buffers).  fmx_spm_upload_streamed does the same
+ * from an image in host memory (copied into pinned memory), and
+ * fmx_spm_open_index from the .mat file plus an index already in memory: the
+ * index's header page (its first 64 bytes must equal the file's) and its
+ * nbr+1 block-row offsets. */
+fmx_status fmx_spm_open(fmx_ctx *ctx, const char *mat_file, const char *idx_file, uint64_t chunk_bytes,
+                        fmx_spm **out);
+fmx_status fmx_spm_upload_streamed(fmx_ctx *ctx, const void *image, size_t bytes, const int64_t *offs, size_t nbr,
+                                   uint64_t chunk_bytes, fmx_spm **out);
+fmx_status fmx_spm_open_index(fmx_ctx *ctx, const char *mat_file, const void *header, const int64_t *offs,
+                              size_t nbr, uint64_t chunk_bytes, fmx_spm **out);
 fmx_status fmx_spm_dump(const fmx_spm *spm, const char *mat_file, const char *idx_file);
 
 /* ---- SpMM ------------------------------------------------------------------

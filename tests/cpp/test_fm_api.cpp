@@ -67,6 +67,15 @@ int main()
         std::vector<long> yb(n);
         Yb->copy_to_host(yb.data());
         if (yb != y) return 1;
+        // out of core: the same file streamed through HBM in 4 KiB chunks
+        // (sparse_matrix::create(index, safs::create_io_factory(...)),
+        // matrix/test/test-2d_multiply.cpp:156-158)
+        fm::sparse_matrix::ptr C =
+            fm::sparse_matrix::create(index, safs::create_io_factory(mat, safs::REMOTE_ACCESS, 4096));
+        if (!C || C->is_in_mem() || !B->is_in_mem() || C->get_nnz() != A->get_nnz()) return 1;
+        std::vector<long> yc(n);
+        C->multiply(X)->copy_to_host(yc.data());
+        if (yc != y) return 1;
         bool io_threw = false;
         try {
             fm::SpM_2d_index::load(std::string(dir) + "/missing.mat_idx");

@@ -105,7 +105,9 @@ def test_widths(fmx, orc, rmat16, k):
     spm = fmx.SparseMatrix.upload(img.data, img.offs)
     rng = np.random.default_rng(k)
     check(fmx, orc, img, rng.random((img.n, k)).astype(np.float32), spm)
+    check(fmx, orc, img, rng.random((img.n, k)), spm)
     check(fmx, orc, img, rng.integers(-9, 9, (img.n, k)).astype(np.int64), spm)
+    check(fmx, orc, img, rng.integers(-9, 9, (img.n, k)).astype(np.int32), spm)
 
 
 def test_strided_operands(fmx, orc, rmat16):
@@ -136,6 +138,10 @@ def test_split_bands_and_determinism(fmx, orc, rmat16):
         spm = fmx.SparseMatrix.upload(img.data, img.offs)
         Y1, _ = check(fmx, orc, img, X, spm)
         check(fmx, orc, img, Xi, spm)
+        # widths drained in 256-word chunks
+        check(fmx, orc, img, rng.random((img.n, 32)).astype(np.float32), spm)
+        check(fmx, orc, img, rng.random((img.n, 16)), spm)
+        check(fmx, orc, img, rng.integers(-9, 9, (img.n, 16)).astype(np.int64), spm)
     finally:
         del os.environ["FMX_ITEM_CAP"]
     spm2 = fmx.SparseMatrix.upload(img.data, img.offs)
