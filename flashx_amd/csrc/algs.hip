@@ -1,4 +1,4 @@
-// algs.hip -- SpMV callers of the SpMM path: PageRank.
+// algs.hip -- callers of the SpMM path: PageRank (k = 1) and NMF (k columns).
 //
 // fmx_pagerank restates matrix/libmatrix-algs/PageRank.cpp:49-137 on the
 // device, FP32 as the reference computes it:
@@ -14,6 +14,7 @@
 // only the SpMV sum differs (FP64 accumulation, rounded once).
 #include <vector>
 
+#include "dense.h"
 #include "fmx_internal.h"
 
 namespace fmx {
@@ -63,6 +64,15 @@ __global__ void __launch_bounds__(kT) pr_update(const float *out, const float *p
     }
     for (int d = 32; d; d >>= 1) c += __shfl_xor(c, d, 64);
     if ((threadIdx.x & 63) == 0 && c) atomicAdd(conv, c);
+}
+
+// X = max(X * N, eps) / (D + eps), element-wise in FP32 with each step
+// rounded (fm.pmax2(X * N, eps) / (den + eps), NMF.cpp:180-192,206-217).
+__global__ void __launch_bounds__(kT) nmf_mult_update(float *X, const float *N, const float *D, uint64_t len,
+                                                      float eps)
+{
+    for (uint64_t i = blockIdx.x * (uint64_t)kT + threadIdx.x; i < len; i += (uint64_t)gridDim.x * kT)
+        X[i] = __fdiv_rn(fmaxf(__fmul_rn(X[i], N[i]), eps), __fadd_rn(D[i], eps));
 }
 
 }  // namespace
@@ -122,6 +132,87 @@ fmx_status fmx_pagerank(fmx_ctx *ctx, const fmx_spm *A, const fmx_spm *At, int m
     (void)hipStreamSynchronize(s);
     (void)hipFree(buf);
     (void)hipFree(conv);
+    return st;
+}
+
+// NMF by Lee-Seung multiplicative updates, matrix/libmatrix-algs/NMF.cpp:
+// 164-230 (update_lee) and 259-316 (NMF), with H kept transposed (tH, m x k
+// row-major: the reference's L_COL k x m H byte for byte).  Per iteration:
+//   tD  = tH * (W^T W)                      (:168-173)
+//   tH  = max(tH .* (A^T W), eps) ./ (tD + eps)   (:176-193)
+//   D2  = W * (tH^T tH)                     (:201-208)
+//   W   = max(W .* (A tH), eps) ./ (D2 + eps)     (:211-224)
+//   WtW = W^T W                             (:227-228)
+// and after iterations 0, 5, 10, ... the distance ||A - W H||_F^2 =
+// nnz - 2 tr((A^T W)^T tH) + tr((tH WtW)^T tH) (Fnorm, :110-126).  The SpMMs
+// are FP32 (FP64 sums rounded once), the Gram and k x k products accumulate
+// in FP64 (the reference: FP32 BLAS), the update is FP32.
+fmx_status fmx_nmf(fmx_ctx *ctx, const fmx_spm *A, const fmx_spm *At, int k, int max_iters, float *W, float *tH,
+                   double *dist, void *stream)
+{
+    if (!ctx || !A || !W || !tH) return fail(ctx, FMX_EINVAL, "null argument");
+    if (A->esz) return fail(ctx, FMX_EUNSUPPORTED, "can only handle binary sparse matrix");
+    if (!At) {
+        if (A->nrows != A->ncols) return fail(ctx, FMX_EDIM, "a non-square matrix needs its transpose");
+        At = A;
+    }
+    if (At->esz) return fail(ctx, FMX_EUNSUPPORTED, "can only handle binary sparse matrix");
+    if (At->nrows != A->ncols || At->ncols != A->nrows) return fail(ctx, FMX_EDIM, "At is not A's transpose");
+    if (k < 1 || k > 64) return fail(ctx, FMX_EDIM, "NMF rank must be in [1, 64]");
+    if (max_iters < 0) return fail(ctx, FMX_EINVAL, "max_iters must be >= 0");
+    FMX_HIP(ctx, hipSetDevice(ctx->device));
+    hipStream_t s = (hipStream_t)stream;
+    const uint64_t n = A->nrows, m = A->ncols;
+    const float eps = 10e-9f;
+    float *buf = nullptr;
+    double *g = nullptr;
+    const uint64_t big = std::max(n, m) * (uint64_t)k;
+    FMX_HIP(ctx, hipMalloc(&buf, 2 * big * sizeof(float)));
+    if (hipMalloc(&g, 2 * (size_t)k * k * sizeof(double)) != hipSuccess) {
+        (void)hipFree(buf);
+        return fail(ctx, FMX_ENOMEM, "NMF Gram buffers");
+    }
+    float *P = buf, *Q = buf + big;  // (m|n) x k products and denominators
+    double *G = g, *G2 = g + (size_t)k * k;
+    std::vector<double> h1((size_t)k * k), h2((size_t)k * k);
+    auto upd = [&](float *X, const float *N, const float *D, uint64_t rows) {
+        const uint64_t len = rows * (uint64_t)k;
+        hipLaunchKernelGGL(nmf_mult_update, dim3(grid_for_n(ctx, len)), dim3(kT), 0, s, X, N, D, len, eps);
+        return hipGetLastError() == hipSuccess ? FMX_OK : fail(ctx, FMX_EHIP, "NMF update launch");
+    };
+    fmx_status st = gram(ctx, FMX_F32, FMX_GRAM_F64, n, k, k, W, k, W, k, G, s);  // W^T W
+    for (int it = 0; !st && it < max_iters; it++) {
+        if (!st) st = gemm_tall(ctx, FMX_F32, m, k, k, 1.0, tH, k, G, 0.0, Q, k, s);
+        if (!st) st = fmx_spmm(ctx, At, FMX_F32, k, W, k, P, k, s);
+        if (!st) st = upd(tH, P, Q, m);
+        if (!st) st = gram(ctx, FMX_F32, FMX_GRAM_F64, m, k, k, tH, k, tH, k, G2, s);  // H H^T
+        if (!st) st = gemm_tall(ctx, FMX_F32, n, k, k, 1.0, W, k, G2, 0.0, Q, k, s);
+        if (!st) st = fmx_spmm(ctx, A, FMX_F32, k, tH, k, P, k, s);
+        if (!st) st = upd(W, P, Q, n);
+        if (!st) st = gram(ctx, FMX_F32, FMX_GRAM_F64, n, k, k, W, k, W, k, G, s);
+        if (!st && dist && it % 5 == 0) {
+            st = fmx_spmm(ctx, At, FMX_F32, k, W, k, P, k, s);                            // A^T W
+            if (!st) st = gram(ctx, FMX_F32, FMX_GRAM_F64, m, k, k, P, k, tH, k, G2, s);  // (A^T W)^T tH
+            if (!st && hipMemcpyAsync(h1.data(), G2, h1.size() * 8, hipMemcpyDeviceToHost, s) != hipSuccess)
+                st = fail(ctx, FMX_EHIP, "NMF distance");
+            if (!st) st = gemm_tall(ctx, FMX_F32, m, k, k, 1.0, tH, k, G, 0.0, Q, k, s);  // tH W^T W
+            if (!st) st = gram(ctx, FMX_F32, FMX_GRAM_F64, m, k, k, Q, k, tH, k, G2, s);
+            if (!st && (hipMemcpyAsync(h2.data(), G2, h2.size() * 8, hipMemcpyDeviceToHost, s) != hipSuccess ||
+                        hipStreamSynchronize(s) != hipSuccess))
+                st = fail(ctx, FMX_EHIP, "NMF distance");
+            if (!st) {
+                double t1 = 0, t2 = 0;
+                for (int j = 0; j < k; j++) {
+                    t1 += h1[(size_t)j * k + j];
+                    t2 += h2[(size_t)j * k + j];
+                }
+                dist[it / 5] = (double)A->nnz - 2.0 * t1 + t2;
+            }
+        }
+    }
+    (void)hipStreamSynchronize(s);
+    (void)hipFree(buf);
+    (void)hipFree(g);
     return st;
 }
 

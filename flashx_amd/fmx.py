@@ -79,6 +79,7 @@ SIGNATURES = {
     "fmx_spm_open": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_char_p, _u64, ctypes.POINTER(_vp)]),
     "fmx_spm_upload_streamed": (ctypes.c_int, [_vp, _vp, ctypes.c_size_t, _vp, ctypes.c_size_t, _u64,
                                                ctypes.POINTER(_vp)]),
+    "fmx_nmf": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, _vp]),
     "fmx_spm_open_index": (ctypes.c_int, [_vp, ctypes.c_char_p, _vp, _vp, ctypes.c_size_t, _u64,
                                           ctypes.POINTER(_vp)]),
     "fmx_spmm": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int, _vp, _i64, _vp, _i64, _vp]),
@@ -470,6 +471,33 @@ def compute_eigen(spm, nev=16, block_size=16, num_blocks=6, tol=1e-6, max_restar
               "fmx_eigen_normalized")
     stats = {f: getattr(st, f) for f, _ in fmx_eigen_stats._fields_}
     return evals, vecs, resid, stats
+
+
+def nmf(A, k, max_iters=10, At=None, W=None, tH=None, seed=0):
+    """fm::alg::NMF (matrix/libmatrix-algs/NMF.cpp): returns (W n x k, tH =
+    H^T m x k, distances after iterations 0, 5, 10, ...).  W and tH start
+    U[0,1) from `seed` unless given (device float32 tensors, updated in
+    place)."""
+    torch = _torch()
+    ctx = A.ctx
+    dev = "cuda:%d" % ctx.device
+    n, m = A.get_num_rows(), A.get_num_cols()
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    if W is None:
+        W = torch.rand((n, k), dtype=torch.float32, device=dev, generator=g)
+    if tH is None:
+        tH = torch.rand((m, k), dtype=torch.float32, device=dev, generator=g)
+    if W.shape != (n, k) or tH.shape != (m, k) or W.dtype != torch.float32 or tH.dtype != torch.float32 \
+            or not W.is_contiguous() or not tH.is_contiguous():
+        raise FmxError(EDIM, "W must be n x k and tH m x k contiguous float32")
+    nd = (max_iters + 4) // 5
+    dist = (ctypes.c_double * max(nd, 1))()
+    torch.cuda.synchronize()
+    ctx.check(lib().fmx_nmf(ctx.handle, A.handle, At.handle if At is not None else None, k, max_iters,
+                            W.data_ptr(), tH.data_ptr(), dist, ctx.stream_ptr()), "fmx_nmf")
+    torch.cuda.synchronize()
+    return W, tH, [dist[i] for i in range(nd)]
 
 
 def pagerank(A, At=None, max_iters=30, damping=0.85):

@@ -27,6 +27,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../fmx.h"
@@ -492,6 +493,43 @@ inline dense_matrix::ptr PageRank(sparse_matrix::ptr mat, size_t max_niters, flo
         return dense_matrix::ptr();
     return pr;
 }
+
+namespace alg {
+/* fm::alg::NMF (matrix/libmatrix-algs/NMF.cpp:259-316): Lee-Seung updates of
+ * a binary matrix from a U[0,1) start (a SplitMix64 stream from `seed`; the
+ * reference's own random stream is not reproducible here).  Returns W
+ * (n x k) and H transposed (m x k row-major, which is the reference's
+ * column-major k x m H byte for byte), or null pointers.  num_in_mem is kept
+ * for the reference's signature; everything is resident in HBM (or streamed,
+ * for a matrix created from a file factory). */
+inline std::pair<dense_matrix::ptr, dense_matrix::ptr> NMF(sparse_matrix::ptr mat, size_t k, size_t max_niters,
+                                                           size_t num_in_mem = 0, uint64_t seed = 1)
+{
+    (void)num_in_mem;
+    std::pair<dense_matrix::ptr, dense_matrix::ptr> none;
+    if (!mat) return none;
+    const size_t n = mat->get_num_rows(), m = mat->get_num_cols();
+    std::vector<float> w(n * k), h(m * k);
+    uint64_t x = seed;
+    auto next = [&x]() {  // SplitMix64 -> U[0,1) with 24 random bits
+        uint64_t z = (x += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return (float)((z ^ (z >> 31)) >> 40) * (1.0f / 16777216.0f);
+    };
+    for (float &v : w) v = next();
+    for (float &v : h) v = next();
+    dense_matrix::ptr W = dense_matrix::create_from_host(w.data(), n, k, FMX_F32);
+    dense_matrix::ptr tH = dense_matrix::create_from_host(h.data(), m, k, FMX_F32);
+    sparse_matrix::ptr t = mat->transpose();
+    if (!W || !tH || !t ||
+        !detail::check(fmx_nmf(detail::global_ctx(), mat->handle(), t->handle(), (int)k, (int)max_niters,
+                               (float *)W->get_raw_arr(), (float *)tH->get_raw_arr(), nullptr, nullptr),
+                       "NMF"))
+        return none;
+    return std::make_pair(W, tH);
+}
+}  // namespace alg
 
 }  // namespace fm
 

@@ -255,6 +255,20 @@ fmx_status fmx_eigen_normalized(fmx_ctx *ctx, const fmx_spm *spm, const fmx_eige
 fmx_status fmx_pagerank(fmx_ctx *ctx, const fmx_spm *A, const fmx_spm *At, int max_iters, float damping,
                         float *pr, int *iters, void *stream);
 
+/* NMF of a binary A (n x m) by Lee-Seung multiplicative updates, as
+ * fm::alg::NMF (matrix/libmatrix-algs/NMF.cpp:164-230,259-316) computes it:
+ * each iteration updates H from A^T W and (W^T W) H, then W from A H^T and
+ * W (H H^T), with X = max(X .* num, 1e-8) ./ (den + 1e-8) in FP32.  W (n x k)
+ * and tH = H^T (m x k; the reference's column-major k x m H) are device FP32
+ * row-major arrays holding the start on entry (the reference draws both
+ * U[0,1)) and the result on return.  At is A's transpose (NULL: A is
+ * symmetric).  dist (host, NULL to skip) receives ||A - W H||_F^2 after
+ * iterations 0, 5, 10, ... (Fnorm, NMF.cpp:110-126): ceil(max_iters/5)
+ * values.  1 <= k <= 64; a weighted A is FMX_EUNSUPPORTED as in the
+ * reference. */
+fmx_status fmx_nmf(fmx_ctx *ctx, const fmx_spm *A, const fmx_spm *At, int k, int max_iters, float *W, float *tH,
+                   double *dist, void *stream);
+
 /* ---- seeded generators (device output) ------------------------------------
  * R-MAT edges with the project's SplitMix64 specification (the reference's
  * generator, flash-graph/tools/rmat-gen.cpp:43-48, is a time-seeded Boost

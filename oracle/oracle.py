@@ -278,6 +278,44 @@ def pagerank(A, At, max_iters=30, damping=0.85):
     return pr, it.value
 
 
+def nmf(A, At, W, tH, iters, nnz):
+    """NMF by Lee-Seung multiplicative updates, restating
+    matrix/libmatrix-algs/NMF.cpp:164-230 (update_lee), 259-316 (the loop) and
+    110-126 (Fnorm), with H kept transposed (tH = H^T, m x k):
+        tD  = tH (W^T W)                              (:168-173)
+        tH  = max(tH * (A^T W), eps) / (tD + eps)     (:176-193)
+        D2  = W (tH^T tH)                             (:201-208)
+        W   = max(W * (A tH), eps) / (D2 + eps)       (:211-224)
+    eps = 10e-9 (:166).  Distances after iterations 0, 5, ...:
+    nnz - 2 sum((A^T W) * tH) + sum((tH W^T W) * tH).  Numerics follow
+    fmx_nmf: sparse products are FP64 sums of the FP32 operands rounded once
+    to FP32, Gram and k x k products FP64 (rounded to FP32 where a matrix is
+    stored), the element-wise update FP32.  A, At: Images (At = A for a
+    symmetric matrix)."""
+    f32, f64 = np.float32, np.float64
+    eps = f32(10e-9)
+    W = np.array(W, f32)
+    tH = np.array(tH, f32)
+    dist = []
+    WtW = W.astype(f64).T @ W.astype(f64)
+    for it in range(iters):
+        tD = (tH.astype(f64) @ WtW).astype(f32)
+        AtW = spmm(At, W.astype(f64)).astype(f32)
+        tH = np.maximum(tH * AtW, eps) / (tD + eps)
+        HHt = tH.astype(f64).T @ tH.astype(f64)
+        D2 = (W.astype(f64) @ HHt).astype(f32)
+        AH = spmm(A, tH.astype(f64)).astype(f32)
+        W = np.maximum(W * AH, eps) / (D2 + eps)
+        WtW = W.astype(f64).T @ W.astype(f64)
+        if it % 5 == 0:
+            AtW = spmm(At, W.astype(f64)).astype(f32)
+            Q = (tH.astype(f64) @ WtW).astype(f32)
+            t1 = float(np.sum(AtW.astype(f64) * tH.astype(f64)))
+            t2 = float(np.sum(Q.astype(f64) * tH.astype(f64)))
+            dist.append(float(nnz) - 2.0 * t1 + t2)
+    return W, tH, dist
+
+
 def gram_f64(A, B):
     A = np.ascontiguousarray(A, np.float64)
     B = np.ascontiguousarray(B, np.float64)

@@ -115,6 +115,37 @@ int main()
                 return 1;
             }
     }
+    // NMF over A and its transpose (matrix/libmatrix-algs/NMF.cpp): positive
+    // factors, and more iterations fit A no worse (Lee-Seung monotonicity),
+    // measured here densely on the host
+    {
+        auto fit = [&](size_t iters) {
+            auto f = fm::alg::NMF(A, 4, iters);
+            if (!f.first || !f.second) return -1.0;
+            std::vector<float> w(n * 4), h(n * 4);
+            f.first->copy_to_host(w.data());
+            f.second->copy_to_host(h.data());
+            std::vector<double> a((size_t)n * n, 0.0);
+            for (size_t e = 0; e < src.size(); e++)
+                if (src[e] != dst[e]) a[(size_t)src[e] * n + dst[e]] = 1.0;
+            double d = 0;
+            for (uint32_t i = 0; i < n; i++)
+                for (uint32_t j = 0; j < n; j++) {
+                    double p = 0;
+                    for (int c = 0; c < 4; c++) {
+                        if (!(w[i * 4 + c] > 0) || !(h[j * 4 + c] > 0)) return -1.0;
+                        p += (double)w[i * 4 + c] * h[j * 4 + c];
+                    }
+                    d += (a[(size_t)i * n + j] - p) * (a[(size_t)i * n + j] - p);
+                }
+            return d;
+        };
+        const double d1 = fit(1), d20 = fit(20);
+        if (d1 < 0 || d20 < 0 || d20 > d1) {
+            printf("NMF: %g after 1 iteration, %g after 20\n", d1, d20);
+            return 1;
+        }
+    }
     // error convention: a dimension mismatch is false + log, not an exception
     fm::dense_matrix::ptr bad = fm::dense_matrix::create(n + 1, 1, FMX_I64);
     if (A->multiply(*bad, *Y)) return 1;
