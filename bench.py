@@ -66,6 +66,7 @@ def main():
                     help="also run the R-MAT scale-27, k=32 SpMM with a sampled CPU check")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-stream", action="store_true", help="skip the out-of-core (streamed image) leg")
+    ap.add_argument("--no-floor", action="store_true", help="skip the gather-floor leg")
     ap.add_argument("--stream-chunk", type=int, default=64 << 20,
                     help="block-row chunk bytes of the streamed leg")
     ap.add_argument("--cpu-reps", type=int, default=5)
@@ -107,6 +108,7 @@ def main():
                                flags=F.BUILD_DEDUP)
     torch.cuda.synchronize()
     t_build = time.perf_counter() - t0
+    cols_seq = None if args.no_floor else stream_columns(torch, src, dst, n, args.log2_h, args.log2_w)
     del src, dst
     info = spm.info
     log("built: n=%d nnz=%d tiles=%d image=%d B (%.2f s incl. host validation)"
@@ -168,6 +170,9 @@ def main():
         "cpu_baseline": None,
     }
 
+    if cols_seq is not None:
+        out["gather_floor"] = gather_floor_leg(F, torch, X, cols_seq, ms_per_step)
+        del cols_seq
     if not args.no_stream:
         out["streamed"] = streamed_leg(F, torch, spm, X, Y, args.stream_chunk)
 
@@ -202,6 +207,42 @@ def main():
         out["eigen"] = eigen_leg(F, torch, args.log2_h, args.log2_w)
         out["learned_warm_start"] = learned_leg(F, torch)
     print(json.dumps(out), flush=True)
+
+
+def stream_columns(torch, src, dst, n, lh, lw):
+    """Columns of the non-zeros in the tiles' order -- block row, block
+    column, then row and column inside the tile (the image's order up to the
+    split of each tile into its multi-entry and single-entry rows) -- from
+    the edge list after self-edge removal and dedup, as the build does."""
+    s = src.long() & 0xFFFFFFFF
+    d = dst.long() & 0xFFFFFFFF
+    keep = s != d
+    u = torch.unique(s[keep] * n + d[keep])
+    s, d = u // n, u % n
+    del u, keep
+    bits = max(1, (n - 1).bit_length())
+    key = ((((s >> lh) << (bits - lw)) | (d >> lw)) << (lh + bits)) | ((s & ((1 << lh) - 1)) << bits) | d
+    order = torch.sort(key).indices
+    return d[order].to(torch.int32).contiguous()
+
+
+def gather_floor_leg(F, torch, X, cols, spmm_ms, steps=10):
+    """The gather-only pass over the SpMM's rows of X in the SpMM's order:
+    the memory floor of any kernel fetching the same rows in that order."""
+    for _ in range(2):
+        F.gather_floor(X, cols)
+    torch.cuda.synchronize()
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(steps):
+        F.gather_floor(X, cols)
+    b.record()
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / steps
+    log("gather floor: %.3f ms for %d rows in tile order (SpMM %.2fx of it)" % (ms, cols.numel(), spmm_ms / ms))
+    return {"order": "tile order of the non-zeros", "rows": int(cols.numel()), "ms": round(ms, 4),
+            "rows_per_s": round(cols.numel() / ms * 1e3), "spmm_over_floor": round(spmm_ms / ms, 3)}
 
 
 def streamed_leg(F, torch, spm, X, Y_ref, chunk_bytes, steps=10):

@@ -79,6 +79,7 @@ SIGNATURES = {
     "fmx_spm_open": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_char_p, _u64, ctypes.POINTER(_vp)]),
     "fmx_spm_upload_streamed": (ctypes.c_int, [_vp, _vp, ctypes.c_size_t, _vp, ctypes.c_size_t, _u64,
                                                ctypes.POINTER(_vp)]),
+    "fmx_gather_floor": (ctypes.c_int, [_vp, _u64, _vp, _i64, _u64, _vp, _u64, _vp]),
     "fmx_nmf": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, _vp]),
     "fmx_spm_open_index": (ctypes.c_int, [_vp, ctypes.c_char_p, _vp, _vp, ctypes.c_size_t, _u64,
                                           ctypes.POINTER(_vp)]),
@@ -471,6 +472,20 @@ def compute_eigen(spm, nev=16, block_size=16, num_blocks=6, tol=1e-6, max_restar
               "fmx_eigen_normalized")
     stats = {f: getattr(st, f) for f, _ in fmx_eigen_stats._fields_}
     return evals, vecs, resid, stats
+
+
+def gather_floor(X, cols, stream=None, ctx=None):
+    """Enqueues the gather-only pass over X's rows cols (int32/uint32 device
+    tensor): the memory floor of an SpMM visiting the same rows in the same
+    order (fmx_gather_floor)."""
+    torch = _torch()
+    ctx = ctx or default_context()
+    if X.dim() != 2 or X.stride(1) != 1 or cols.dtype not in (torch.int32, torch.uint32) or not cols.is_contiguous():
+        raise FmxError(EINVAL, "X must be a row-major matrix and cols a contiguous 32-bit tensor")
+    es = X.element_size()
+    st = stream if stream is not None else torch.cuda.current_stream(ctx.device).cuda_stream
+    ctx.check(lib().fmx_gather_floor(ctx.handle, X.shape[1] * es, X.data_ptr(), X.stride(0) * es, X.shape[0],
+                                     cols.data_ptr(), cols.numel(), st), "fmx_gather_floor")
 
 
 def nmf(A, k, max_iters=10, At=None, W=None, tH=None, seed=0):

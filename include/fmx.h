@@ -269,6 +269,17 @@ fmx_status fmx_pagerank(fmx_ctx *ctx, const fmx_spm *A, const fmx_spm *At, int m
 fmx_status fmx_nmf(fmx_ctx *ctx, const fmx_spm *A, const fmx_spm *At, int k, int max_iters, float *W, float *tH,
                    double *dist, void *stream);
 
+/* ---- measurement -------------------------------------------------------------
+ * No reference counterpart: the memory-system floor of the SpMM's dense-row
+ * gathers.  Reads row cols[i] (clamped to x_rows - 1) of X for i < n with the
+ * SpMM's load shape -- 16 bytes per lane, the lanes of a row side by side,
+ * eight loads in flight per lane -- and nothing else.  row_bytes = k * element
+ * size must be 16 times a power of two (at most 1024); X and ldx_bytes are
+ * 16-byte aligned.  Timing it over a matrix's column sequence bounds what any
+ * SpMM fetching the same rows in the same order can reach. */
+fmx_status fmx_gather_floor(fmx_ctx *ctx, uint64_t row_bytes, const void *X, int64_t ldx_bytes, uint64_t x_rows,
+                            const uint32_t *cols, uint64_t n, void *stream);
+
 /* ---- seeded generators (device output) ------------------------------------
  * R-MAT edges with the project's SplitMix64 specification (the reference's
  * generator, flash-graph/tools/rmat-gen.cpp:43-48, is a time-seeded Boost
