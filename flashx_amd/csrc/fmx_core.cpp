@@ -829,4 +829,45 @@ fmx_status fmx_spmm(fmx_ctx *ctx, const fmx_spm *cspm, fmx_dtype dtype, int k,
     return spmm_streamed(ctx, spm, plan, dtype, k, X, ldx, Y, ldy, s);
 }
 
+fmx_status fmx_spmm_host(fmx_ctx *ctx, const fmx_spm *spm, fmx_dtype dtype, int k, const void *X, int64_t ldx,
+                         void *Y, int64_t ldy, uint64_t mem_bytes, void *stream)
+{
+    if (!ctx || !spm) return fail(ctx, FMX_EINVAL, "null context or matrix");
+    const int es = elem_size(dtype);
+    if (!es || dtype == FMX_BF16) return fail(ctx, FMX_ETYPE, "wrong matrix type in spmm");
+    if (k <= 0) return fail(ctx, FMX_EDIM, "k must be positive");
+    if (ldx < k || ldy < k) return fail(ctx, FMX_EDIM, "leading dimension smaller than k");
+    if ((!X && spm->ncols) || (!Y && spm->nrows)) return fail(ctx, FMX_EINVAL, "null dense operand");
+    if (spm->nrows == 0) return FMX_OK;
+    const uint64_t per_col = (spm->ncols + spm->nrows) * (uint64_t)es;
+    const uint64_t kb = std::min<uint64_t>((uint64_t)k, mem_bytes / per_col);
+    if (kb == 0) return fail(ctx, FMX_ENOMEM, "The memory is too small for the SpMM");
+    FMX_HIP(ctx, hipSetDevice(ctx->device));
+    hipStream_t s = (hipStream_t)stream;
+    void *dx = nullptr, *dy = nullptr;
+    FMX_HIP(ctx, hipMalloc(&dx, std::max<uint64_t>(spm->ncols * kb * es, 16)));
+    if (hipMalloc(&dy, spm->nrows * kb * es) != hipSuccess) {
+        (void)hipFree(dx);
+        return fail(ctx, FMX_ENOMEM, "dense panel");
+    }
+    fmx_status st = FMX_OK;
+    for (uint64_t j0 = 0; j0 < (uint64_t)k && !st; j0 += kb) {
+        const uint64_t w = std::min<uint64_t>(kb, (uint64_t)k - j0);
+        if (spm->ncols &&
+            hipMemcpy2DAsync(dx, w * es, (const char *)X + j0 * es, (size_t)ldx * es, w * es, spm->ncols,
+                             hipMemcpyHostToDevice, s) != hipSuccess) {
+            st = fail(ctx, FMX_EHIP, "dense panel copy");
+            break;
+        }
+        st = fmx_spmm(ctx, spm, dtype, (int)w, dx, (int64_t)w, dy, (int64_t)w, s);
+        if (!st && hipMemcpy2DAsync((char *)Y + j0 * es, (size_t)ldy * es, dy, w * es, w * es, spm->nrows,
+                                    hipMemcpyDeviceToHost, s) != hipSuccess)
+            st = fail(ctx, FMX_EHIP, "dense panel copy");
+    }
+    if (hipStreamSynchronize(s) != hipSuccess && !st) st = fail(ctx, FMX_EHIP, "dense panel copy");
+    (void)hipFree(dx);
+    (void)hipFree(dy);
+    return st;
+}
+
 }  // extern "C"

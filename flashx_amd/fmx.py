@@ -79,6 +79,7 @@ SIGNATURES = {
     "fmx_spm_open": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_char_p, _u64, ctypes.POINTER(_vp)]),
     "fmx_spm_upload_streamed": (ctypes.c_int, [_vp, _vp, ctypes.c_size_t, _vp, ctypes.c_size_t, _u64,
                                                ctypes.POINTER(_vp)]),
+    "fmx_spmm_host": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int, _vp, _i64, _vp, _i64, _u64, _vp]),
     "fmx_gather_floor": (ctypes.c_int, [_vp, _u64, _vp, _i64, _u64, _vp, _u64, _vp]),
     "fmx_nmf": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, _vp]),
     "fmx_spm_open_index": (ctypes.c_int, [_vp, ctypes.c_char_p, _vp, _vp, ctypes.c_size_t, _u64,
@@ -366,6 +367,26 @@ class SparseMatrix:
         self.multiply_async(X2, Y)
         torch.cuda.current_stream(self.ctx.device).synchronize()
         return Y.view(-1) if squeeze else Y
+
+    def multiply_host(self, X, mem_bytes):
+        """Y = A*X for a host numpy X, through HBM in column panels that fit
+        mem_bytes (sparse_matrix::multiply(dense_matrix, mem_size) with an
+        external-memory X); returns a host numpy Y."""
+        X = np.ascontiguousarray(X)
+        if X.ndim == 1:
+            X = X.reshape(-1, 1)
+        dt = {np.dtype(np.int32): I32, np.dtype(np.int64): I64, np.dtype(np.float32): F32,
+              np.dtype(np.float64): F64}.get(X.dtype)
+        if dt is None:
+            raise FmxError(ETYPE, "wrong matrix type in spmm creator")
+        if X.shape[0] != self.info.ncols:
+            raise FmxError(EDIM, "dimension mismatch: A is %dx%d, X %s"
+                           % (self.info.nrows, self.info.ncols, X.shape))
+        k = X.shape[1]
+        Y = np.empty((self.info.nrows, k), X.dtype)
+        self.ctx.check(lib().fmx_spmm_host(self.ctx.handle, self.handle, dt, k, X.ctypes.data, k, Y.ctypes.data, k,
+                                           int(mem_bytes), None), "fmx_spmm_host")
+        return Y
 
     def close(self):
         if self.handle:

@@ -171,6 +171,19 @@ fmx_status fmx_spmm(fmx_ctx *ctx, const fmx_spm *spm, fmx_dtype dtype, int k,
                     const void *X, int64_t ldx, void *Y, int64_t ldy,
                     void *stream);
 
+/* Y = A * X with X (ncols x k) and Y (nrows x k) in HOST memory, for dense
+ * operands larger than the HBM set aside for them: the column-panel branch
+ * of sparse_matrix::multiply(dense_matrix, mem_size) (matrix/
+ * sparse_matrix.cpp:773-831, panels at :796-830).  Columns go through HBM in
+ * panels of kb = mem_bytes / ((ncols + nrows) * element size) columns (all
+ * k if they fit): each panel of X is copied in, multiplied with fmx_spmm and
+ * its Y panel copied back.  kb == 0 is FMX_ENOMEM ("The memory is too small
+ * for the SpMM", :801-804).  Synchronous; leading dimensions in elements.
+ * Pinned host buffers copy fastest.  Each column's result is the one
+ * fmx_spmm gives for it. */
+fmx_status fmx_spmm_host(fmx_ctx *ctx, const fmx_spm *spm, fmx_dtype dtype, int k, const void *X, int64_t ldx,
+                         void *Y, int64_t ldy, uint64_t mem_bytes, void *stream);
+
 /* The algorithmic bytes one fmx_spmm moves: tile bytes + 8*(nbr+1) +
  * ncols*k*s (X read once) + nrows*k*s (Y written once). */
 fmx_status fmx_spmm_alg_bytes(const fmx_spm *spm, fmx_dtype dtype, int k,

@@ -147,3 +147,32 @@ def test_streamed_errors(fmx, orc, tmp_path):
     with pytest.raises(fmx.FmxError) as e:
         fmx.SparseMatrix.open(str(tmp_path / "missing.mat"), idx)
     assert e.value.status == fmx.EINVAL
+
+
+@pytest.mark.parametrize("k,dt", [(37, np.int32), (16, np.float32), (5, np.float64)])
+def test_host_operands_in_column_panels(fmx, orc, edges, k, dt):
+    """X and Y in host memory, columns through HBM in panels (the column-panel
+    branch of multiply(dense_matrix, mem_size), sparse_matrix.cpp:796-830):
+    every column equals the device SpMM's (FP64: within 1e-12, since panels
+    of other widths may order the FP64 adds of one row differently), also for
+    a streamed matrix."""
+    s, d = edges
+    n = 1 << 15
+    img = orc.build_2d(s, d, n=n, log2h=8, log2w=13, flags=orc.DEDUP)
+    res = fmx.SparseMatrix.upload(img.data, img.offs)
+    st = fmx.SparseMatrix.upload_streamed(img.data, img.offs, chunk_bytes=1 << 16)
+    rng = np.random.default_rng(k)
+    X = (rng.integers(-9, 10, (n, k)) if np.issubdtype(dt, np.integer) else rng.random((n, k))).astype(dt)
+    Yd = res.multiply(to_dev(X)).cpu().numpy()
+    es = X.dtype.itemsize
+    for mem in (2 * n * es * 3, 2 * n * es * k, 1 << 40):  # panels of 3 columns, exactly k, all
+        for A in (res, st):
+            Yh = A.multiply_host(X, mem)
+            if dt == np.float64:
+                assert np.allclose(Yh, Yd, rtol=1e-12, atol=0), (mem, A.streamed)
+            else:
+                assert np.array_equal(Yh, Yd), (mem, A.streamed)
+    assert close(Yd, orc.spmm(img, X), dt)
+    with pytest.raises(fmx.FmxError) as e:  # not one column fits
+        res.multiply_host(X, 2 * n * es - 1)
+    assert e.value.status == fmx.ENOMEM
