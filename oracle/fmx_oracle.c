@@ -968,6 +968,118 @@ int orc_spmm_range(const uint8_t *image, uint64_t bytes, const int64_t *offs,
     return ORC_OK;
 }
 
+/* ---- tile visiting order inside a task ------------------------------------
+ * Position of cell (x, y) on the Hilbert curve filling an n x n grid (n a
+ * power of two), the published recursive construction (the "xy2d" of the
+ * Wikipedia article that matrix/hilbert_curve.cpp:1-34 names): at each
+ * scale s the quadrant (rx, ry) contributes s*s*((3*rx) ^ ry) and the
+ * coordinates below s are reflected/transposed into that quadrant's frame. */
+uint64_t orc_hilbert_xy2d(uint64_t n, uint64_t x, uint64_t y)
+{
+    uint64_t d = 0;
+    for (uint64_t s = n / 2; s > 0; s /= 2) {
+        const uint64_t rx = (x & s) ? 1 : 0, ry = (y & s) ? 1 : 0;
+        d += s * s * ((3 * rx) ^ ry);
+        x &= s - 1;
+        y &= s - 1;
+        if (ry == 0) {
+            if (rx == 1) {
+                x = s - 1 - x;
+                y = s - 1 - y;
+            }
+            const uint64_t t = x;
+            x = y;
+            y = t;
+        }
+    }
+    return d;
+}
+
+typedef struct { uint64_t d, i, j; } hcell_t;
+
+static int hcell_cmp(const void *a, const void *b)
+{
+    const uint64_t x = ((const hcell_t *)a)->d, y = ((const hcell_t *)b)->d;
+    return x < y ? -1 : x > y;
+}
+
+/*
+ * Y = A*X sequentially, with the block rows taken in tasks of task_rows and
+ * the tiles of each task visited as block_compute_task::run does
+ * (matrix/sparse_matrix.cpp:246-314): super blocks of task_rows x task_rows
+ * tiles (block rows of the task, the next task_rows block columns), each
+ * visited in seq_exec_order (block row by block row, ascending column;
+ * sparse_matrix.h:161-176) or, with hilbert != 0, in hilbert_exec_order
+ * (cells sorted by their Hilbert position, sparse_matrix.cpp:144-186).  The
+ * Hilbert order needs every task to be a full task_rows x task_rows square
+ * (:170-173), so nbr must be a multiple of task_rows, a power of two.
+ */
+int orc_spmm_exec(const uint8_t *image, uint64_t bytes, const int64_t *offs,
+                  uint64_t nbr, int dtype, int k, const void *X, int64_t ldx,
+                  void *Y, int64_t ldy, uint64_t task_rows, int hilbert)
+{
+    img_info_t inf;
+    int rc = read_info(image, bytes, offs, nbr, &inf);
+    if (rc) return rc;
+    const int es = elem_size(dtype);
+    if (!es) return ORC_ETYPE;
+    if (k <= 0 || ldx < k || ldy < k || task_rows == 0 || task_rows > 64) return ORC_EDIM;
+    if (hilbert && ((task_rows & (task_rows - 1)) || nbr % task_rows)) return ORC_EDIM;
+    const uint64_t n = task_rows, cells = n * n;
+    hcell_t *order = (hcell_t *)malloc(cells * sizeof(hcell_t));
+    tile_t *tiles = (tile_t *)malloc(cells * sizeof(tile_t));
+    char *have = (char *)malloc(cells);
+    if (!order || !tiles || !have) {
+        free(order); free(tiles); free(have);
+        return ORC_ENOMEM;
+    }
+    for (uint64_t i = 0; i < n; i++)
+        for (uint64_t j = 0; j < n; j++) {
+            hcell_t *c = &order[i * n + j];
+            c->i = i;
+            c->j = j;
+            c->d = hilbert ? orc_hilbert_xy2d(n, i, j) : i * n + j;
+        }
+    qsort(order, cells, sizeof(hcell_t), hcell_cmp);
+    for (uint64_t r = 0; r < inf.nrows; r++) memset((uint8_t *)Y + r * ldy * es, 0, (size_t)k * es);
+    for (uint64_t b0 = 0; b0 < nbr && !rc; b0 += n) {
+        const uint64_t nb = nbr - b0 < n ? nbr - b0 : n;
+        const uint8_t *it[64], *itend[64];
+        for (uint64_t i = 0; i < nb; i++) {
+            it[i] = image + offs[b0 + i];
+            itend[i] = image + offs[b0 + i + 1];
+        }
+        for (uint64_t sbc = 0;; sbc += n) {
+            int more = 0;
+            memset(have, 0, cells);
+            for (uint64_t i = 0; i < nb && !rc; i++) {
+                while (it[i] < itend[i]) {
+                    tile_t tl;
+                    if ((rc = parse_tile(it[i], itend[i], inf.esz, &tl))) break;
+                    if (tl.nnz == 0) { it[i] += tl.size; continue; }
+                    if (tl.bc >= sbc + n) break;
+                    tiles[i * n + (tl.bc - sbc)] = tl;
+                    have[i * n + (tl.bc - sbc)] = 1;
+                    it[i] += tl.size;
+                }
+                if (it[i] < itend[i]) more = 1;
+            }
+            if (rc) break;
+            for (uint64_t c = 0; c < cells; c++) {
+                const uint64_t i = order[c].i, j = order[c].j;
+                if (i >= nb || !have[i * n + j]) continue;
+                uint8_t *band = (uint8_t *)Y + (b0 + i) * inf.H * ldy * es;
+                run_tile_any(dtype, &tiles[i * n + j], &inf, k, X, ldx, band, ldy);
+            }
+            if (!more) break;
+        }
+    }
+    free(order);
+    free(tiles);
+    free(have);
+    return rc;
+}
+
 int orc_spmm_mt(const uint8_t *image, uint64_t bytes, const int64_t *offs,
                 uint64_t nbr, int dtype, int k, const void *X, int64_t ldx,
                 void *Y, int64_t ldy, int nthreads, int copy_block_rows)

@@ -87,6 +87,13 @@ int orc_spmm(const uint8_t *image, uint64_t bytes, const int64_t *offs,
              uint64_t nbr, int dtype, int k, const void *X, int64_t ldx,
              void *Y, int64_t ldy);
 /* Multi-threaded restatement of the reference engine, timed as baseline. */
+/* Hilbert position of (x, y) in an n x n grid (hilbert_curve.cpp:24-34). */
+uint64_t orc_hilbert_xy2d(uint64_t n, uint64_t x, uint64_t y);
+/* Sequential SpMM with tasks of task_rows block rows whose super blocks are
+ * visited in seq (hilbert = 0) or Hilbert order (sparse_matrix.cpp:120-314). */
+int orc_spmm_exec(const uint8_t *image, uint64_t bytes, const int64_t *offs,
+                  uint64_t nbr, int dtype, int k, const void *X, int64_t ldx,
+                  void *Y, int64_t ldy, uint64_t task_rows, int hilbert);
 int orc_spmm_range(const uint8_t *image, uint64_t bytes, const int64_t *offs,
                    uint64_t nbr, uint64_t br0, uint64_t br1, int dtype, int k,
                    const void *X, int64_t ldx, void *Y, int64_t ldy);

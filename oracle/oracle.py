@@ -260,6 +260,34 @@ def spmm_range(image, X, br0, br1):
     return Y
 
 
+def hilbert_xy2d(n, x, y):
+    L = lib()
+    L.orc_hilbert_xy2d.restype = ctypes.c_uint64
+    L.orc_hilbert_xy2d.argtypes = [ctypes.c_uint64] * 3
+    return int(L.orc_hilbert_xy2d(n, x, y))
+
+
+def spmm_exec(image, X, task_rows, hilbert):
+    """Sequential Y = A*X with tasks of task_rows block rows whose super
+    blocks are visited in seq or Hilbert order (block_compute_task::run)."""
+    X = np.ascontiguousarray(X)
+    if X.ndim == 1:
+        X = X.reshape(-1, 1)
+    dt = _NP2DT[X.dtype]
+    n_rows = int(image.data[24:32].view(np.uint64)[0])
+    k = X.shape[1]
+    Y = np.empty((n_rows, k), X.dtype)
+    L = lib()
+    L.orc_spmm_exec.restype = ctypes.c_int
+    L.orc_spmm_exec.argtypes = [_u8p, ctypes.c_uint64, _i64p, ctypes.c_uint64, ctypes.c_int, ctypes.c_int, _vp,
+                                ctypes.c_int64, _vp, ctypes.c_int64, ctypes.c_uint64, ctypes.c_int]
+    rc = L.orc_spmm_exec(_p(image.data, _u8p), len(image.data), _p(image.offs, _i64p), image.nbr, dt, k,
+                         X.ctypes.data, k, Y.ctypes.data, k, task_rows, int(bool(hilbert)))
+    if rc:
+        raise ValueError("orc_spmm_exec failed with status %d" % rc)
+    return Y
+
+
 def pagerank(A, At, max_iters=30, damping=0.85):
     """(pr float32 array, iterations) as matrix/libmatrix-algs/PageRank.cpp computes them."""
     L = lib()

@@ -207,3 +207,18 @@ def test_row_32767_header(fmx, orc):
                                  log2_h=15, log2_w=15, flags=fmx.BUILD_DEDUP)
     data, offs = spm.download()
     assert np.array_equal(data, img.data) and np.array_equal(offs, img.offs)
+
+
+def test_fp32_within_bound_of_either_tile_order(fmx, orc, rmat16):
+    """With hilbert_order the reference visits each task's tiles along a
+    Hilbert curve (sparse_matrix.cpp:120-186) and its FP32 row sums change
+    rounding; the FP64-accumulated GPU result is within 1e-5 of the
+    reference's sums in both orders."""
+    src, dst = rmat16
+    img = orc.build_2d(src, dst, n=1 << 16, log2h=8, log2w=8)
+    spm = fmx.SparseMatrix.upload(img.data, img.offs)
+    X = np.random.default_rng(21).random((img.n, 16)).astype(np.float32)
+    Y = spm.multiply(to_dev(X)).cpu().numpy()
+    for hil in (False, True):
+        ref = orc.spmm_exec(img, X, 16, hil)
+        assert rel_err(Y, ref) <= 1e-5, hil
