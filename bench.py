@@ -340,23 +340,23 @@ def large_leg(F, torch, scale=27, k=32, steps=5, lh=7, lw=15):
 
 def eigen_leg(F, torch, log2_h, log2_w):
     """Top-16 eigenpairs of D^-1/2 A D^-1/2 for the symmetrised scale-22
-    R-MAT graph: block 16, FP32 SpMM, BF16-MFMA first Gram pass, FP64
-    reorthogonalisation / Rayleigh-Ritz / residuals, tolerance 1e-6."""
+    R-MAT graph: block 16, FP32 SpMM, FP64-MFMA Gram and tall products, FP64
+    Rayleigh-Ritz / residuals, tolerance 1e-6."""
     n = 1 << SCALE
     src, dst = F.rmat_edges(SCALE, EDGE_FACTOR * n, SEED)
     spm = F.SparseMatrix.build(src, dst, n=n, log2_h=log2_h, log2_w=log2_w,
                                flags=F.BUILD_DEDUP | F.BUILD_SYMMETRIZE)
     del src, dst
-    F.compute_eigen(spm, nev=16, block_size=16, tol=1e-2, precision=F.EIG_MIXED, seed=3)  # warm-up
+    F.compute_eigen(spm, nev=16, block_size=16, tol=1e-2, precision=F.EIG_FP32, seed=3)  # warm-up
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    ev, vecs, res, st = F.compute_eigen(spm, nev=16, block_size=16, tol=1e-6, precision=F.EIG_MIXED,
+    ev, vecs, res, st = F.compute_eigen(spm, nev=16, block_size=16, tol=1e-6, precision=F.EIG_FP32,
                                         seed=1, max_restarts=200)
     torch.cuda.synchronize()
     sec = time.perf_counter() - t0
     log("eigen: %.3f s, %d block SpMMs, max residual %.2e" % (sec, st["spmm_blocks"], res.max()))
     return {"workload": "rmat22_ef16 symmetrised, top-16 of D^-1/2 A D^-1/2, block 16, tol 1e-6",
-            "precision": "mixed (fp32 spmm, bf16-mfma gram, fp64 reorth/rr/residual)",
+            "precision": "fp32 spmm, fp64-mfma gram and tall products, fp64 rr/residual",
             "sec": round(sec, 4), "spmm_blocks": int(st["spmm_blocks"]), "restarts": int(st["restarts"]),
             "converged": int(st["converged"]), "max_resid_fp64": float(res.max()),
             "eigenvalues_top3": [round(float(x), 8) for x in ev[:3]], "nnz": int(spm.info.nnz)}

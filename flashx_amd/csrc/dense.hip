@@ -22,6 +22,11 @@ constexpr int kT = 256;
 template <typename T>
 __device__ __forceinline__ double ld64(const T *p) { return (double)*p; }
 
+template <typename T>
+struct alignas(16) Vec16 {
+    T v[16 / sizeof(T)];
+};
+
 // ---- FP64 Gram ----------------------------------------------------------------
 // Block b accumulates G_b = sum over its rows of A[r][:]^T B[r][:] for the
 // output slice [o0, o0 + 4096) of the ka*kb outputs, rows staged in LDS.
@@ -83,6 +88,242 @@ __global__ void __launch_bounds__(kT) reduce_partials(const TP *partial, int nbl
         double v = (double)s;
         out[o] = (TO)(sqrt_out ? sqrt(v) : v);
     }
+}
+
+// ---- FP64 MFMA Gram and tall GEMM ------------------------------------------------
+// v_mfma_f64_16x16x4f64 with operands A: 16 x 4 (lane l holds A[l%16][l/16]),
+// B: 4 x 16 (lane l holds B[l/16][l%16]); products of FP32 inputs are exact
+// in FP64.  For the Gram G = A^T B the rows are the K dimension, so every
+// output tile accumulates over a wave's rows in registers; per-wave tiles go
+// to partial[wave] and are reduced in wave order.
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+constexpr int kGramTiles = 16;  // output tiles per wave (64 FP64 accumulators per lane)
+
+// Staged forms: each wave copies 16 rows of its operands into its own LDS
+// slice with 16-byte loads (the next 16 rows are in flight while the current
+// ones feed the MFMAs), and the MFMA operands are read from LDS.  Rows are
+// padded in LDS so that the lanes of one operand read spread over the banks.
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_wave_barrier();
+    asm volatile("" ::: "memory");
+}
+
+constexpr int kStageRows = 16;
+
+// Copies rows [r, r + 16) x columns [0, k) of M (row stride ld) as 16-byte
+// pieces into registers; pieces past the matrix are zero.  The slice is
+// 16 x KP elements; lane l takes pieces l, l + 64, ...
+template <typename T, int KP>
+struct Stage {
+    static constexpr int V = 16 / sizeof(T);               // elements per piece
+    static constexpr int NP = kStageRows * KP / V / 64;    // pieces per lane
+    Vec16<T> reg[NP > 0 ? NP : 1];
+    __device__ __forceinline__ void load(const T *M, int64_t ld, int k, uint64_t r, uint64_t r1, int lane, int vec)
+    {
+#pragma unroll
+        for (int p = 0; p < NP; p++) {
+            const int e = (p * 64 + lane) * V, row = e / KP, col = e - row * KP;
+            const uint64_t gr = r + row;
+            Vec16<T> v;
+#pragma unroll
+            for (int u = 0; u < V; u++) v.v[u] = T(0);
+            if (gr < r1) {
+                const T *src = M + gr * ld + col;
+                if (vec && col + V <= k) {
+                    v = *(const Vec16<T> *)src;
+                } else {
+#pragma unroll
+                    for (int u = 0; u < V; u++)
+                        if (col + u < k) v.v[u] = src[u];
+                }
+            }
+            reg[p] = v;
+        }
+    }
+    // into LDS rows of stride LS elements
+    template <int LS>
+    __device__ __forceinline__ void store(T *lds, int lane) const
+    {
+#pragma unroll
+        for (int p = 0; p < NP; p++) {
+            const int e = (p * 64 + lane) * V, row = e / KP, col = e - row * KP;
+#pragma unroll
+            for (int u = 0; u < V; u++) lds[row * LS + col + u] = reg[p].v[u];
+        }
+    }
+};
+
+// G = A^T B, ka <= 16*TI, kb <= 16*TJ: per 16 staged rows, four K = 4 steps.
+template <typename T, int TI, int TJ>
+__global__ void __launch_bounds__(kT) gram_stage_kernel(const T *A, int64_t lda, const T *B, int64_t ldb,
+                                                        uint64_t n, int ka, int kb, uint64_t rows_per_wave,
+                                                        uint64_t nwaves, int veca, int vecb, double *partial)
+{
+    constexpr int KAP = TI * 16, KBP = TJ * 16;
+    constexpr int LA = KAP + 16, LB = KBP + 16;  // padded LDS row strides (elements)
+    extern __shared__ __align__(16) unsigned char smg[];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    T *As = (T *)smg + w * kStageRows * (LA + LB);
+    T *Bs = As + kStageRows * LA;
+    const uint64_t wave = (blockIdx.x * (uint64_t)kT + threadIdx.x) >> 6;  // past nwaves: no rows
+    (void)nwaves;
+    const int li = lane & 15, lk = lane >> 4;
+    doublex4 acc[TI * TJ];
+#pragma unroll
+    for (int t = 0; t < TI * TJ; t++) acc[t] = doublex4{0.0, 0.0, 0.0, 0.0};
+    const uint64_t r0 = wave * rows_per_wave;
+    const uint64_t r1 = min(n, r0 + rows_per_wave);
+    Stage<T, KAP> sa;
+    Stage<T, KBP> sb;
+    if (r0 < r1) {
+        sa.load(A, lda, ka, r0, r1, lane, veca);
+        sb.load(B, ldb, kb, r0, r1, lane, vecb);
+    }
+    for (uint64_t r = r0; r < r1; r += kStageRows) {
+        sa.template store<LA>(As, lane);
+        sb.template store<LB>(Bs, lane);
+        wave_lds_sync();
+        if (r + kStageRows < r1) {
+            sa.load(A, lda, ka, r + kStageRows, r1, lane, veca);
+            sb.load(B, ldb, kb, r + kStageRows, r1, lane, vecb);
+        }
+#pragma unroll
+        for (int ks = 0; ks < kStageRows / 4; ks++) {
+            const int row = ks * 4 + lk;
+            double a[TI], b[TJ];
+#pragma unroll
+            for (int t = 0; t < TI; t++) a[t] = (double)As[row * LA + t * 16 + li];
+#pragma unroll
+            for (int t = 0; t < TJ; t++) b[t] = (double)Bs[row * LB + t * 16 + li];
+#pragma unroll
+            for (int ti = 0; ti < TI; ti++)
+#pragma unroll
+                for (int tj = 0; tj < TJ; tj++)
+                    acc[ti * TJ + tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ti], b[tj], acc[ti * TJ + tj], 0, 0, 0);
+        }
+        wave_lds_sync();
+    }
+    // the block's four waves combine in LDS as (w0 + w2) + (w1 + w3); the
+    // block's sum goes to partial[block]
+    const int nout = ka * kb;
+    double *red = (double *)smg;  // 2 x nout
+    auto exchange = [&](int writer_lo, int writers, bool add) {
+        __syncthreads();
+        if (w >= writer_lo && w < writer_lo + writers) {
+            double *slot = red + (w - writer_lo) * nout;
+#pragma unroll
+            for (int ti = 0; ti < TI; ti++)
+#pragma unroll
+                for (int tj = 0; tj < TJ; tj++)
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        // FP64 16x16x4 output: value q of lane l is row l/16 + 4q, column l%16
+                        const int i = ti * 16 + lk + 4 * q, j = tj * 16 + li;
+                        if (i < ka && j < kb) slot[i * kb + j] = acc[ti * TJ + tj][q];
+                    }
+        }
+        __syncthreads();
+        if (add && w < writers) {
+            const double *slot = red + w * nout;
+#pragma unroll
+            for (int ti = 0; ti < TI; ti++)
+#pragma unroll
+                for (int tj = 0; tj < TJ; tj++)
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        const int i = ti * 16 + lk + 4 * q, j = tj * 16 + li;
+                        if (i < ka && j < kb) acc[ti * TJ + tj][q] += slot[i * kb + j];
+                    }
+        }
+    };
+    exchange(2, 2, true);   // w0 += w2, w1 += w3
+    exchange(1, 1, true);   // w0 += w1
+    if (w == 0) {
+#pragma unroll
+        for (int ti = 0; ti < TI; ti++)
+#pragma unroll
+            for (int tj = 0; tj < TJ; tj++)
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const int i = ti * 16 + lk + 4 * q, j = tj * 16 + li;
+                    if (i < ka && j < kb) partial[(uint64_t)blockIdx.x * nout + i * kb + j] = acc[ti * TJ + tj][q];
+                }
+    }
+}
+
+// Y = alpha A C + beta Y for kc <= 16*TJ, ka <= KAP: per 16 staged rows of A,
+// ka/4 MFMA steps against C held in registers (lane l: C[4s + l/16][l%16]).
+// A row is staged whole before any output of it is written, so Y may be A.
+template <typename T, typename TY, int KAP, int TJ>
+__global__ void __launch_bounds__(kT) gemm_stage_kernel(const T *A, int64_t lda, uint64_t n, int ka, int kc,
+                                                        const double *C, double alpha, double beta, TY *Y,
+                                                        int64_t ldy, uint64_t rows_per_wave, uint64_t nwaves,
+                                                        int veca)
+{
+    constexpr int LA = KAP + 1;       // odd stride: the 16 rows of a k column hit distinct banks
+    constexpr int KS = KAP / 4;       // MFMA K steps
+    extern __shared__ __align__(16) unsigned char smg[];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    T *As = (T *)smg + w * kStageRows * LA;
+    const uint64_t wave = (blockIdx.x * (uint64_t)kT + threadIdx.x) >> 6;
+    if (wave >= nwaves) return;
+    const int li = lane & 15, lk = lane >> 4;
+    double c[TJ][KS];
+#pragma unroll
+    for (int tj = 0; tj < TJ; tj++)
+#pragma unroll
+        for (int s = 0; s < KS; s++) {
+            const int k = 4 * s + lk, j = tj * 16 + li;
+            c[tj][s] = (k < ka && j < kc) ? C[k * kc + j] : 0.0;
+        }
+    const uint64_t r0 = wave * rows_per_wave;
+    const uint64_t r1 = min(n, r0 + rows_per_wave);
+    Stage<T, KAP> sa;
+    if (r0 < r1) sa.load(A, lda, ka, r0, r1, lane, veca);
+    for (uint64_t r = r0; r < r1; r += kStageRows) {
+        sa.template store<LA>(As, lane);
+        wave_lds_sync();
+        if (r + kStageRows < r1) sa.load(A, lda, ka, r + kStageRows, r1, lane, veca);
+        doublex4 acc[TJ];
+#pragma unroll
+        for (int tj = 0; tj < TJ; tj++) acc[tj] = doublex4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int s = 0; s < KS; s++) {
+            const double a = (double)As[li * LA + 4 * s + lk];
+#pragma unroll
+            for (int tj = 0; tj < TJ; tj++) acc[tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, c[tj][s], acc[tj], 0, 0, 0);
+        }
+        wave_lds_sync();
+#pragma unroll
+        for (int tj = 0; tj < TJ; tj++) {
+            const int j = tj * 16 + li;
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const uint64_t gr = r + lk + 4 * q;
+                if (gr < r1 && j < kc) {
+                    TY *y = Y + gr * ldy + j;
+                    const double prev = (beta == 0.0) ? 0.0 : beta * (double)*y;
+                    *y = (TY)(prev + alpha * acc[tj][q]);
+                }
+            }
+        }
+    }
+}
+
+// out[o] = sum over p of partial[p][o], p in order 0..np-1 within four
+// interleaved quarters that are then added in order (deterministic).
+__global__ void __launch_bounds__(kT) reduce_partials4(const double *partial, int np, int nout, double *out)
+{
+    __shared__ double red[kT];
+    const int ol = threadIdx.x & 63, qt = threadIdx.x >> 6;
+    const int o = blockIdx.x * 64 + ol;
+    double s = 0.0;
+    if (o < nout)
+        for (int p = qt; p < np; p += 4) s += partial[(uint64_t)p * nout + o];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    if (qt == 0 && o < nout) out[o] = ((red[ol] + red[64 + ol]) + red[128 + ol]) + red[192 + ol];
 }
 
 // ---- BF16 MFMA Gram ------------------------------------------------------------
@@ -196,6 +437,62 @@ __global__ void __launch_bounds__(kT) gemm_tall_kernel(const T *A, int64_t lda, 
     }
 }
 
+// One thread per row: the row of A is read once (16-byte loads when rows are
+// aligned), C sits in LDS zero-padded to whole JT-column tiles (every lane
+// reads the same C words: broadcasts), and each tile's JT sums stay in
+// registers in FP64.  With Y == A a row is overwritten only after its last
+// tile, so the in-place form needs kc <= JT.
+template <typename T, typename TY, int JT>
+__global__ void __launch_bounds__(kT) gemm_rows_kernel(const T *A, int64_t lda, uint64_t n, int ka, int kc,
+                                                       int kcp, const double *C, double alpha, double beta,
+                                                       TY *Y, int64_t ldy, int vec)
+{
+    extern __shared__ double sg[];
+    double *Cs = sg;  // ka x kcp
+    for (int e = threadIdx.x; e < ka * kcp; e += kT) {
+        const int i = e / kcp, j = e - i * kcp;
+        Cs[e] = j < kc ? C[i * kc + j] : 0.0;
+    }
+    __syncthreads();
+    constexpr int V = 16 / sizeof(T);  // elements per 16-byte load
+    for (uint64_t r = blockIdx.x * (uint64_t)kT + threadIdx.x; r < n; r += (uint64_t)gridDim.x * kT) {
+        const T *a = A + r * lda;
+        for (int j0 = 0; j0 < kc; j0 += JT) {
+            double acc[JT];
+#pragma unroll
+            for (int j = 0; j < JT; j++) acc[j] = 0.0;
+            int i = 0;
+            if (vec) {
+                for (; i + V <= ka; i += V) {
+                    T av[V];
+                    *(Vec16<T> *)av = *(const Vec16<T> *)(a + i);
+#pragma unroll
+                    for (int u = 0; u < V; u++) {
+                        const double x = (double)av[u];
+                        const double *c = Cs + (i + u) * kcp + j0;
+#pragma unroll
+                        for (int j = 0; j < JT; j++) acc[j] = fma(x, c[j], acc[j]);
+                    }
+                }
+            }
+            for (; i < ka; i++) {
+                const double x = (double)a[i];
+                const double *c = Cs + i * kcp + j0;
+#pragma unroll
+                for (int j = 0; j < JT; j++) acc[j] = fma(x, c[j], acc[j]);
+            }
+            TY *y = Y + r * ldy + j0;
+            const int w = kc - j0 < JT ? kc - j0 : JT;
+#pragma unroll
+            for (int j = 0; j < JT; j++)
+                if (j < w) {
+                    const double prev = (beta == 0.0) ? 0.0 : beta * (double)y[j];
+                    y[j] = (TY)(prev + alpha * acc[j]);
+                }
+        }
+    }
+}
+
 // ---- column norms / sums of squares ----------------------------------------------
 template <typename T>
 __global__ void __launch_bounds__(kT) colsq_kernel(const T *A, int64_t lda, uint64_t n, int k,
@@ -288,6 +585,82 @@ fmx_status gram(fmx_ctx *ctx, fmx_dtype dt, int mode, uint64_t n, int ka, int kb
         return FMX_OK;
     }
     if (mode != FMX_GRAM_F64) return fail(ctx, FMX_EINVAL, "unknown Gram mode");
+    const int tjn = (kb + 15) / 16, tin = (ka + 15) / 16;
+    const int es = dt == FMX_F64 ? 8 : 4;
+    const int tip = tin <= 1 ? 1 : tin <= 2 ? 2 : tin <= 4 ? 4 : tin <= 8 ? 8 : 16;
+    const int tjp = tjn <= 1 ? 1 : tjn <= 2 ? 2 : 4;
+    const size_t stage_lds = (size_t)4 * kStageRows * (tip * 16 + 16 + tjp * 16 + 16) * es;
+    if (n == 0) {  // an empty sum: every entry 0
+        hipLaunchKernelGGL(reduce_partials4, dim3((nout + 63) / 64), dim3(kT), 0, s, (const double *)nullptr, 0,
+                           nout, G);
+        FMX_HIP(ctx, hipGetLastError());
+        return FMX_OK;
+    }
+    if (tjn <= 4 && tip * tjp <= kGramTiles && std::max(stage_lds, (size_t)2 * nout * sizeof(double)) <= 96 * 1024 &&
+        (dt == FMX_F32 || dt == FMX_F64)) {
+        // waves of at least 1024 rows (a multiple of 16), at most 8 per CU
+        uint64_t waves = (n + 1023) / 1024;
+        const uint64_t cap = (uint64_t)ctx->num_cus * 8;
+        if (waves > cap) waves = cap;
+        if (waves < 1) waves = 1;
+        const uint64_t rpw = ((n + waves - 1) / waves + 15) & ~15ull;
+        waves = (n + rpw - 1) / rpw;
+        if (waves < 1) waves = 1;
+        const uint32_t nblk = (uint32_t)((waves * 64 + kT - 1) / kT);  // waves past the rows add zeros
+        void *ws;
+        fmx_status st = ctx_scratch(ctx, (size_t)nblk * nout * sizeof(double), &ws);
+        if (st) return st;
+        const size_t glds = std::max(stage_lds, (size_t)2 * nout * sizeof(double));
+        const int veca = ((uintptr_t)A % 16 == 0) && ((lda * es) % 16 == 0);
+        const int vecb = ((uintptr_t)B % 16 == 0) && ((ldb * es) % 16 == 0);
+        static bool attr_g = false;
+        if (!attr_g) {
+            const void *ks[] = {
+                (const void *)gram_stage_kernel<float, 1, 1>, (const void *)gram_stage_kernel<float, 2, 1>,
+                (const void *)gram_stage_kernel<float, 4, 1>, (const void *)gram_stage_kernel<float, 8, 1>,
+                (const void *)gram_stage_kernel<float, 16, 1>, (const void *)gram_stage_kernel<float, 1, 2>,
+                (const void *)gram_stage_kernel<float, 2, 2>, (const void *)gram_stage_kernel<float, 4, 2>,
+                (const void *)gram_stage_kernel<float, 8, 2>, (const void *)gram_stage_kernel<float, 1, 4>,
+                (const void *)gram_stage_kernel<float, 2, 4>, (const void *)gram_stage_kernel<float, 4, 4>,
+                (const void *)gram_stage_kernel<double, 1, 1>, (const void *)gram_stage_kernel<double, 2, 1>,
+                (const void *)gram_stage_kernel<double, 4, 1>, (const void *)gram_stage_kernel<double, 8, 1>,
+                (const void *)gram_stage_kernel<double, 16, 1>, (const void *)gram_stage_kernel<double, 1, 2>,
+                (const void *)gram_stage_kernel<double, 2, 2>, (const void *)gram_stage_kernel<double, 4, 2>,
+                (const void *)gram_stage_kernel<double, 8, 2>, (const void *)gram_stage_kernel<double, 1, 4>,
+                (const void *)gram_stage_kernel<double, 2, 4>, (const void *)gram_stage_kernel<double, 4, 4>};
+            for (const void *k : ks)
+                FMX_HIP(ctx, hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024));
+            attr_g = true;
+        }
+#define FMX_GRAMS(TT, I, J)                                                                                   \
+    hipLaunchKernelGGL((gram_stage_kernel<TT, I, J>), dim3(nblk), dim3(kT), glds, s, (const TT *)A, lda,        \
+                       (const TT *)B, ldb, n, ka, kb, rpw, waves, veca, vecb, (double *)ws)
+#define FMX_GRAMS_I(TT, J)                                                                                    \
+    do {                                                                                                      \
+        if (tip == 1) FMX_GRAMS(TT, 1, J);                                                                    \
+        else if (tip == 2) FMX_GRAMS(TT, 2, J);                                                               \
+        else if (tip == 4) FMX_GRAMS(TT, 4, J);                                                               \
+        else if (J == 4) FMX_GRAMS(TT, 4, 4);                                                                 \
+        else if (tip == 8 || J == 2) FMX_GRAMS(TT, 8, (J == 2 ? 2 : 1));                                      \
+        else FMX_GRAMS(TT, 16, 1);                                                                            \
+    } while (0)
+        if (dt == FMX_F32) {
+            if (tjp == 1) FMX_GRAMS_I(float, 1);
+            else if (tjp == 2) FMX_GRAMS_I(float, 2);
+            else FMX_GRAMS_I(float, 4);
+        } else {
+            if (tjp == 1) FMX_GRAMS_I(double, 1);
+            else if (tjp == 2) FMX_GRAMS_I(double, 2);
+            else FMX_GRAMS_I(double, 4);
+        }
+#undef FMX_GRAMS_I
+#undef FMX_GRAMS
+        FMX_HIP(ctx, hipGetLastError());
+        hipLaunchKernelGGL(reduce_partials4, dim3((nout + 63) / 64), dim3(kT), 0, s, (const double *)ws,
+                           (int)nblk, nout, G);
+        FMX_HIP(ctx, hipGetLastError());
+        return FMX_OK;
+    }
     const uint32_t nblk = blocks_for(ctx, n, 512);
     const uint64_t rpb = (n + nblk - 1) / nblk;
     void *ws;
@@ -310,12 +683,62 @@ fmx_status gram(fmx_ctx *ctx, fmx_dtype dt, int mode, uint64_t n, int ka, int kb
     return FMX_OK;
 }
 
+// Staged MFMA tall GEMM (kc <= 32, ka <= 128): 16-row slices of A in LDS,
+// C in registers; A may be Y.
+template <typename T, typename TY>
+fmx_status gemm_staged(fmx_ctx *ctx, uint64_t n, int ka, int kc, double alpha, const T *A, int64_t lda,
+                       const double *C, double beta, TY *Y, int64_t ldy, hipStream_t s)
+{
+    if (n == 0) return FMX_OK;
+    const int kap = ka <= 16 ? 16 : ka <= 32 ? 32 : ka <= 64 ? 64 : ka <= 96 ? 96 : 128;
+    const size_t lds = (size_t)4 * kStageRows * (kap + 1) * sizeof(T);
+    uint64_t waves = (n + 255) / 256;
+    const uint64_t cap = (uint64_t)ctx->num_cus * 8;
+    if (waves > cap) waves = cap;
+    if (waves < 1) waves = 1;
+    const uint64_t rpw = ((n + waves - 1) / waves + 15) & ~15ull;
+    waves = (n + rpw - 1) / rpw;
+    if (waves < 1) waves = 1;
+    const uint32_t nblk = (uint32_t)((waves * 64 + kT - 1) / kT);
+    const int veca = ((uintptr_t)A % 16 == 0) && ((lda * (int64_t)sizeof(T)) % 16 == 0);
+    static bool attr = false;
+    if (!attr) {
+        const void *ks[] = {(const void *)gemm_stage_kernel<T, TY, 16, 1>, (const void *)gemm_stage_kernel<T, TY, 32, 1>,
+                            (const void *)gemm_stage_kernel<T, TY, 64, 1>, (const void *)gemm_stage_kernel<T, TY, 96, 1>,
+                            (const void *)gemm_stage_kernel<T, TY, 128, 1>, (const void *)gemm_stage_kernel<T, TY, 16, 2>,
+                            (const void *)gemm_stage_kernel<T, TY, 32, 2>, (const void *)gemm_stage_kernel<T, TY, 64, 2>,
+                            (const void *)gemm_stage_kernel<T, TY, 96, 2>, (const void *)gemm_stage_kernel<T, TY, 128, 2>};
+        for (const void *k : ks)
+            FMX_HIP(ctx, hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024));
+        attr = true;
+    }
+#define FMX_GEMMS(P, J)                                                                                       \
+    hipLaunchKernelGGL((gemm_stage_kernel<T, TY, P, J>), dim3(nblk), dim3(kT), lds, s, A, lda, n, ka, kc, C,     \
+                       alpha, beta, Y, ldy, rpw, waves, veca)
+#define FMX_GEMMS_P(J)                                                                                        \
+    do {                                                                                                      \
+        if (kap == 16) FMX_GEMMS(16, J);                                                                      \
+        else if (kap == 32) FMX_GEMMS(32, J);                                                                 \
+        else if (kap == 64) FMX_GEMMS(64, J);                                                                 \
+        else if (kap == 96) FMX_GEMMS(96, J);                                                                 \
+        else FMX_GEMMS(128, J);                                                                               \
+    } while (0)
+    if (kc <= 16) FMX_GEMMS_P(1);
+    else FMX_GEMMS_P(2);
+#undef FMX_GEMMS_P
+#undef FMX_GEMMS
+    FMX_HIP(ctx, hipGetLastError());
+    return FMX_OK;
+}
+
 fmx_status gemm_tall_f32_f64(fmx_ctx *ctx, uint64_t n, int ka, int kc, double alpha, const float *A,
                              int64_t lda, const double *C, double beta, double *Y, int64_t ldy,
                              hipStream_t s)
 {
     if (ka <= 0 || kc <= 0 || (size_t)ka * kc > 8192 || ka > 512)
         return fail(ctx, FMX_EDIM, "gemm_tall needs ka*kc <= 8192");
+    if (kc <= 32 && ka <= 128)
+        return gemm_staged<float, double>(ctx, n, ka, kc, alpha, A, lda, C, beta, Y, ldy, s);
     const size_t lds = ((size_t)ka * kc + (size_t)kMRows * ka) * sizeof(double);
     const uint32_t nblk = blocks_for(ctx, n, kMRows);
     static bool attr = false;
@@ -337,6 +760,51 @@ fmx_status gemm_tall(fmx_ctx *ctx, fmx_dtype dt, uint64_t n, int ka, int kc, dou
     if (ka <= 0 || kc <= 0 || (size_t)ka * kc > 8192 || ka > 512)
         return fail(ctx, FMX_EDIM, "gemm_tall needs ka*kc <= 8192");
     if (lda < ka || ldy < kc) return fail(ctx, FMX_EDIM, "leading dimension too small");
+    if (dt != FMX_F32 && dt != FMX_F64) return fail(ctx, FMX_ETYPE, "gemm_tall takes FP32 or FP64 operands");
+    if (kc <= 32 && ka <= 128) {
+        if (dt == FMX_F32)
+            return gemm_staged<float, float>(ctx, n, ka, kc, alpha, (const float *)A, lda, C, beta, (float *)Y, ldy, s);
+        return gemm_staged<double, double>(ctx, n, ka, kc, alpha, (const double *)A, lda, C, beta, (double *)Y,
+                                           ldy, s);
+    }
+    const bool in_place = (const void *)A == (const void *)Y;
+    const int jt = kc <= 8 ? 8 : (kc <= 16 ? 16 : 32);
+    if (!in_place || kc <= jt) {
+        const int kcp = (kc + jt - 1) / jt * jt;
+        const size_t lds_r = (size_t)ka * kcp * sizeof(double);
+        if (lds_r <= 64 * 1024) {
+            const int es = dt == FMX_F32 ? 4 : 8;
+            const int vec = ((uintptr_t)A % 16 == 0) && ((lda * es) % 16 == 0);
+            const uint32_t nblk = blocks_for(ctx, n, kT);
+            static bool attr_r = false;
+            if (!attr_r) {
+                const void *ks[] = {(const void *)gemm_rows_kernel<float, float, 8>,
+                                    (const void *)gemm_rows_kernel<float, float, 16>,
+                                    (const void *)gemm_rows_kernel<float, float, 32>,
+                                    (const void *)gemm_rows_kernel<double, double, 8>,
+                                    (const void *)gemm_rows_kernel<double, double, 16>,
+                                    (const void *)gemm_rows_kernel<double, double, 32>};
+                for (const void *k : ks)
+                    FMX_HIP(ctx, hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 64 * 1024));
+                attr_r = true;
+            }
+#define FMX_GEMM_ROWS(TT, J)                                                                                  \
+    hipLaunchKernelGGL((gemm_rows_kernel<TT, TT, J>), dim3(nblk), dim3(kT), lds_r, s, (const TT *)A, lda, n, ka, kc, \
+                       kcp, C, alpha, beta, (TT *)Y, ldy, vec)
+            if (dt == FMX_F32) {
+                if (jt == 8) FMX_GEMM_ROWS(float, 8);
+                else if (jt == 16) FMX_GEMM_ROWS(float, 16);
+                else FMX_GEMM_ROWS(float, 32);
+            } else {
+                if (jt == 8) FMX_GEMM_ROWS(double, 8);
+                else if (jt == 16) FMX_GEMM_ROWS(double, 16);
+                else FMX_GEMM_ROWS(double, 32);
+            }
+#undef FMX_GEMM_ROWS
+            FMX_HIP(ctx, hipGetLastError());
+            return FMX_OK;
+        }
+    }
     const size_t lds = ((size_t)ka * kc + (size_t)kMRows * ka) * sizeof(double);
     const uint32_t nblk = blocks_for(ctx, n, kMRows);
     static bool attr = false;

@@ -334,7 +334,7 @@ extern "C" fmx_status fmx_eigen_opts_init(fmx_eigen_opts *o, int nev)
     o->num_blocks = 6;
     o->max_restarts = 100;
     o->tol = 1e-6;
-    o->precision = FMX_EIG_MIXED;
+    o->precision = FMX_EIG_FP32;
     o->seed = 1;
     o->verbose = 0;
     return FMX_OK;

@@ -12,21 +12,29 @@ namespace {
 
 constexpr int kJT = 512;   // threads of the Jacobi workgroup
 
-// Cyclic Jacobi for a symmetric m x m matrix (m even, m <= 128): A is kept
-// in LDS, the eigenvector matrix V (row-major m x m) in global memory.  Each
-// round rotates m/2 disjoint pairs (round-robin ordering), rows first then
-// columns, so A' = P^T A P exactly as the sequential method would.
+// Cyclic Jacobi for a symmetric m x m matrix (m even, m <= 128), A in LDS.
+// Each round rotates m/2 disjoint pairs (round-robin ordering), rows first
+// then columns, so A' = P^T A P exactly as the sequential method would.
+// LDS rows are m + 1 doubles apart, so the column updates (one row per
+// thread) spread over the banks.  With VL the eigenvector matrix V is kept in
+// LDS after A (same padded stride) and copied out at the end; otherwise V is
+// updated in place in global memory (row-major, stride m).
+template <bool VL>
 __global__ void __launch_bounds__(kJT) jacobi_kernel(const double *Ain, int m, double *evals,
-                                                     double *V, int max_sweeps, double *info)
+                                                     double *Vout, int max_sweeps, double *info)
 {
-    extern __shared__ double A[];        // m * m
+    extern __shared__ double A[];        // m x (m + 1), then V with VL
     __shared__ double cs[64], sn[64];
     __shared__ int pp[64], qq[64];
     __shared__ double red[kJT];
     const int tid = threadIdx.x;
+    const int S = m + 1;                 // LDS row stride
+    double *V = VL ? A + m * S : Vout;
+    const int SV = VL ? S : m;           // V row stride
     for (int e = tid; e < m * m; e += kJT) {
-        A[e] = 0.5 * (Ain[e] + Ain[(e % m) * m + e / m]);  // symmetrise
-        V[e] = (e / m == e % m) ? 1.0 : 0.0;
+        const int r = e / m, c = e - r * m;
+        A[r * S + c] = 0.5 * (Ain[e] + Ain[c * m + r]);  // symmetrise
+        V[r * SV + c] = (r == c) ? 1.0 : 0.0;
     }
     __syncthreads();
     const int half = m / 2;
@@ -36,9 +44,10 @@ __global__ void __launch_bounds__(kJT) jacobi_kernel(const double *Ain, int m, d
         // convergence test: off-diagonal Frobenius norm vs whole norm
         double so = 0.0, st = 0.0;
         for (int e = tid; e < m * m; e += kJT) {
-            double v = A[e] * A[e];
+            const int r = e / m, c = e - r * m;
+            const double v = A[r * S + c] * A[r * S + c];
             st += v;
-            if (e / m != e % m) so += v;
+            if (r != c) so += v;
         }
         red[tid] = so;
         __syncthreads();
@@ -65,10 +74,10 @@ __global__ void __launch_bounds__(kJT) jacobi_kernel(const double *Ain, int m, d
                 if (a > b) { int t = a; a = b; b = t; }
                 pp[k] = a;
                 qq[k] = b;
-                const double apq = A[a * m + b];
+                const double apq = A[a * S + b];
                 double c = 1.0, s = 0.0;
                 if (apq != 0.0) {
-                    const double theta = (A[b * m + b] - A[a * m + a]) / (2.0 * apq);
+                    const double theta = (A[b * S + b] - A[a * S + a]) / (2.0 * apq);
                     const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
                     c = 1.0 / sqrt(t * t + 1.0);
                     s = t * c;
@@ -82,9 +91,9 @@ __global__ void __launch_bounds__(kJT) jacobi_kernel(const double *Ain, int m, d
                 const int k = e / m, col = e - k * m;
                 const int p = pp[k], q = qq[k];
                 const double c = cs[k], s = sn[k];
-                const double ap = A[p * m + col], aq = A[q * m + col];
-                A[p * m + col] = c * ap - s * aq;
-                A[q * m + col] = s * ap + c * aq;
+                const double ap = A[p * S + col], aq = A[q * S + col];
+                A[p * S + col] = c * ap - s * aq;
+                A[q * S + col] = s * ap + c * aq;
             }
             __syncthreads();
             // columns p, q of A and V
@@ -92,17 +101,19 @@ __global__ void __launch_bounds__(kJT) jacobi_kernel(const double *Ain, int m, d
                 const int k = e / m, row = e - k * m;
                 const int p = pp[k], q = qq[k];
                 const double c = cs[k], s = sn[k];
-                const double ap = A[row * m + p], aq = A[row * m + q];
-                A[row * m + p] = c * ap - s * aq;
-                A[row * m + q] = s * ap + c * aq;
-                const double vp = V[row * m + p], vq = V[row * m + q];
-                V[row * m + p] = c * vp - s * vq;
-                V[row * m + q] = s * vp + c * vq;
+                const double ap = A[row * S + p], aq = A[row * S + q];
+                A[row * S + p] = c * ap - s * aq;
+                A[row * S + q] = s * ap + c * aq;
+                const double vp = V[row * SV + p], vq = V[row * SV + q];
+                V[row * SV + p] = c * vp - s * vq;
+                V[row * SV + q] = s * vp + c * vq;
             }
             __syncthreads();
         }
     }
-    for (int i = tid; i < m; i += kJT) evals[i] = A[i * m + i];
+    for (int i = tid; i < m; i += kJT) evals[i] = A[i * S + i];
+    if (VL)
+        for (int e = tid; e < m * m; e += kJT) Vout[e] = V[(e / m) * S + e % m];
     if (tid == 0 && info) {
         info[0] = sweep;
         info[1] = sqrt(off / (tot > 0 ? tot : 1.0));
@@ -276,12 +287,17 @@ fmx_status small_sym_eig(fmx_ctx *ctx, const double *H, int m, double *evals, do
     if (m <= 0 || m > 128 || (m & 1)) return fail(ctx, FMX_EDIM, "Rayleigh-Ritz size must be even and <= 128");
     static bool attr = false;
     if (!attr) {
-        FMX_HIP(ctx, hipFuncSetAttribute((const void *)jacobi_kernel,
+        FMX_HIP(ctx, hipFuncSetAttribute((const void *)jacobi_kernel<true>,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
+        FMX_HIP(ctx, hipFuncSetAttribute((const void *)jacobi_kernel<false>,
                                          hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
         attr = true;
     }
-    hipLaunchKernelGGL(jacobi_kernel, dim3(1), dim3(kJT), (size_t)m * m * sizeof(double), s, H, m, evals,
-                       V, 40, info);
+    const size_t mm = (size_t)m * (m + 1) * sizeof(double);  // padded rows
+    if (2 * mm <= 150 * 1024)
+        hipLaunchKernelGGL(jacobi_kernel<true>, dim3(1), dim3(kJT), 2 * mm, s, H, m, evals, V, 40, info);
+    else
+        hipLaunchKernelGGL(jacobi_kernel<false>, dim3(1), dim3(kJT), mm, s, H, m, evals, V, 40, info);
     FMX_HIP(ctx, hipGetLastError());
     return FMX_OK;
 }

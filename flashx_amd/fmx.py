@@ -466,7 +466,7 @@ EIG_MIXED, EIG_FP32, EIG_FP64 = 0, 1, 2
 
 
 def compute_eigen(spm, nev=16, block_size=16, num_blocks=6, tol=1e-6, max_restarts=100,
-                  precision=EIG_MIXED, X0=None, seed=1, verbose=False):
+                  precision=EIG_FP32, X0=None, seed=1, verbose=False):
     """Top-nev eigenpairs of D^-1/2 A D^-1/2 (fm::eigen::compute_eigen with
     NA_eigen_Operator).  Returns (evals, evecs n x nev FP64 device tensor,
     relative residuals, stats dict)."""

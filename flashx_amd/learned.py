@@ -79,7 +79,7 @@ class Graph:
 
     def solve(self, X0=None, tol=1e-6, seed=1, verbose=False):
         return F.compute_eigen(self.spm, nev=NEV, block_size=16, num_blocks=6, tol=tol, X0=X0, seed=seed,
-                               precision=F.EIG_MIXED, verbose=verbose, max_restarts=400)
+                               precision=F.EIG_FP32, verbose=verbose, max_restarts=400)
 
 
 class WarmStartNet:

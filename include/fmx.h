@@ -220,7 +220,8 @@ fmx_status fmx_row_scale(fmx_ctx *ctx, fmx_dtype dtype, uint64_t n, int k, const
 
 /* ---- block Krylov eigensolver ---------------------------------------------- */
 #define FMX_EIG_MIXED 0  /* FP32 SpMM, BF16-MFMA first Gram pass, FP64 reorth/RR/residual */
-#define FMX_EIG_FP32  1  /* FP32 SpMM, FP64 Gram passes, FP64 RR/residual                 */
+#define FMX_EIG_FP32  1  /* FP32 SpMM, FP64(-MFMA) Gram passes, FP64 RR/residual (default;
+                            the BF16 pass reads the same FP32 operands and is no faster) */
 #define FMX_EIG_FP64  2  /* everything FP64                                              */
 
 /* eigen_options (matrix/eigensolver/eigensolver.h:59-73): which is "LA"

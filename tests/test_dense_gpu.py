@@ -18,7 +18,8 @@ def bf16_round(x):
 
 
 @pytest.mark.parametrize("n,ka,kb", [(100003, 128, 16), (5000, 16, 16), (777, 20, 3), (65536, 64, 64),
-                                     (1, 5, 7)])
+                                     (1, 5, 7), (1000003, 96, 16), (70001, 33, 50), (4099, 250, 16),
+                                     (3001, 70, 20), (513, 250, 40)])
 def test_gram_f64(fmx, orc, n, ka, kb):
     rng = np.random.default_rng(n + ka)
     for dt in (np.float32, np.float64):
@@ -47,7 +48,21 @@ def test_gram_bf16_mfma(fmx, orc, n, ka, kb):
     assert np.all(np.abs(G - exact) <= 1e-2 * scale)
 
 
-@pytest.mark.parametrize("n,ka,kc", [(100003, 128, 16), (3000, 16, 48), (10, 3, 5)])
+def test_gram_f64_strided(fmx, orc):
+    """Operands that are column windows of wider row-major arrays (lda > ka)."""
+    import torch
+    rng = np.random.default_rng(4)
+    Vfull = rng.standard_normal((30011, 96)).astype(np.float32)
+    W = rng.standard_normal((30011, 16)).astype(np.float32)
+    V = dev(Vfull)[:, :72]
+    G = fmx.gram(V, dev(W)).cpu().numpy()
+    ref = orc.gram_f64(Vfull[:, :72].astype(np.float64), W.astype(np.float64))
+    scale = np.abs(Vfull[:, :72].astype(np.float64)).T @ np.abs(W.astype(np.float64))
+    assert np.all(np.abs(G - ref) <= 1e-13 * scale)
+
+
+@pytest.mark.parametrize("n,ka,kc", [(100003, 128, 16), (3000, 16, 48), (10, 3, 5), (1000003, 96, 16),
+                                     (20011, 7, 40), (5003, 96, 80)])
 def test_gemm_tall(fmx, orc, n, ka, kc):
     rng = np.random.default_rng(ka * kc)
     C = rng.standard_normal((ka, kc))
@@ -59,6 +74,25 @@ def test_gemm_tall(fmx, orc, n, ka, kc):
         tol = 1e-6 if dt == np.float32 else 1e-12
         scale = 0.5 * np.abs(A.astype(np.float64)) @ np.abs(C) + 2 * np.abs(Y0)
         assert np.all(np.abs(Y - ref) <= tol * scale)
+
+
+@pytest.mark.parametrize("m,ka,kc", [(96, 96, 16), (96, 80, 32), (96, 96, 48), (33, 33, 8), (16, 16, 16)])
+def test_gemm_tall_in_place(fmx, orc, m, ka, kc):
+    """Y = A C written over A's own first kc columns (the thick restart's
+    V = V S): every row is read in full before it is overwritten."""
+    import torch
+    rng = np.random.default_rng(m + ka + kc)
+    C = rng.standard_normal((ka, kc))
+    for dt in (np.float32, np.float64):
+        T0 = rng.standard_normal((40009, m)).astype(dt)
+        T = dev(T0)
+        fmx.gemm_tall(T[:, :ka], dev(C), Y=T[:, :kc])
+        got = T.cpu().numpy()
+        ref = T0[:, :ka].astype(np.float64) @ C
+        tol = 1e-6 if dt == np.float32 else 1e-12
+        scale = np.abs(T0[:, :ka].astype(np.float64)) @ np.abs(C)
+        assert np.all(np.abs(got[:, :kc] - ref) <= tol * scale + 1e-300)
+        assert np.array_equal(got[:, kc:], T0[:, kc:])
 
 
 def test_colnorm_and_row_scale(fmx, orc):
