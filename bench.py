@@ -71,6 +71,8 @@ def main():
                     help="block-row chunk bytes of the streamed leg")
     ap.add_argument("--cpu-reps", type=int, default=5)
     ap.add_argument("--no-eigen", action="store_true", help="skip the eigensolver legs")
+    ap.add_argument("--eigen-only", action="store_true",
+                    help="only the scale-22 eigensolve leg (one JSON line), e.g. under a profiler")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run three 1 GiB device copies (the known-size kernel that scales "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE into bytes)")
@@ -92,6 +94,9 @@ def main():
         return
     import flashx_amd.fmx as F
 
+    if args.eigen_only:
+        print(json.dumps({"eigen": eigen_leg(F, torch, 7, 15)}), flush=True)
+        return
     ctx = F.default_context(0)
     n = 1 << SCALE
     m = EDGE_FACTOR * n

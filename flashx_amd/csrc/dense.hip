@@ -285,6 +285,17 @@ __global__ void __launch_bounds__(kT) gemm_stage_kernel(const T *A, int64_t lda,
         sa.template store<LA>(As, lane);
         wave_lds_sync();
         if (r + kStageRows < r1) sa.load(A, lda, ka, r + kStageRows, r1, lane, veca);
+        // this stage's old Y values, requested before the MFMAs that hide them
+        double prev[TJ][4];
+#pragma unroll
+        for (int tj = 0; tj < TJ; tj++) {
+            const int j = tj * 16 + li;
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const uint64_t gr = r + lk + 4 * q;
+                prev[tj][q] = (beta != 0.0 && gr < r1 && j < kc) ? beta * (double)Y[gr * ldy + j] : 0.0;
+            }
+        }
         doublex4 acc[TJ];
 #pragma unroll
         for (int tj = 0; tj < TJ; tj++) acc[tj] = doublex4{0.0, 0.0, 0.0, 0.0};
@@ -301,11 +312,7 @@ __global__ void __launch_bounds__(kT) gemm_stage_kernel(const T *A, int64_t lda,
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 const uint64_t gr = r + lk + 4 * q;
-                if (gr < r1 && j < kc) {
-                    TY *y = Y + gr * ldy + j;
-                    const double prev = (beta == 0.0) ? 0.0 : beta * (double)*y;
-                    *y = (TY)(prev + alpha * acc[tj][q]);
-                }
+                if (gr < r1 && j < kc) Y[gr * ldy + j] = (TY)(prev[tj][q] + alpha * acc[tj][q]);
             }
         }
     }
