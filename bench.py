@@ -214,21 +214,42 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def stream_columns(torch, src, dst, n, lh, lw):
+def stream_columns(torch, src, dst, n, lh, lw, chunk_edges=1 << 28):
     """Columns of the non-zeros in the tiles' order -- block row, block
     column, then row and column inside the tile (the image's order up to the
     split of each tile into its multi-entry and single-entry rows) -- from
-    the edge list after self-edge removal and dedup, as the build does."""
-    s = src.long() & 0xFFFFFFFF
-    d = dst.long() & 0xFFFFFFFF
-    keep = s != d
-    u = torch.unique(s[keep] * n + d[keep])
-    s, d = u // n, u % n
-    del u, keep
+    the edge list after self-edge removal and dedup, as the build does.
+    Edge lists beyond chunk_edges are taken in ranges of whole block rows,
+    which the order puts one after the other."""
+    m = src.numel()
+    nbr = (n + (1 << lh) - 1) >> lh
+    nchunks = max(1, (m + chunk_edges - 1) // chunk_edges)
+    step = ((nbr + nchunks - 1) // nchunks) << lh  # rows per range, whole block rows
     bits = max(1, (n - 1).bit_length())
-    key = ((((s >> lh) << (bits - lw)) | (d >> lw)) << (lh + bits)) | ((s & ((1 << lh) - 1)) << bits) | d
-    order = torch.sort(key).indices
-    return d[order].to(torch.int32).contiguous()
+    out = []
+    piece = 1 << 29  # torch indexing of 2^31 elements or more overflows: work on slices
+    for lo in range(0, n, step):
+        ss, ds = [], []
+        for p0 in range(0, m, piece):
+            sp = src[p0:p0 + piece].long() & 0xFFFFFFFF
+            dp = dst[p0:p0 + piece].long() & 0xFFFFFFFF
+            keep = (sp >= lo) & (sp < lo + step)
+            ss.append(sp[keep])
+            ds.append(dp[keep])
+            del sp, dp, keep
+        s = torch.cat(ss)
+        d = torch.cat(ds)
+        del ss, ds
+        keep = s != d
+        u = torch.unique(s[keep] * n + d[keep])  # sorted by (row, column)
+        del s, d, keep
+        # a stable sort by (block row, block column) keeps (row, column) order inside each tile
+        tile = (((u // n) >> lh) << max(0, bits - lw)) | ((u % n) >> lw)
+        order = torch.sort(tile, stable=True).indices
+        del tile
+        out.append((u[order] % n).to(torch.int32))
+        del u, order
+    return torch.cat(out).contiguous() if len(out) > 1 else out[0].contiguous()
 
 
 def gather_floor_leg(F, torch, X, cols, spmm_ms, steps=10):
@@ -294,6 +315,8 @@ def large_leg(F, torch, scale=27, k=32, steps=5, lh=7, lw=15):
     log("large: generating %d edges" % (EDGE_FACTOR * n))
     src, dst = F.rmat_edges(scale, EDGE_FACTOR * n, scale)
     X = torch.rand((n, k), dtype=torch.float32, device="cuda", generator=g)
+    cols = stream_columns(torch, src, dst, n, lh, lw)
+    torch.cuda.empty_cache()
     torch.cuda.synchronize()
     log("large: building %dx%d tiles" % (1 << lh, 1 << lw))
     t0 = time.perf_counter()
@@ -316,6 +339,8 @@ def large_leg(F, torch, scale=27, k=32, steps=5, lh=7, lw=15):
     torch.cuda.synchronize()
     ms = a.elapsed_time(b) / steps
     alg = spm.alg_bytes(torch.float32, k)
+    floor = gather_floor_leg(F, torch, X, cols, ms, steps=3)
+    del cols
     log("large: %.3f ms per SpMM; checking a sample on the CPU" % ms)
     H = 1 << lh
     rng = np.random.default_rng(scale)
@@ -337,7 +362,7 @@ def large_leg(F, torch, scale=27, k=32, steps=5, lh=7, lw=15):
     return {"workload": "spmm rmat%d_ef%d_f32_k%d" % (scale, EDGE_FACTOR, k), "nnz": int(spm.info.nnz),
             "image_bytes": int(spm.info.image_bytes), "tile": "%dx%d" % (H, 1 << lw),
             "build_s": round(t_build, 2), "ms_per_step": round(ms, 4), "GBps": round(alg / ms / 1e6, 1),
-            "pct_hbm_peak": round(100 * alg / ms / 1e6 / PEAK_HBM_GBS, 2),
+            "pct_hbm_peak": round(100 * alg / ms / 1e6 / PEAK_HBM_GBS, 2), "gather_floor": floor,
             "sample": {"rows": nb * H, "first_row": br0 * H, "max_rel_err_vs_cpu_fp32": rel,
                        "max_rel_err_vs_fp64": rel64,
                        "cpu_sec_single_thread": round(t_cpu, 2)}}
