@@ -60,7 +60,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--log2-h", type=int, default=None,
-                    help="tile height (log2); default: autotuned over 64/128/256 rows")
+                    help="tile height (log2); default: autotuned over 32/64/128 rows")
     ap.add_argument("--log2-w", type=int, default=15)
     ap.add_argument("--large", action="store_true",
                     help="also run the R-MAT scale-27, k=32 SpMM with a sampled CPU check")
@@ -302,28 +302,28 @@ def streamed_leg(F, torch, spm, X, Y_ref, chunk_bytes, steps=10):
     return out
 
 
-def large_leg(F, torch, scale=27, k=32, steps=5, lh=7, lw=15):
+def large_leg(F, torch, scale=27, k=32, steps=5, candidates=((4, 15), (5, 15), (6, 15))):
     """R-MAT scale 27 (2^27 vertices, 2^31 edges), X 2^27 x 32 FP32, all in
-    HBM; 128 x 32768 tiles (the band of a k = 32 wave then leaves room for
-    four waves per CU); FP32 check of a seeded sample of >= 1M rows (whole
-    block rows) against the oracle's reference-order CPU SpMM."""
+    HBM; the tile height is autotuned over 16/32/64 rows (the fastest
+    candidate's matrix is kept); FP32 check of a seeded sample of >= 1M rows
+    (whole block rows) against the oracle's reference-order CPU SpMM."""
     import numpy as np
     from oracle import oracle as O
+    from flashx_amd.tune import autotune_tile
     n = 1 << scale
     g = torch.Generator(device="cuda")
     g.manual_seed(scale)
     log("large: generating %d edges" % (EDGE_FACTOR * n))
     src, dst = F.rmat_edges(scale, EDGE_FACTOR * n, scale)
     X = torch.rand((n, k), dtype=torch.float32, device="cuda", generator=g)
+    log("large: autotuning the tile shape over %s" % (candidates,))
+    (lh, lw), tt, spm, t_build = autotune_tile(src, dst, n, k, torch.float32, candidates=candidates, X=X,
+                                               keep=True)
+    tune = {"%dx%d" % (1 << a, 1 << b): round(v, 2) for (a, b), v in tt.items()}
+    log("large: autotune (ms per SpMM): %s" % tune)
+    log("large: %dx%d tiles built in %.1f s, nnz %d, image %.2f GB" % (1 << lh, 1 << lw, t_build, spm.info.nnz,
+                                                                      spm.info.image_bytes / 1e9))
     cols = stream_columns(torch, src, dst, n, lh, lw)
-    torch.cuda.empty_cache()
-    torch.cuda.synchronize()
-    log("large: building %dx%d tiles" % (1 << lh, 1 << lw))
-    t0 = time.perf_counter()
-    spm = F.SparseMatrix.build(src, dst, n=n, log2_h=lh, log2_w=lw, flags=F.BUILD_DEDUP)
-    torch.cuda.synchronize()
-    t_build = time.perf_counter() - t0
-    log("large: built in %.1f s, nnz %d, image %.2f GB" % (t_build, spm.info.nnz, spm.info.image_bytes / 1e9))
     del src, dst
     torch.cuda.empty_cache()
     Y = torch.empty((n, k), dtype=torch.float32, device="cuda")
@@ -360,7 +360,7 @@ def large_leg(F, torch, scale=27, k=32, steps=5, lh=7, lw=15):
     rel64 = float((np.abs(got - ref64)[nzm] / np.abs(ref64[nzm])).max())
     log("large: %.3f ms per SpMM, %.1f GB/s, sample rel err %.2e" % (ms, alg / ms / 1e6, rel))
     return {"workload": "spmm rmat%d_ef%d_f32_k%d" % (scale, EDGE_FACTOR, k), "nnz": int(spm.info.nnz),
-            "image_bytes": int(spm.info.image_bytes), "tile": "%dx%d" % (H, 1 << lw),
+            "image_bytes": int(spm.info.image_bytes), "tile": "%dx%d" % (H, 1 << lw), "tile_autotune_ms": tune,
             "build_s": round(t_build, 2), "ms_per_step": round(ms, 4), "GBps": round(alg / ms / 1e6, 1),
             "pct_hbm_peak": round(100 * alg / ms / 1e6 / PEAK_HBM_GBS, 2), "gather_floor": floor,
             "sample": {"rows": nb * H, "first_row": br0 * H, "max_rel_err_vs_cpu_fp32": rel,

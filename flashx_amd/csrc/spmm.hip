@@ -19,9 +19,9 @@
 //     L lanes per non-zero).  Lane group g takes P consecutive entries, so a
 //     run of one row (a row part of a tile) is summed in registers and reaches
 //     the band with one LDS add per element at each row change;
-//   * for compile-time widths a chunk is exactly one drain batch and the loop
-//     is software-pipelined: the gathers of chunk i are in flight while chunk
-//     i+1 is decoded and the words of chunk i+2 are requested;
+//   * the next chunk's words are requested before the drain; a wave's own
+//     decode and gathers alternate, and the other resident waves (up to four
+//     per SIMD, as LDS allows) keep gathers in flight meanwhile;
 //   * the band is written to Y once (rows without non-zeros are 0), or to a
 //     partial slot when a power-law band was split over several items; split
 //     bands are summed slot by slot, in stream order, by spmm_reduce.
@@ -485,10 +485,13 @@ __device__ __forceinline__ ItemState item_state(const SpmmArgs &a, const Item &i
 }
 
 // ---------------------------------------------------------------------------
-// Non-pipelined kernel: runtime widths (K = 0) and widths whose drain batch
-// is smaller than a chunk.  Chunks of 64*R words, drained batch by batch.
+// The SpMM kernel: chunks of 64*R words, each decoded and then drained
+// batch by batch.  The next chunk's words and window are requested before
+// the drain, so their latency hides behind the gathers; the gathers of one
+// wave are not overlapped with its own decode, other resident waves fill that
+// gap (four per SIMD where LDS allows).
 template <typename T, int K, int R, int P>
-__global__ void __launch_bounds__(64 * kWPB) __attribute__((amdgpu_waves_per_eu(2, 2)))
+__global__ void __launch_bounds__(64 * kWPB) __attribute__((amdgpu_waves_per_eu(4, 4)))
 spmm_lane(SpmmArgs a)
 {
     using LC = LaneCfg<R>;
@@ -549,83 +552,6 @@ spmm_lane(SpmmArgs a)
         wave_sync();
         c = cn;
     }
-    }
-    write_band<T, K, A>(a, band, S, st.row0, st.ic.rows, st.slot, lane);
-}
-
-// Software-pipelined kernel for compile-time widths whose drain batch is a
-// whole chunk (64*R = NZP*P words): the gathers of chunk i are in flight
-// while chunk i+1 is decoded, and the words and window of chunk i+2 are
-// requested before they are consumed.  Every iteration issues the same loads
-// in the same order (past the item's end they are clamped and decode to no
-// entries), so the waits before the decode cover only the earlier requests.
-template <typename T, int K, int P>
-__global__ void __launch_bounds__(64 * kWPB) __attribute__((amdgpu_waves_per_eu(2, 2)))
-spmm_pipe(SpmmArgs a)
-{
-    using C = Lanes<T, K>;
-    constexpr int R = C::NZP * P / 64;
-    using LC = LaneCfg<R>;
-    constexpr int CW = LC::CW, DWIN = LC::DWIN, DPL = LC::DPL;
-    constexpr int QS = CW + CW / P + 1;
-    extern __shared__ __align__(16) unsigned char smem[];
-    const int lane = threadIdx.x & 63;
-    const uint32_t wv = uni(threadIdx.x >> 6);
-    const uint32_t item_id = blockIdx.x * kWPB + wv;
-    if (item_id >= a.nitems) return;
-
-    unsigned char *wbase = smem + wv * a.wave_bytes;
-    using A = typename AccOf<T>::type;
-    A *band = (A *)wbase;
-    uint32_t *qcol = (uint32_t *)(wbase + a.band_bytes);
-    uint16_t *qrow = (uint16_t *)(qcol + QS);
-    uint2 *win = (uint2 *)(wbase + ((a.band_bytes + QS * 6 + 7) & ~7u));
-
-    ItemState st = item_state(a, a.items[item_id]);
-    const uint32_t S = a.band_stride;
-    for (uint32_t i = lane; i < st.ic.rows * S; i += 64) band[i] = A(0);
-    if (lane == 0) win[DWIN] = make_uint2(0xFFFFFFFFu, 0u);
-
-    uint64_t abs_end, lastw, c;
-    uint32_t wreg[R / 2];
-    uint2 dreg[DPL];
-    bool live[DPL];
-
-    // decodes chunk cc (its words and window are in registers) and requests
-    // chunk cc + CW
-    auto decode_step = [&](uint64_t cc) -> uint32_t {
-        store_window<DPL>(win, dreg, live, lane);
-        wave_sync();
-        const uint32_t n = decode_chunk<R, P>(st.ic, cc, wreg, win, st.carry, qcol, qrow, lane);
-        const uint64_t cn = cc + CW;
-        const uint32_t adv = window_advance<DPL>(dreg, live, (uint32_t)((int64_t)cn - (int64_t)st.ic.bw));
-        if (cn < abs_end) st.ct += adv - 1;
-        load_words<R>(a.img, cn, lastw, lane, wreg);
-        load_window<DPL>(a.dir, st.ct, st.dlast, lane, dreg, live);
-        wave_sync();
-        return n;
-    };
-    for (int seg = 0; seg < 2; seg++) {
-    if (!select_segment(st, seg)) break;
-    abs_end = st.ic.bw + (uint64_t)st.ic.pend;
-    lastw = abs_end - 1;
-    c = (st.ic.bw + (uint64_t)st.ic.begin) & ~7ull;
-    load_words<R>(a.img, c, lastw, lane, wreg);
-    load_window<DPL>(a.dir, st.ct, st.dlast, lane, dreg, live);
-    uint32_t qn = decode_step(c);
-    while (true) {
-        Vec<T, C::E> v[P];
-        uint32_t rr[P];
-        gather_batch<T, K, P, typename AccOf<T>::type>(a, qcol, qrow, 0, qn, lane, v, rr);
-        c += CW;
-        const bool more = c < abs_end;
-        const uint32_t qn_next = decode_step(c);  // no entries past the item's end
-        accumulate_batch<T, K, P, A>(band, S, lane, v, rr);
-        if (!more) break;
-        qn = qn_next;
-        wave_sync();
-    }
-    wave_sync();
     }
     write_band<T, K, A>(a, band, S, st.row0, st.ic.rows, st.slot, lane);
 }
@@ -795,7 +721,7 @@ __device__ __forceinline__ T load_value(const uint16_t *p)
 }
 
 template <typename T, int K, int R, int P>
-__global__ void __launch_bounds__(64 * kWPB) __attribute__((amdgpu_waves_per_eu(1, 2)))
+__global__ void __launch_bounds__(64 * kWPB) __attribute__((amdgpu_waves_per_eu(4, 4)))
 spmm_lane_w(SpmmArgs a)
 {
     using LC = LaneCfg<R>;
@@ -856,8 +782,12 @@ spmm_lane_w(SpmmArgs a)
                                                     qcol, qrow, qval, lane);
         const uint64_t cn = c + CW;
         if (cn < abs_end) {
-            const uint32_t adv = window_advance<DPL>(dreg, live, (uint32_t)((int64_t)cn - (int64_t)st.ic.bw));
-            c0 = (int)qn - wq[adv - 1];
+            const uint32_t pn = (uint32_t)((int64_t)cn - (int64_t)st.ic.bw);
+            const uint32_t adv = window_advance<DPL>(dreg, live, pn);
+            // the next chunk's window tile 0 is window tile adv-1; one that
+            // starts exactly at the next chunk's first word has no columns in
+            // this chunk, and its wq slot was not written by this decode
+            c0 = win[adv - 1].x == pn ? 0 : (int)qn - wq[adv - 1];
             st.ct += adv - 1;
             load_words<R>(a.img, cn, lastw, lane, wreg);
             load_window<DPL>(a.dir, st.ct, st.dlast, lane, dreg, live);
@@ -960,49 +890,30 @@ __global__ void __launch_bounds__(256) spmm_reduce(const Split *splits, const A 
     }
 }
 
-// Gathers in flight per lane group: ~128 VGPRs of loaded data, and no more
-// entries than a 512-word chunk can hold.
+// Gathers in flight per lane group.  Eight 16-byte gathers per lane keep
+// spmm_lane under 80 VGPRs, so the LDS of its band and queue, not registers,
+// bounds the waves per CU.  More resident waves with short batches put more
+// gathers in flight than fewer waves with long ones: on R-MAT scale 22, FP32
+// k = 16, 64x32768 tiles, 0.93 ms against 1.30 ms for 512-entry batches
+// overlapped with the next decode at two waves per SIMD (the best earlier
+// schedule), and 57 ms against 92 ms on scale 27, FP32 k = 32.
 template <typename T, int K>
 constexpr int passes_for()
 {
-    if constexpr (K == 0) {
-        return 8;
-    } else {
-        constexpr int NZP = Lanes<T, K>::NZP;
-        constexpr int E = Lanes<T, K>::E;
-        constexpr int vgprs_per_pass = E * (int)sizeof(T) / 4;
-        constexpr int p = 128 / vgprs_per_pass;
-        constexpr int pmax = 512 / NZP;
-        return p < pmax ? (p < 1 ? 1 : p) : pmax;
-    }
+    if constexpr (K == 0) return 8;
+    else return 512 / Lanes<T, K>::NZP < 8 ? 512 / Lanes<T, K>::NZP : 8;
 }
 
-// The pipelined kernel drains a whole 512-word chunk (8 words per lane) in
-// one batch.  Narrower batches would need smaller chunks, whose fixed decode
-// cost outweighs the overlap on images with few entries per word (a 256-word
-// chunk took 340 ms against 192 ms for the 1024-word chunks of spmm_lane on
-// R-MAT scale 27 with k = 32 FP32), so those widths use spmm_lane.
-template <typename T, int K>
-constexpr int pipe_words()
-{
-    if constexpr (K == 0) return 0;
-    else return Lanes<T, K>::NZP * passes_for<T, K>() == 512 ? 512 : 0;
-}
-
-template <typename T, int K>
-constexpr bool use_pipe() { return pipe_words<T, K>() != 0; }
-
-constexpr int kLaneR = 16;  // words per lane in the non-pipelined kernel
+constexpr int kLaneR = 8;  // words per lane: 512-word chunks
 
 template <typename T, int K>
 uint32_t wave_bytes_for(uint32_t band_bytes)
 {
     constexpr int P = passes_for<T, K>();
-    constexpr int R = use_pipe<T, K>() ? pipe_words<T, K>() / 64 : kLaneR;
-    constexpr int CW = 64 * R;
+    constexpr int CW = 64 * kLaneR;
     constexpr int QS = CW + CW / (K ? P : CW) + 1;
     const uint32_t win_off = (band_bytes + QS * 6 + 7) & ~7u;
-    return (win_off + (LaneCfg<R>::DWIN + 1) * 8 + 15) & ~15u;
+    return (win_off + (LaneCfg<kLaneR>::DWIN + 1) * 8 + 15) & ~15u;
 }
 
 template <typename T, int K>
@@ -1036,9 +947,7 @@ fmx_status launch_k(fmx_ctx *ctx, SpmmArgs a, const Plan *plan, hipStream_t stre
 {
     if (a.vdir) return launch_kw<T, K>(ctx, a, plan, stream);
     constexpr int P = passes_for<T, K>();
-    void (*kern)(SpmmArgs);
-    if constexpr (use_pipe<T, K>()) kern = spmm_pipe<T, K, P>;
-    else kern = spmm_lane<T, K, kLaneR, P>;
+    void (*kern)(SpmmArgs) = spmm_lane<T, K, kLaneR, P>;
     a.wave_bytes = wave_bytes_for<T, K>(a.band_bytes);
     const uint32_t lds = a.wave_bytes * kWPB;
     if (lds > ctx->lds_per_block) return fail(ctx, FMX_EUNSUPPORTED, "band does not fit in LDS");

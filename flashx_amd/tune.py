@@ -10,7 +10,7 @@ import time
 
 from . import fmx as F
 
-DEFAULT_CANDIDATES = ((6, 15), (7, 15), (8, 15))
+DEFAULT_CANDIDATES = ((5, 15), (6, 15), (7, 15))
 
 
 def _time_spmm(spm, X, Y, reps):
@@ -27,17 +27,29 @@ def _time_spmm(spm, X, Y, reps):
     return a.elapsed_time(b) / reps
 
 
-def autotune_tile(src, dst, n, k, dtype, flags=F.BUILD_DEDUP, candidates=DEFAULT_CANDIDATES, reps=3, X=None):
+def autotune_tile(src, dst, n, k, dtype, flags=F.BUILD_DEDUP, candidates=DEFAULT_CANDIDATES, reps=3, X=None,
+                  keep=False):
     """Returns ((log2_h, log2_w), {candidate: ms per SpMM}) for the edge list
-    src, dst (device tensors) multiplied by an n x k operand of `dtype`."""
+    src, dst (device tensors) multiplied by an n x k operand of `dtype`.  With
+    keep=True it also returns the fastest candidate's matrix and its build
+    seconds, so the caller need not build it again."""
     torch = F._torch()
     if X is None:
         X = torch.rand((n, k), device=src.device, dtype=torch.float32).to(dtype)
     Y = torch.empty((n, k), device=src.device, dtype=dtype)
-    times = {}
+    times, best, best_spm, best_build = {}, None, None, None
     for lh, lw in candidates:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
         spm = F.SparseMatrix.build(src, dst, n=n, log2_h=lh, log2_w=lw, flags=flags)
+        torch.cuda.synchronize()
+        build_s = time.perf_counter() - t0
         times[(lh, lw)] = _time_spmm(spm, X, Y, reps)
+        if best is None or times[(lh, lw)] < times[best]:
+            best, best_build = (lh, lw), build_s
+            best_spm = spm if keep else None
         del spm
-    best = min(times, key=times.get)
+    del Y
+    if keep:
+        return best, times, best_spm, best_build
     return best, times

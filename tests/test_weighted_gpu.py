@@ -113,3 +113,46 @@ def test_weighted_device_build_bit_exact(fmx, orc, dt, lh, lw, flags):
     data, offs = spm.download()
     assert np.array_equal(offs, img.offs)
     assert np.array_equal(data, img.data)
+
+
+def tile_starts_at_chunk_edges(img, esz, chunk_words):
+    """Tiles (after a chunk holding entries) that start exactly where an
+    unsplit block row's chunk of `chunk_words` words begins: the SpMM decode
+    must not carry value positions across such an edge."""
+    data, offs = img.data, img.offs
+    hits = 0
+    for br in range(len(offs) - 1):
+        p, end = int(offs[br]), int(offs[br + 1])
+        bw = p // 2
+        starts = []
+        while p < end:
+            tnnz = int(data[p + 8:p + 12].view("<u4")[0])
+            tnrow = int(data[p + 12:p + 14].view("<u2")[0])
+            starts.append(p // 2)
+            p += 16 + (2 * tnrow + 2 * tnnz + 2 + esz * tnnz if tnnz else 0)
+        edges = set(range((bw & ~7) + chunk_words, end // 2, chunk_words))
+        hits += sum(1 for s in starts[1:] if s in edges)
+    return hits
+
+
+@pytest.mark.parametrize("dt,lh,lw", [(np.float32, 6, 8), (np.float64, 7, 7), (np.int32, 6, 10),
+                                      (np.int64, 8, 6)])
+def test_weighted_tiles_starting_at_chunk_edges(fmx, orc, dt, lh, lw):
+    """Narrow tiles put many tile starts exactly on the decoder's chunk
+    edges (512- and 1024-word chunks); every value must still be the one
+    stored for its non-zero."""
+    s, d = orc.rmat_edges(15, 16 << 15, 77)
+    n = 1 << 15
+    rng = np.random.default_rng(lh * 16 + lw)
+    if np.issubdtype(dt, np.integer):
+        v = rng.integers(-9, 10, len(s)).astype(dt)
+    else:
+        v = (rng.random(len(s)) * 4 + 0.25).astype(dt)
+    img = orc.build_2d_w(s, d, v, n=n, log2h=lh, log2w=lw, flags=1)
+    esz = np.dtype(dt).itemsize
+    assert tile_starts_at_chunk_edges(img, esz, 512) > 10
+    assert tile_starts_at_chunk_edges(img, esz, 1024) > 5
+    spm = fmx.SparseMatrix.upload(img.data, img.offs)
+    for k in (1, 4, 16):
+        X = (rng.integers(-7, 8, (n, k)) if np.issubdtype(dt, np.integer) else rng.random((n, k))).astype(dt)
+        compare(spm.multiply(to_dev(X)).cpu().numpy(), orc.spmm(img, X))
