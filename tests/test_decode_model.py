@@ -254,13 +254,15 @@ def lane_decode_w(words_abs, bw, d, begin, pend, log2w, R, vw):
         cn = c + CW
         adv = sum(1 for e in win[:DWIN] if e[0] <= cn - bw)
         if cn < bw + pend:
-            c0 = qn - wq[adv - 1]
+            # window tile adv-1 holds word cn; one that starts exactly there
+            # has no columns in this chunk and no published value base
+            c0 = 0 if win[adv - 1][0] == cn - bw else qn - wq[adv - 1]
             ct += adv - 1
         c = cn
     return out
 
 
-@pytest.mark.parametrize("dt,lh,lw,R", [(np.float32, 5, 6, 16), (np.float64, 7, 10, 16), (np.int32, 4, 4, 16),
+@pytest.mark.parametrize("dt,lh,lw,R", [(np.float32, 5, 6, 8), (np.float64, 7, 10, 16), (np.int32, 4, 4, 8),
                                         (np.int64, 9, 9, 16)])
 def test_weighted_lane_decode_model(orc, dt, lh, lw, R):
     rng = np.random.default_rng(lh * 7 + lw)
@@ -280,3 +282,38 @@ def test_weighted_lane_decode_model(orc, dt, lh, lw, R):
             ref = seq_decode_w(words[bs:be], d, a0, a1, lw, vw)
             got = lane_decode_w(words, bs, d, a0, a1, lw, R, vw)
             assert got == ref, (br, a0, a1)
+
+
+def weighted_image(orc, dt, lh, lw, scale=10):
+    rng = np.random.default_rng(lh * 7 + lw)
+    s, dd = orc.rmat_edges(scale, 16 << scale, lh)
+    v = (rng.random(len(s)) * 10).astype(dt)
+    return orc.build_2d_w(s, dd, v, n=1 << scale, log2h=lh, log2w=lw, flags=orc.DEDUP)
+
+
+@pytest.mark.parametrize("dt,lh,lw", [(np.float32, 4, 4), (np.int64, 6, 5)])
+def test_weighted_decode_model_tile_at_chunk_edge(orc, dt, lh, lw):
+    """Whole block rows of images in which tiles start exactly on a chunk
+    edge right after a chunk that held entries: the next chunk's value
+    positions restart at the new tile (no columns carried into it)."""
+    img = weighted_image(orc, dt, lh, lw)
+    vw = np.dtype(dt).itemsize // 2
+    words = np.frombuffer(img.data, dtype=np.uint16)
+    offs = img.offs
+    for R in (8, 16):
+        edges_after_entries = 0
+        for br in range(len(offs) - 1):
+            bs, be = offs[br] // 2, offs[br + 1] // 2
+            d = tiles_of_w(words, bs, be, vw)
+            starts = set(e[0] for e in d[1:])
+            ref = seq_decode_w(words[bs:be], d, 0, be - bs, lw, vw)
+            entry = np.zeros(be - bs, bool)  # column words of the block row
+            for ts, _, vb in d:
+                body = words[bs + ts + 8:bs + vb]
+                entry[ts + 8:vb] = (body & 0x8000) == 0
+            for cn in range((bs & ~7) + 64 * R, be, 64 * R):
+                if cn - bs in starts and entry[max(0, cn - 64 * R - bs):cn - bs].any():
+                    edges_after_entries += 1
+            got = lane_decode_w(words, bs, d, 0, be - bs, lw, R, vw)
+            assert got == ref, (R, br)
+        assert edges_after_entries > 0, R
