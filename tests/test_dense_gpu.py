@@ -19,7 +19,8 @@ def bf16_round(x):
 
 @pytest.mark.parametrize("n,ka,kb", [(100003, 128, 16), (5000, 16, 16), (777, 20, 3), (65536, 64, 64),
                                      (1, 5, 7), (1000003, 96, 16), (70001, 33, 50), (4099, 250, 16),
-                                     (3001, 70, 20), (513, 250, 40)])
+                                     (3001, 70, 20), (513, 250, 40), (20011, 40, 16), (65537, 80, 9),
+                                     (3001, 48, 1)])
 def test_gram_f64(fmx, orc, n, ka, kb):
     rng = np.random.default_rng(n + ka)
     for dt in (np.float32, np.float64):
