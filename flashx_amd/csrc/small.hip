@@ -10,7 +10,7 @@
 namespace fmx {
 namespace {
 
-constexpr int kJT = 512;   // threads of the Jacobi workgroup
+constexpr int kJT = 1024;  // threads of the Jacobi workgroup (16 waves)
 
 // Cyclic Jacobi for a symmetric m x m matrix (m even, m <= 128), A in LDS.
 // Each round rotates m/2 disjoint pairs (round-robin ordering), rows first
@@ -38,6 +38,8 @@ __global__ void __launch_bounds__(kJT) jacobi_kernel(const double *Ain, int m, d
     }
     __syncthreads();
     const int half = m / 2;
+    const int k0 = tid / m, c0 = tid - k0 * m;  // this thread's first (pair, column)
+    const int dk = kJT / m, dc = kJT - dk * m;  // and its step
     int sweep = 0;
     double off = 0.0, tot = 0.0;
     for (; sweep < max_sweeps; sweep++) {
@@ -86,19 +88,21 @@ __global__ void __launch_bounds__(kJT) jacobi_kernel(const double *Ain, int m, d
                 sn[k] = s;
             }
             __syncthreads();
-            // rows p, q
-            for (int e = tid; e < half * m; e += kJT) {
-                const int k = e / m, col = e - k * m;
+            // rows p, q: element e = k*m + col of the half*m (pair, column)
+            // grid, stepped by kJT without divisions
+            for (int k = k0, col = c0; k < half;) {
                 const int p = pp[k], q = qq[k];
                 const double c = cs[k], s = sn[k];
                 const double ap = A[p * S + col], aq = A[q * S + col];
                 A[p * S + col] = c * ap - s * aq;
                 A[q * S + col] = s * ap + c * aq;
+                k += dk;
+                col += dc;
+                if (col >= m) { col -= m; k++; }
             }
             __syncthreads();
             // columns p, q of A and V
-            for (int e = tid; e < half * m; e += kJT) {
-                const int k = e / m, row = e - k * m;
+            for (int k = k0, row = c0; k < half;) {
                 const int p = pp[k], q = qq[k];
                 const double c = cs[k], s = sn[k];
                 const double ap = A[row * S + p], aq = A[row * S + q];
@@ -107,6 +111,9 @@ __global__ void __launch_bounds__(kJT) jacobi_kernel(const double *Ain, int m, d
                 const double vp = V[row * SV + p], vq = V[row * SV + q];
                 V[row * SV + p] = c * vp - s * vq;
                 V[row * SV + q] = s * vp + c * vq;
+                k += dk;
+                row += dc;
+                if (row >= m) { row -= m; k++; }
             }
             __syncthreads();
         }
