@@ -222,3 +222,31 @@ def test_fp32_within_bound_of_either_tile_order(fmx, orc, rmat16):
     for hil in (False, True):
         ref = orc.spmm_exec(img, X, 16, hil)
         assert rel_err(Y, ref) <= 1e-5, hil
+
+
+@pytest.mark.parametrize("lh", [6, 7])
+def test_config2_rmat22_spmm_parity(fmx, orc, lh):
+    """The scale-22 workload itself: R-MAT scale 22, edge factor 16, dedup,
+    in the autotuned 64 x 32768 tiles and in 128 x 32768 tiles, times
+    4,194,304 x 16 operands.  int32 bit-exact; FP32 within 1e-5 of the
+    oracle's reference-order FP32 sums and within one rounding (2^-24 =
+    5.96e-8 relative, plus the FP64 sums' own ~1e-12) of the FP64 sums,
+    since the kernel sums in FP64 and rounds once; FP64 within 1e-12; rows
+    without non-zeros exactly 0."""
+    n = 1 << 22
+    s, d = fmx.rmat_edges(22, 16 << 22, 22)
+    spm = fmx.SparseMatrix.build(s, d, n=n, log2_h=lh, log2_w=15, flags=fmx.BUILD_DEDUP)
+    del s, d
+    data, offs = spm.download()
+    img = orc.Image(data, offs, n)
+    rng = np.random.default_rng(22 + lh)
+    Xi = rng.integers(0, 8, (n, 16)).astype(np.int32)
+    assert np.array_equal(spm.multiply(to_dev(Xi)).cpu().numpy(), orc.spmm(img, Xi, threads=8))
+    del Xi
+    Xf = rng.random((n, 16)).astype(np.float32)
+    Y = spm.multiply(to_dev(Xf)).cpu().numpy()
+    assert rel_err(Y, orc.spmm(img, Xf, threads=8)) <= 1e-5
+    assert rel_err(Y, orc.spmm(img, Xf.astype(np.float64), threads=8)) <= 6.0e-8
+    del Xf, Y
+    Xd = rng.random((n, 16))
+    assert rel_err(spm.multiply(to_dev(Xd)).cpu().numpy(), orc.spmm(img, Xd, threads=8)) <= 1e-12
