@@ -64,6 +64,8 @@ def main():
     ap.add_argument("--log2-w", type=int, default=15)
     ap.add_argument("--large", action="store_true",
                     help="also run the R-MAT scale-27, k=32 SpMM with a sampled CPU check")
+    ap.add_argument("--large-log2-h", type=int, default=None,
+                    help="tile height (log2) of the scale-27 leg; default: autotuned over 16/32/64 rows")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-stream", action="store_true", help="skip the out-of-core (streamed image) leg")
     ap.add_argument("--no-floor", action="store_true", help="skip the gather-floor leg")
@@ -206,7 +208,8 @@ def main():
         del spm, X, Y
         torch.cuda.empty_cache()
     if args.large:
-        out["large"] = large_leg(F, torch)
+        out["large"] = large_leg(F, torch, candidates=((args.large_log2_h, 15),) if args.large_log2_h is not None
+                                 else ((4, 15), (5, 15), (6, 15)))
         torch.cuda.empty_cache()
     if not args.no_eigen:
         out["eigen"] = eigen_leg(F, torch, args.log2_h, args.log2_w)
