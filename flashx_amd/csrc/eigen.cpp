@@ -76,8 +76,8 @@ private:
     hipStream_t s_;
     uint64_t n_;
     int b_, m_;
-    Dev<T> V_, W_, T1_, T2_;
-    Dev<double> dinv_, H_, S_, ev_, C1_, C2_, G_, Rinv_, info_, X64_, Y64_, Ssel_, theta_, rn_, part_;
+    Dev<T> V_, W_, T1_;
+    Dev<double> dinv_, H_, S_, ev_, C1_, C2_, G_, Rinv_, info_, X64_, Y64_, Z64_, Ssel_, Scols_, theta_, rn_, part_;
     Dev<float> deg_;
     int spmm_blocks_ = 0, spmm64_blocks_ = 0;
 };
@@ -89,12 +89,12 @@ fmx_status Solver<T>::alloc()
     FMX_HIP(ctx_, hipMalloc(&V_.p, n * m_ * sizeof(T)));
     FMX_HIP(ctx_, hipMalloc(&W_.p, n * b_ * sizeof(T)));
     FMX_HIP(ctx_, hipMalloc(&T1_.p, n * b_ * sizeof(T)));
-    FMX_HIP(ctx_, hipMalloc(&T2_.p, n * b_ * sizeof(T)));
     FMX_HIP(ctx_, hipMalloc(&dinv_.p, n * sizeof(double)));
     FMX_HIP(ctx_, hipMalloc(&deg_.p, n * sizeof(float)));
     FMX_HIP(ctx_, hipMalloc(&H_.p, (size_t)m_ * m_ * sizeof(double)));
     FMX_HIP(ctx_, hipMalloc(&S_.p, (size_t)m_ * m_ * sizeof(double)));
     FMX_HIP(ctx_, hipMalloc(&Ssel_.p, (size_t)m_ * m_ * sizeof(double)));
+    FMX_HIP(ctx_, hipMalloc(&Scols_.p, (size_t)m_ * m_ * sizeof(double)));
     FMX_HIP(ctx_, hipMalloc(&ev_.p, (size_t)m_ * sizeof(double)));
     FMX_HIP(ctx_, hipMalloc(&C1_.p, (size_t)m_ * b_ * sizeof(double)));
     FMX_HIP(ctx_, hipMalloc(&C2_.p, (size_t)m_ * b_ * sizeof(double)));
@@ -103,6 +103,7 @@ fmx_status Solver<T>::alloc()
     FMX_HIP(ctx_, hipMalloc(&info_.p, 4 * sizeof(double)));
     FMX_HIP(ctx_, hipMalloc(&X64_.p, n * o_.nev * sizeof(double)));
     FMX_HIP(ctx_, hipMalloc(&Y64_.p, n * o_.nev * sizeof(double)));
+    FMX_HIP(ctx_, hipMalloc(&Z64_.p, n * o_.nev * sizeof(double)));
     FMX_HIP(ctx_, hipMalloc(&theta_.p, (size_t)m_ * sizeof(double)));
     FMX_HIP(ctx_, hipMalloc(&rn_.p, (size_t)m_ * sizeof(double)));
     FMX_HIP(ctx_, hipMalloc(&part_.p, (size_t)1024 * 256 * sizeof(double)));
@@ -113,24 +114,22 @@ fmx_status Solver<T>::alloc()
 template <typename T>
 fmx_status Solver<T>::apply(const T *X, int64_t ldx, T *Y, int64_t ldy)
 {
+    // D^-1/2 X, then A (D^-1/2 X) with the outer D^-1/2 applied to each
+    // row's FP64 sum before its rounding
     TRY(row_scale(ctx_, dt_of<T>(), n_, b_, dinv_.p, X, ldx, T1_.p, b_, s_));
-    TRY(fmx_spmm(ctx_, spm_, dt_of<T>(), b_, T1_.p, b_, T2_.p, b_, s_));
-    TRY(row_scale(ctx_, dt_of<T>(), n_, b_, dinv_.p, T2_.p, b_, Y, ldy, s_));
+    TRY(spmm_scaled(ctx_, spm_, dt_of<T>(), b_, T1_.p, b_, Y, ldy, dinv_.p, s_));
     spmm_blocks_++;
     return FMX_OK;
 }
 
-// FP64 operator on k columns (residual check); Y64 is the output
+// FP64 operator on k <= nev columns (residual check) into Y, through the
+// persistent Z64_ (the scaled input)
 template <typename T>
 fmx_status Solver<T>::apply64(const double *X, double *Y, int k)
 {
-    TRY(row_scale(ctx_, FMX_F64, n_, k, dinv_.p, X, k, Y, k, s_));
-    // reuse X64_-sized scratch: Y holds the scaled input, SpMM back into a temp
-    Dev<double> tmp;
-    FMX_HIP(ctx_, hipMalloc(&tmp.p, (n_ ? n_ : 1) * k * sizeof(double)));
-    TRY(fmx_spmm(ctx_, spm_, FMX_F64, k, Y, k, tmp.p, k, s_));
-    TRY(row_scale(ctx_, FMX_F64, n_, k, dinv_.p, tmp.p, k, Y, k, s_));
-    TRY(sync());
+    if (k > o_.nev) return fail(ctx_, FMX_EDIM, "residual block wider than nev");
+    TRY(row_scale(ctx_, FMX_F64, n_, k, dinv_.p, X, k, Z64_.p, k, s_));
+    TRY(spmm_scaled(ctx_, spm_, FMX_F64, k, Z64_.p, k, Y, k, dinv_.p, s_));
     spmm64_blocks_++;
     return FMX_OK;
 }
@@ -251,14 +250,13 @@ fmx_status Solver<T>::run(const void *X0, int64_t ldx0, fmx_dtype x0_dt, double 
         // Ritz vectors (FP64) of the wanted pairs and their FP64 residuals;
         // Ssel is m x m, the first nev columns are wanted: copy them to m x nev
         {
-            Dev<double> Sw;
-            FMX_HIP(ctx_, hipMalloc(&Sw.p, (size_t)m * nev * sizeof(double)));
-            FMX_HIP(ctx_, hipMemcpy2DAsync(Sw.p, nev * sizeof(double), Ssel_.p, m * sizeof(double),
+            double *Sw = Scols_.p;  // m x nev, stream-ordered reuse
+            FMX_HIP(ctx_, hipMemcpy2DAsync(Sw, nev * sizeof(double), Ssel_.p, m * sizeof(double),
                                            nev * sizeof(double), m, hipMemcpyDeviceToDevice, s_));
             if (std::is_same<T, float>::value)
-                TRY(gemm_tall_f32_f64(ctx_, n_, m, nev, 1.0, (const float *)V_.p, m, Sw.p, 0.0, X64_.p, nev, s_));
+                TRY(gemm_tall_f32_f64(ctx_, n_, m, nev, 1.0, (const float *)V_.p, m, Sw, 0.0, X64_.p, nev, s_));
             else
-                TRY(gemm_tall(ctx_, FMX_F64, n_, m, nev, 1.0, V_.p, m, Sw.p, 0.0, X64_.p, nev, s_));
+                TRY(gemm_tall(ctx_, FMX_F64, n_, m, nev, 1.0, V_.p, m, Sw, 0.0, X64_.p, nev, s_));
             TRY(apply64(X64_.p, Y64_.p, nev));
         }
         TRY(resid_norms(ctx_, Y64_.p, nev, X64_.p, nev, theta_.p, n_, nev, rn_.p, part_.p, 1024, s_));
@@ -290,11 +288,10 @@ fmx_status Solver<T>::run(const void *X0, int64_t ldx0, fmx_dtype x0_dt, double 
         // thick restart: V[:, :keep] = V S[:, :keep], H = diag(theta), then the
         // last residual block W (orthogonal to V) becomes block keep/b
         {
-            Dev<double> Sk;
-            FMX_HIP(ctx_, hipMalloc(&Sk.p, (size_t)m * keep * sizeof(double)));
-            FMX_HIP(ctx_, hipMemcpy2DAsync(Sk.p, keep * sizeof(double), Ssel_.p, m * sizeof(double),
+            double *Sk = Scols_.p;  // m x keep, stream-ordered after the Ritz-vector product
+            FMX_HIP(ctx_, hipMemcpy2DAsync(Sk, keep * sizeof(double), Ssel_.p, m * sizeof(double),
                                            keep * sizeof(double), m, hipMemcpyDeviceToDevice, s_));
-            TRY(gemm_tall(ctx_, dt_of<T>(), n_, m, keep, 1.0, V_.p, m, Sk.p, 0.0, V_.p, m, s_));
+            TRY(gemm_tall(ctx_, dt_of<T>(), n_, m, keep, 1.0, V_.p, m, Sk, 0.0, V_.p, m, s_));
             std::vector<double> hH((size_t)m * m, 0.0);
             for (int i = 0; i < keep; i++) hH[(size_t)i * m + i] = th[i];
             FMX_HIP(ctx_, hipMemcpyAsync(H_.p, hH.data(), (size_t)m * m * sizeof(double),

@@ -414,7 +414,7 @@ fmx_status finish_device_image(fmx_ctx *ctx, fmx_spm *s)
 // streams), and the copies wait for the caller's earlier work on its stream.
 // Split bands are reduced once, after the last chunk.
 fmx_status spmm_streamed(fmx_ctx *ctx, fmx_spm *spm, Plan *plan, fmx_dtype dtype, int k, const void *X,
-                         int64_t ldx, void *Y, int64_t ldy, hipStream_t s)
+                         int64_t ldx, void *Y, int64_t ldy, hipStream_t s, const double *rscale)
 {
     const size_t nch = spm->chunk_br.size() - 1;
     FMX_HIP(ctx, hipEventRecord(spm->ev_start, s));
@@ -432,7 +432,7 @@ fmx_status spmm_streamed(fmx_ctx *ctx, fmx_spm *spm, Plan *plan, fmx_dtype dtype
         // absolute word w of the image is at staging byte 2w - lo
         const uint16_t *img = (const uint16_t *)(spm->d_stage[b] - lo);
         fmx_status st = launch_spmm(ctx, spm, plan, dtype, k, X, ldx, Y, ldy, spm->partial.ptr, s, img,
-                                    plan->chunk_items[c], plan->chunk_items[c + 1], c + 1 == nch);
+                                    plan->chunk_items[c], plan->chunk_items[c + 1], c + 1 == nch, rscale);
         if (st) return st;
         FMX_HIP(ctx, hipEventRecord(spm->ev_free[b], s));
         spm->free_recorded[b] = true;
@@ -486,6 +486,46 @@ fmx_status alloc_pinned(fmx_ctx *ctx, fmx_spm *s, uint64_t len)
     memset(s->h_pinned + len, 0, padded - len);
     s->h_pinned_len = len;
     return FMX_OK;
+}
+
+// The SpMM entry (fmx_spmm) with an optional per-row output scale.
+fmx_status spmm_scaled(fmx_ctx *ctx, const fmx_spm *cspm, fmx_dtype dtype, int k, const void *X, int64_t ldx,
+                       void *Y, int64_t ldy, const double *rscale, hipStream_t s)
+{
+    if (!ctx || !cspm) return fail(ctx, FMX_EINVAL, "null context or matrix");
+    fmx_spm *spm = const_cast<fmx_spm *>(cspm);
+    if (spm->ctx != ctx) return fail(ctx, FMX_EINVAL, "matrix belongs to another context");
+    if (dtype != FMX_I32 && dtype != FMX_I64 && dtype != FMX_F32 && dtype != FMX_F64)
+        return fail(ctx, FMX_ETYPE, "wrong matrix type in spmm");
+    // a weighted matrix multiplies dense data of its own value type
+    // (sparse_matrix.cpp:754-766)
+    if (spm->esz) {
+        const int want = dtype == FMX_I32 ? 4 : dtype == FMX_I64 ? 6 : dtype == FMX_F32 ? 8 : 9;
+        if (spm->prim != want) return fail(ctx, FMX_ETYPE, "wrong matrix type in spmm");
+    }
+    if (k <= 0) return fail(ctx, FMX_EDIM, "k must be positive");
+    if (ldx < k || ldy < k) return fail(ctx, FMX_EDIM, "leading dimension smaller than k");
+    const int es = elem_size(dtype);
+    const int acc = dtype == FMX_F32 ? 8 : es;  // LDS accumulator bytes (FP32 sums in FP64)
+    if ((uint64_t)k * acc > 32768) return fail(ctx, FMX_EUNSUPPORTED, "dense rows wider than 32 KiB");
+    if ((!X && spm->ncols) || (!Y && spm->nrows)) return fail(ctx, FMX_EINVAL, "null dense operand");
+    if (spm->nrows == 0) return FMX_OK;
+    FMX_HIP(ctx, hipSetDevice(ctx->device));
+    Plan *plan = nullptr;
+    fmx_status st = get_plan(ctx, spm, band_rows(spm, k, acc), &plan);
+    if (st) return st;
+    size_t need = (size_t)plan->nslots * plan->hs * k * acc;
+    if (need > spm->partial.bytes) {
+        if (spm->partial.ptr) FMX_HIP(ctx, hipFree(spm->partial.ptr));
+        spm->partial.ptr = nullptr;
+        spm->partial.bytes = 0;
+        FMX_HIP(ctx, hipMalloc(&spm->partial.ptr, need));
+        spm->partial.bytes = need;
+    }
+    if (!spm->streamed())
+        return launch_spmm(ctx, spm, plan, dtype, k, X, ldx, Y, ldy, spm->partial.ptr, s,
+                           (const uint16_t *)spm->d_image, 0, plan->nitems, true, rscale);
+    return spmm_streamed(ctx, spm, plan, dtype, k, X, ldx, Y, ldy, s, rscale);
 }
 
 }  // namespace fmx
@@ -792,41 +832,7 @@ fmx_status fmx_spmm_alg_bytes(const fmx_spm *spm, fmx_dtype dtype, int k, uint64
 fmx_status fmx_spmm(fmx_ctx *ctx, const fmx_spm *cspm, fmx_dtype dtype, int k,
                     const void *X, int64_t ldx, void *Y, int64_t ldy, void *stream)
 {
-    if (!ctx || !cspm) return fail(ctx, FMX_EINVAL, "null context or matrix");
-    fmx_spm *spm = const_cast<fmx_spm *>(cspm);
-    if (spm->ctx != ctx) return fail(ctx, FMX_EINVAL, "matrix belongs to another context");
-    if (dtype != FMX_I32 && dtype != FMX_I64 && dtype != FMX_F32 && dtype != FMX_F64)
-        return fail(ctx, FMX_ETYPE, "wrong matrix type in spmm");
-    // a weighted matrix multiplies dense data of its own value type
-    // (sparse_matrix.cpp:754-766)
-    if (spm->esz) {
-        const int want = dtype == FMX_I32 ? 4 : dtype == FMX_I64 ? 6 : dtype == FMX_F32 ? 8 : 9;
-        if (spm->prim != want) return fail(ctx, FMX_ETYPE, "wrong matrix type in spmm");
-    }
-    if (k <= 0) return fail(ctx, FMX_EDIM, "k must be positive");
-    if (ldx < k || ldy < k) return fail(ctx, FMX_EDIM, "leading dimension smaller than k");
-    const int es = elem_size(dtype);
-    const int acc = dtype == FMX_F32 ? 8 : es;  // LDS accumulator bytes (FP32 sums in FP64)
-    if ((uint64_t)k * acc > 32768) return fail(ctx, FMX_EUNSUPPORTED, "dense rows wider than 32 KiB");
-    if ((!X && spm->ncols) || (!Y && spm->nrows)) return fail(ctx, FMX_EINVAL, "null dense operand");
-    if (spm->nrows == 0) return FMX_OK;
-    FMX_HIP(ctx, hipSetDevice(ctx->device));
-    Plan *plan = nullptr;
-    fmx_status st = get_plan(ctx, spm, band_rows(spm, k, acc), &plan);
-    if (st) return st;
-    size_t need = (size_t)plan->nslots * plan->hs * k * acc;
-    if (need > spm->partial.bytes) {
-        if (spm->partial.ptr) FMX_HIP(ctx, hipFree(spm->partial.ptr));
-        spm->partial.ptr = nullptr;
-        spm->partial.bytes = 0;
-        FMX_HIP(ctx, hipMalloc(&spm->partial.ptr, need));
-        spm->partial.bytes = need;
-    }
-    hipStream_t s = (hipStream_t)stream;
-    if (!spm->streamed())
-        return launch_spmm(ctx, spm, plan, dtype, k, X, ldx, Y, ldy, spm->partial.ptr, s,
-                           (const uint16_t *)spm->d_image, 0, plan->nitems, true);
-    return spmm_streamed(ctx, spm, plan, dtype, k, X, ldx, Y, ldy, s);
+    return fmx::spmm_scaled(ctx, cspm, dtype, k, X, ldx, Y, ldy, nullptr, (hipStream_t)stream);
 }
 
 fmx_status fmx_spmm_host(fmx_ctx *ctx, const fmx_spm *spm, fmx_dtype dtype, int k, const void *X, int64_t ldx,

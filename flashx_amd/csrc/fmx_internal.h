@@ -143,7 +143,13 @@ fmx_status launch_spmm(fmx_ctx *ctx, const fmx_spm *spm, const Plan *plan,
                        fmx_dtype dt, int k, const void *X, int64_t ldx,
                        void *Y, int64_t ldy, void *partial,
                        hipStream_t stream, const uint16_t *img,
-                       uint32_t item_begin, uint32_t item_end, bool reduce);
+                       uint32_t item_begin, uint32_t item_end, bool reduce,
+                       const double *rscale = nullptr);
+
+// fmx_spmm with each output row r scaled by rscale[r] (FP32/FP64 data; null:
+// none), applied to the FP64 sum before its single rounding.
+fmx_status spmm_scaled(fmx_ctx *ctx, const fmx_spm *spm, fmx_dtype dtype, int k, const void *X, int64_t ldx,
+                       void *Y, int64_t ldy, const double *rscale, hipStream_t stream);
 
 }  // namespace fmx
 

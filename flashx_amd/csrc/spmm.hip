@@ -29,6 +29,7 @@
 // FP32 data accumulate in an FP64 band and are rounded once on write-out.
 #include <algorithm>
 #include <cstdlib>
+#include <type_traits>
 
 #include "fmx_internal.h"
 
@@ -56,6 +57,7 @@ struct SpmmArgs {
     uint64_t ncols;
     const uint32_t *vdir;  // weighted images: values-region word of each tile
     int vwords;            // weighted images: u16 words per value (2 or 4)
+    const double *rscale;  // floating-point data: Y row r is scaled by rscale[r] (or null)
 };
 
 template <typename T, int E>
@@ -419,8 +421,11 @@ __device__ __forceinline__ void write_band(const SpmmArgs &a, const A *band, uin
         T *dst = (T *)a.Y + row0 * (uint64_t)a.ldy;
         for (uint32_t i = lane; i < rows * k; i += 64) {
             const uint32_t r = i / k, cc = i - r * k;
+            A v = band[r * S + bcol(cc)];
+            if constexpr (std::is_floating_point<T>::value)
+                if (a.rscale) v = (A)((double)v * a.rscale[row0 + r]);  // one rounding to T
             // written once and not read here: non-temporal
-            __builtin_nontemporal_store((T)band[r * S + bcol(cc)], &dst[r * (uint64_t)a.ldy + cc]);
+            __builtin_nontemporal_store((T)v, &dst[r * (uint64_t)a.ldy + cc]);
         }
     } else {
         A *dst = (A *)a.partial + (uint64_t)slot * a.hs * k;
@@ -877,7 +882,8 @@ spmm_lane_w(SpmmArgs a)
 // Y rows of a split band = partial slots summed in stream order.
 template <typename T, typename A>
 __global__ void __launch_bounds__(256) spmm_reduce(const Split *splits, const A *partial,
-                                                    uint32_t hs, uint32_t k, T *Y, int64_t ldy)
+                                                    uint32_t hs, uint32_t k, T *Y, int64_t ldy,
+                                                    const double *rscale)
 {
     const Split sp = splits[blockIdx.x];
     const uint32_t tot = sp.rows * k;
@@ -886,6 +892,8 @@ __global__ void __launch_bounds__(256) spmm_reduce(const Split *splits, const A 
         for (uint32_t s = 1; s < sp.nslots; s++)
             acc += partial[(uint64_t)(sp.slot_begin + s) * hs * k + i];
         uint32_t r = i / k, c = i - r * k;
+        if constexpr (std::is_floating_point<T>::value)
+            if (rscale) acc = (A)((double)acc * rscale[sp.row0 + r]);
         Y[(sp.row0 + r) * (uint64_t)ldy + c] = (T)acc;
     }
 }
@@ -969,7 +977,7 @@ template <typename T>
 fmx_status launch_t(fmx_ctx *ctx, const fmx_spm *spm, const Plan *plan, int k,
                     const void *X, int64_t ldx, void *Y, int64_t ldy, void *partial,
                     hipStream_t stream, const uint16_t *img, uint32_t item_begin, uint32_t item_end,
-                    bool reduce)
+                    bool reduce, const double *rscale)
 {
     SpmmArgs a;
     a.img = img;
@@ -992,6 +1000,7 @@ fmx_status launch_t(fmx_ctx *ctx, const fmx_spm *spm, const Plan *plan, int k,
     a.ncols = spm->ncols;
     a.vdir = spm->esz ? spm->d_vdir : nullptr;
     a.vwords = (int)(spm->esz / 2);
+    a.rscale = rscale;
     // 16-byte vector gathers need aligned rows; otherwise the runtime-width path.
     const int vb = (k * (int)sizeof(T) < 16) ? k * (int)sizeof(T) : 16;
     const bool aligned = ((uintptr_t)X % vb == 0) && ((ldx * (int64_t)sizeof(T)) % vb == 0);
@@ -1009,7 +1018,7 @@ fmx_status launch_t(fmx_ctx *ctx, const fmx_spm *spm, const Plan *plan, int k,
     if (st) return st;
     if (reduce && plan->nsplits) {
         hipLaunchKernelGGL((spmm_reduce<T, A>), dim3(plan->nsplits), dim3(256), 0, stream,
-                           plan->d_splits, (const A *)partial, plan->hs, (uint32_t)k, (T *)Y, ldy);
+                           plan->d_splits, (const A *)partial, plan->hs, (uint32_t)k, (T *)Y, ldy, rscale);
         FMX_HIP(ctx, hipGetLastError());
     }
     return FMX_OK;
@@ -1020,13 +1029,14 @@ fmx_status launch_t(fmx_ctx *ctx, const fmx_spm *spm, const Plan *plan, int k,
 fmx_status launch_spmm(fmx_ctx *ctx, const fmx_spm *spm, const Plan *plan, fmx_dtype dt,
                        int k, const void *X, int64_t ldx, void *Y, int64_t ldy,
                        void *partial, hipStream_t stream, const uint16_t *img,
-                       uint32_t ib, uint32_t ie, bool reduce)
+                       uint32_t ib, uint32_t ie, bool reduce, const double *rscale)
 {
+    if (rscale && dt != FMX_F32 && dt != FMX_F64) return fail(ctx, FMX_ETYPE, "row scaling needs FP data");
     switch (dt) {
-    case FMX_I32: return launch_t<int32_t>(ctx, spm, plan, k, X, ldx, Y, ldy, partial, stream, img, ib, ie, reduce);
-    case FMX_I64: return launch_t<int64_t>(ctx, spm, plan, k, X, ldx, Y, ldy, partial, stream, img, ib, ie, reduce);
-    case FMX_F32: return launch_t<float>(ctx, spm, plan, k, X, ldx, Y, ldy, partial, stream, img, ib, ie, reduce);
-    case FMX_F64: return launch_t<double>(ctx, spm, plan, k, X, ldx, Y, ldy, partial, stream, img, ib, ie, reduce);
+    case FMX_I32: return launch_t<int32_t>(ctx, spm, plan, k, X, ldx, Y, ldy, partial, stream, img, ib, ie, reduce, rscale);
+    case FMX_I64: return launch_t<int64_t>(ctx, spm, plan, k, X, ldx, Y, ldy, partial, stream, img, ib, ie, reduce, rscale);
+    case FMX_F32: return launch_t<float>(ctx, spm, plan, k, X, ldx, Y, ldy, partial, stream, img, ib, ie, reduce, rscale);
+    case FMX_F64: return launch_t<double>(ctx, spm, plan, k, X, ldx, Y, ldy, partial, stream, img, ib, ie, reduce, rscale);
     default: return fail(ctx, FMX_ETYPE, "wrong matrix type in spmm");
     }
 }

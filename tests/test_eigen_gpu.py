@@ -82,6 +82,30 @@ def test_rmat_vs_eigsh(fmx, orc, precision):
     check_pairs(N, evals, vecs, 1e-6)
 
 
+@pytest.mark.parametrize("streamed", [False, True])
+def test_rmat_split_bands_and_streamed(fmx, orc, monkeypatch, streamed):
+    """Hub bands split into partial slots (a small work cap), and the same
+    matrix streamed from host memory: the operator's outer D^-1/2, applied
+    by the SpMM to each row's sum, also reaches rows written by the split
+    reduction and by every streamed chunk."""
+    monkeypatch.setenv("FMX_ITEM_CAP", "300")
+    s, d = fmx.rmat_edges(13, 16 << 13, 77)
+    flags = fmx.BUILD_DEDUP | fmx.BUILD_SYMMETRIZE
+    spm = fmx.SparseMatrix.build(s, d, n=1 << 13, log2_h=8, log2_w=13, flags=flags)
+    if streamed:
+        data, offs = spm.download()
+        spm = fmx.SparseMatrix.upload_streamed(data, offs, chunk_bytes=1 << 14)
+        assert spm.streamed
+    evals, vecs, resid, st = fmx.compute_eigen(spm, nev=16, tol=1e-6, precision=0)
+    assert st["converged"] == 1 and np.all(resid <= 1e-6)
+    img = orc.build_2d(s.cpu().numpy().view(np.uint32), d.cpu().numpy().view(np.uint32),
+                       n=1 << 13, log2h=8, log2w=13, flags=orc.DEDUP | orc.SYMMETRIZE)
+    N = normalized_operator(orc, img)
+    ref = np.sort(sla.eigsh(N, k=16, which="LA", tol=1e-12, return_eigenvectors=False))[::-1]
+    assert np.allclose(evals, ref, atol=2e-6)
+    check_pairs(N, evals, vecs, 1e-6)
+
+
 def test_warm_start_from_solution(fmx, orc):
     import torch
     p, q = 30, 33
