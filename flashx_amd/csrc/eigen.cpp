@@ -10,7 +10,8 @@
 // assembled from the projection coefficients, so after a restart it simply
 // becomes diag(kept Ritz values) and fills again as the next blocks are
 // expanded.  Each expansion:
-//   W = N V_j                              (SpMM + two row scalings)
+//   W = N V_j                              (row scaling, then the SpMM with the
+//                                           outer scaling applied to its sums)
 //   C1 = V^T W, W -= V C1                  (first pass; BF16 MFMA Gram in the
 //                                           mixed mode, FP64 otherwise)
 //   C2 = V^T W, W -= V C2                  (FP64 reorthogonalisation)
