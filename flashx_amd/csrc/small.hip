@@ -38,8 +38,10 @@ __global__ void __launch_bounds__(kJT) jacobi_kernel(const double *Ain, int m, d
     }
     __syncthreads();
     const int half = m / 2;
-    const int k0 = tid / m, c0 = tid - k0 * m;  // this thread's first (pair, column)
+    const int k0 = tid / m, c0 = tid - k0 * m;  // this thread's first (pair, row)
     const int dk = kJT / m, dc = kJT - dk * m;  // and its step
+    const int kb0 = tid / half, lb0 = tid - kb0 * half;  // first (pair, pair) block
+    const int dkb = kJT / half, dlb = kJT - dkb * half;
     int sweep = 0;
     double off = 0.0, tot = 0.0;
     for (; sweep < max_sweeps; sweep++) {
@@ -88,26 +90,30 @@ __global__ void __launch_bounds__(kJT) jacobi_kernel(const double *Ain, int m, d
                 sn[k] = s;
             }
             __syncthreads();
-            // rows p, q: element e = k*m + col of the half*m (pair, column)
-            // grid, stepped by kJT without divisions
-            for (int k = k0, col = c0; k < half;) {
-                const int p = pp[k], q = qq[k];
-                const double c = cs[k], s = sn[k];
-                const double ap = A[p * S + col], aq = A[q * S + col];
-                A[p * S + col] = c * ap - s * aq;
-                A[q * S + col] = s * ap + c * aq;
-                k += dk;
-                col += dc;
-                if (col >= m) { col -= m; k++; }
+            // A <- G A G^T, one 2x2 block (rows of pair k, columns of pair l)
+            // per element of the half x half grid: the blocks are disjoint,
+            // so the row and the column rotation of a block need no barrier
+            // between them (the same operations as rotating all rows, then
+            // all columns)
+            for (int k = kb0, l = lb0; k < half;) {
+                const int p = pp[k], q = qq[k], u = pp[l], v = qq[l];
+                const double ck = cs[k], sk = sn[k], cl = cs[l], sl = sn[l];
+                const double a = A[p * S + u], b = A[p * S + v], c = A[q * S + u], d = A[q * S + v];
+                const double a1 = ck * a - sk * c, b1 = ck * b - sk * d;
+                const double c1 = sk * a + ck * c, d1 = sk * b + ck * d;
+                A[p * S + u] = cl * a1 - sl * b1;
+                A[p * S + v] = sl * a1 + cl * b1;
+                A[q * S + u] = cl * c1 - sl * d1;
+                A[q * S + v] = sl * c1 + cl * d1;
+                k += dkb;
+                l += dlb;
+                if (l >= half) { l -= half; k++; }
             }
-            __syncthreads();
-            // columns p, q of A and V
+            // V <- V G^T: element e = k*m + row of the half*m (pair, row) grid,
+            // stepped by kJT without divisions
             for (int k = k0, row = c0; k < half;) {
                 const int p = pp[k], q = qq[k];
                 const double c = cs[k], s = sn[k];
-                const double ap = A[row * S + p], aq = A[row * S + q];
-                A[row * S + p] = c * ap - s * aq;
-                A[row * S + q] = s * ap + c * aq;
                 const double vp = V[row * SV + p], vq = V[row * SV + q];
                 V[row * SV + p] = c * vp - s * vq;
                 V[row * SV + q] = s * vp + c * vq;

@@ -408,9 +408,33 @@ __device__ __forceinline__ void drain_generic(const SpmmArgs &a, A *band, uint32
     }
 }
 
+// The output scales of a band's first 128 rows, requested when the item
+// starts so that their latency is hidden by the item's work.
+struct RowScale {
+    double v[2];
+};
+
+template <typename T>
+__device__ __forceinline__ RowScale prefetch_rscale(const SpmmArgs &a, uint64_t row0, uint32_t rows, int lane)
+{
+    RowScale rs{{1.0, 1.0}};
+    if constexpr (std::is_floating_point<T>::value) {
+        if (a.rscale) {
+#pragma unroll
+            for (int h = 0; h < 2; h++)
+                if ((uint32_t)(h * 64 + lane) < rows) rs.v[h] = a.rscale[row0 + h * 64 + lane];
+        }
+    }
+    return rs;
+}
+
+// Writes the band to Y, or to its partial slot.  With row scales, the
+// scales of bands of at most 128 rows are staged in scratch LDS (the drained
+// queue), so the write loop waits on no global load.
 template <typename T, int K, typename A>
 __device__ __forceinline__ void write_band(const SpmmArgs &a, const A *band, uint32_t S, uint64_t row0,
-                                           uint32_t rows, uint32_t slot, int lane)
+                                           uint32_t rows, uint32_t slot, int lane, const RowScale &rs,
+                                           double *scratch)
 {
     const uint32_t k = K ? K : (uint32_t)a.k;
     auto bcol = [&](uint32_t cc) -> uint32_t {
@@ -419,11 +443,22 @@ __device__ __forceinline__ void write_band(const SpmmArgs &a, const A *band, uin
     };
     if (slot == kNone) {
         T *dst = (T *)a.Y + row0 * (uint64_t)a.ldy;
+        bool staged = false;
+        if constexpr (std::is_floating_point<T>::value) {
+            if (a.rscale && rows <= 128) {
+                wave_sync();
+#pragma unroll
+                for (int h = 0; h < 2; h++)
+                    if ((uint32_t)(h * 64 + lane) < rows) scratch[h * 64 + lane] = rs.v[h];
+                wave_sync();
+                staged = true;
+            }
+        }
         for (uint32_t i = lane; i < rows * k; i += 64) {
             const uint32_t r = i / k, cc = i - r * k;
             A v = band[r * S + bcol(cc)];
             if constexpr (std::is_floating_point<T>::value)
-                if (a.rscale) v = (A)((double)v * a.rscale[row0 + r]);  // one rounding to T
+                if (a.rscale) v = (A)((double)v * (staged ? scratch[r] : a.rscale[row0 + r]));  // one rounding
             // written once and not read here: non-temporal
             __builtin_nontemporal_store((T)v, &dst[r * (uint64_t)a.ldy + cc]);
         }
@@ -517,6 +552,7 @@ spmm_lane(SpmmArgs a)
     uint2 *win = (uint2 *)(wbase + ((a.band_bytes + QS * 6 + 7) & ~7u));
 
     ItemState st = item_state(a, a.items[item_id]);
+    const RowScale rs = prefetch_rscale<T>(a, st.row0, st.ic.rows, lane);
     const uint32_t S = a.band_stride;
     for (uint32_t i = lane; i < st.ic.rows * S; i += 64) band[i] = A(0);
     if (lane == 0) win[DWIN] = make_uint2(0xFFFFFFFFu, 0u);
@@ -558,7 +594,7 @@ spmm_lane(SpmmArgs a)
         c = cn;
     }
     }
-    write_band<T, K, A>(a, band, S, st.row0, st.ic.rows, st.slot, lane);
+    write_band<T, K, A>(a, band, S, st.row0, st.ic.rows, st.slot, lane, rs, (double *)qcol);
 }
 
 
@@ -751,6 +787,7 @@ spmm_lane_w(SpmmArgs a)
     int32_t *wq = (int32_t *)(wbase + win_off + (DWIN + 1) * 12);
 
     ItemState st = item_state(a, a.items[item_id]);
+    const RowScale rs = prefetch_rscale<T>(a, st.row0, st.ic.rows, lane);
     const uint32_t S = a.band_stride;
     for (uint32_t i = lane; i < st.ic.rows * S; i += 64) band[i] = A(0);
     if (lane == 0) {
@@ -876,7 +913,7 @@ spmm_lane_w(SpmmArgs a)
         c = cn;
     }
     }
-    write_band<T, K, A>(a, band, S, st.row0, st.ic.rows, st.slot, lane);
+    write_band<T, K, A>(a, band, S, st.row0, st.ic.rows, st.slot, lane, rs, (double *)qcol);
 }
 
 // Y rows of a split band = partial slots summed in stream order.
