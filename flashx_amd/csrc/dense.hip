@@ -412,6 +412,115 @@ __global__ void __launch_bounds__(kT) gram_bf16_kernel(const float *A, int64_t l
     }
 }
 
+// Fast path of gram_bf16_kernel for 16-byte aligned rows whose real widths
+// are multiples of 4 (ka <= 128, kb <= 32): each stage's rows are read as
+// float4 pieces, and the next stage's pieces are requested before the
+// current stage's MFMAs, so the loads overlap the products.
+constexpr int kBVA = kBK * 128 / 4 / kT;  // float4 pieces of A per thread and stage (8)
+constexpr int kBVB = kBK * 32 / 4 / kT;   // and of B (2)
+__global__ void __launch_bounds__(kT) gram_bf16_vec_kernel(const float *A, int64_t lda, const float *B,
+                                                           int64_t ldb, uint64_t n, int ka, int kb,
+                                                           int ka_real, int kb_real,
+                                                           uint64_t rows_per_blk, float *partial)
+{
+    extern __shared__ unsigned short smb[];
+    unsigned short *At = smb;                    // ka x kBPad  (A transposed)
+    unsigned short *Bt = smb + ka * kBPad;       // kb x kBPad
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int tiles_i = ka / 16, tiles_j = kb / 16, ntiles = tiles_i * tiles_j;
+    constexpr int kMaxTiles = 4;                 // ka <= 128, kb <= 32: at most 16 tiles
+    floatx4 acc[kMaxTiles];
+#pragma unroll
+    for (int t = 0; t < kMaxTiles; t++) acc[t] = floatx4{0.f, 0.f, 0.f, 0.f};
+    const uint64_t r0 = blockIdx.x * rows_per_blk;
+    const uint64_t r1 = min(n, r0 + rows_per_blk);
+    const int qa = ka / 4, qb = kb / 4;          // pieces per (padded) row
+    const int pa = kBK * qa, pb = kBK * qb;      // pieces per stage
+    float4 ra[kBVA], rbv[kBVB];
+    auto load = [&](uint64_t rb) {
+#pragma unroll
+        for (int t = 0; t < kBVA; t++) {
+            const int e = threadIdx.x + t * kT;
+            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (e < pa) {
+                const int r = e / qa, c4 = e - r * qa;
+                if (rb + r < r1 && c4 * 4 < ka_real) v = *(const float4 *)(A + (rb + r) * lda + c4 * 4);
+            }
+            ra[t] = v;
+        }
+#pragma unroll
+        for (int t = 0; t < kBVB; t++) {
+            const int e = threadIdx.x + t * kT;
+            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (e < pb) {
+                const int r = e / qb, c4 = e - r * qb;
+                if (rb + r < r1 && c4 * 4 < kb_real) v = *(const float4 *)(B + (rb + r) * ldb + c4 * 4);
+            }
+            rbv[t] = v;
+        }
+    };
+    auto store = [&]() {
+#pragma unroll
+        for (int t = 0; t < kBVA; t++) {
+            const int e = threadIdx.x + t * kT;
+            if (e < pa) {
+                const int r = e / qa, c = (e - r * qa) * 4;
+                At[(c + 0) * kBPad + r] = f2bf(ra[t].x);
+                At[(c + 1) * kBPad + r] = f2bf(ra[t].y);
+                At[(c + 2) * kBPad + r] = f2bf(ra[t].z);
+                At[(c + 3) * kBPad + r] = f2bf(ra[t].w);
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < kBVB; t++) {
+            const int e = threadIdx.x + t * kT;
+            if (e < pb) {
+                const int r = e / qb, c = (e - r * qb) * 4;
+                Bt[(c + 0) * kBPad + r] = f2bf(rbv[t].x);
+                Bt[(c + 1) * kBPad + r] = f2bf(rbv[t].y);
+                Bt[(c + 2) * kBPad + r] = f2bf(rbv[t].z);
+                Bt[(c + 3) * kBPad + r] = f2bf(rbv[t].w);
+            }
+        }
+    };
+    if (r0 < r1) load(r0);
+    for (uint64_t rb = r0; rb < r1; rb += kBK) {
+        store();
+        __syncthreads();
+        if (rb + kBK < r1) load(rb + kBK);
+#pragma unroll
+        for (int t = 0; t < kMaxTiles; t++) {
+            const int tile = w + 4 * t;
+            if (tile < ntiles) {
+                const int ti = tile / tiles_j, tj = tile - ti * tiles_j;
+#pragma unroll
+                for (int ks = 0; ks < kBK / 32; ks++) {
+                    const int k0 = ks * 32 + (lane >> 4) * 8;
+                    bf16x8 a = *(const bf16x8 *)(At + (ti * 16 + (lane & 15)) * kBPad + k0);
+                    bf16x8 b = *(const bf16x8 *)(Bt + (tj * 16 + (lane & 15)) * kBPad + k0);
+                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[t], 0, 0, 0);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    const int nout = ka_real * kb_real;
+#pragma unroll
+    for (int t = 0; t < kMaxTiles; t++) {
+        const int tile = w + 4 * t;
+        if (tile < ntiles) {
+            const int ti = tile / tiles_j, tj = tile - ti * tiles_j;
+            const int j = tj * 16 + (lane & 15);
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int i = ti * 16 + (lane >> 4) * 4 + q;
+                if (i < ka_real && j < kb_real)
+                    partial[(uint64_t)blockIdx.x * nout + i * kb_real + j] = acc[t][q];
+            }
+        }
+    }
+}
+
 // ---- gemm_tall -------------------------------------------------------------------
 // Y[r][j] = alpha * sum_i A[r][i] C[i][j] + beta * Y[r][j], FP64 accumulation.
 constexpr int kMRows = 32;
@@ -581,10 +690,18 @@ fmx_status gram(fmx_ctx *ctx, fmx_dtype dt, int mode, uint64_t n, int ka, int kb
         if (!attr) {
             FMX_HIP(ctx, hipFuncSetAttribute((const void *)gram_bf16_kernel,
                                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+            FMX_HIP(ctx, hipFuncSetAttribute((const void *)gram_bf16_vec_kernel,
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
             attr = true;
         }
-        hipLaunchKernelGGL(gram_bf16_kernel, dim3(nblk), dim3(kT), lds, s, (const float *)A, lda,
-                           (const float *)B, ldb, n, kap, kbp, ka, kb, rpb, (float *)ws);
+        const bool vec = ka % 4 == 0 && kb % 4 == 0 && kap <= 128 && kbp <= 32 && (uintptr_t)A % 16 == 0 &&
+                         (uintptr_t)B % 16 == 0 && lda % 4 == 0 && ldb % 4 == 0;
+        if (vec)
+            hipLaunchKernelGGL(gram_bf16_vec_kernel, dim3(nblk), dim3(kT), lds, s, (const float *)A, lda,
+                               (const float *)B, ldb, n, kap, kbp, ka, kb, rpb, (float *)ws);
+        else
+            hipLaunchKernelGGL(gram_bf16_kernel, dim3(nblk), dim3(kT), lds, s, (const float *)A, lda,
+                               (const float *)B, ldb, n, kap, kbp, ka, kb, rpb, (float *)ws);
         FMX_HIP(ctx, hipGetLastError());
         hipLaunchKernelGGL((reduce_partials<float, double>), dim3((nout + kT - 1) / kT), dim3(kT), 0, s,
                            (const float *)ws, (int)nblk, nout, G, 0);

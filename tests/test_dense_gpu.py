@@ -49,6 +49,21 @@ def test_gram_bf16_mfma(fmx, orc, n, ka, kb):
     assert np.all(np.abs(G - exact) <= 1e-2 * scale)
 
 
+@pytest.mark.parametrize("cols", [72, 96, 16])
+def test_gram_bf16_strided_window(fmx, orc, cols):
+    """BF16 Gram of a column window of a wider row-major basis (the
+    eigensolver's first projection pass: lda = 96, kb = 16)."""
+    rng = np.random.default_rng(cols)
+    Vfull = rng.standard_normal((70001, 96)).astype(np.float32)
+    W = rng.standard_normal((70001, 16)).astype(np.float32)
+    G = fmx.gram(dev(Vfull)[:, :cols], dev(W), mode=fmx.GRAM_BF16).cpu().numpy()
+    Ab = bf16_round(Vfull[:, :cols]).astype(np.float64)
+    Bb = bf16_round(W).astype(np.float64)
+    ref = orc.gram_f64(Ab, Bb)
+    scale = np.abs(Ab).T @ np.abs(Bb)
+    assert np.all(np.abs(G - ref) <= 2e-6 * scale)
+
+
 def test_gram_f64_strided(fmx, orc):
     """Operands that are column windows of wider row-major arrays (lda > ka)."""
     import torch
