@@ -177,6 +177,7 @@ def main():
         "cpu_baseline": None,
     }
 
+    out["operand_variants"] = variants_leg(F, torch, spm, n, check=not args.no_cpu)
     if cols_seq is not None:
         out["gather_floor"] = gather_floor_leg(F, torch, X, cols_seq, ms_per_step)
         del cols_seq
@@ -215,6 +216,51 @@ def main():
         out["eigen"] = eigen_leg(F, torch, args.log2_h, args.log2_w)
         out["learned_warm_start"] = learned_leg(F, torch)
     print(json.dumps(out), flush=True)
+
+
+def variants_leg(F, torch, spm, n, steps=10, check=True):
+    """The scale-22 matrix times its other named operands: 4,194,304 x 16 FP64
+    U[0,1) and int32 {0..7} (seeded), timed like the FP32 product; with
+    check, int32 is compared bit for bit and FP64 within 1e-12 relative with
+    the oracle's reference-order CPU product."""
+    import numpy as np
+    from oracle import oracle as O
+    g = torch.Generator(device="cuda")
+    g.manual_seed(SEED + 1)
+    out = {}
+    img = None
+    if check:
+        data, offs = spm.download()
+        img = O.Image(data, offs, n)
+    threads = len(os.sched_getaffinity(0))
+    for name, X in (("f64_k16", torch.rand((n, K), dtype=torch.float64, device="cuda", generator=g)),
+                    ("i32_k16", torch.randint(0, 8, (n, K), dtype=torch.int32, device="cuda", generator=g))):
+        Y = torch.empty_like(X)
+        spm.multiply_async(X, Y)
+        torch.cuda.synchronize()
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(steps):
+            spm.multiply_async(X, Y)
+        b.record()
+        torch.cuda.synchronize()
+        ms = a.elapsed_time(b) / steps
+        r = {"ms": round(ms, 4), "GBps": round(spm.alg_bytes(X.dtype, K) / ms / 1e6, 1)}
+        if img is not None:
+            ref = O.spmm(img, X.cpu().numpy(), threads=threads)
+            got = Y.cpu().numpy()
+            if X.dtype == torch.int32:
+                r["bit_exact_vs_cpu"] = bool(np.array_equal(got, ref))
+            else:
+                nz = ref != 0
+                r["max_rel_err_vs_cpu"] = float((np.abs(got - ref)[nz] / np.abs(ref[nz])).max())
+                r["zero_rows_exact"] = bool(np.all(got[~nz] == 0))
+        log("spmm %s: %.3f ms, %.1f GB/s %s" % (name, ms, r["GBps"],
+                                               {k: v for k, v in r.items() if k not in ("ms", "GBps")}))
+        out[name] = r
+        del X, Y
+    return out
 
 
 def stream_columns(torch, src, dst, n, lh, lw, chunk_edges=1 << 28):
