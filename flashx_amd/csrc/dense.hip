@@ -128,7 +128,7 @@ struct Stage {
             Vec16<T> v;
 #pragma unroll
             for (int u = 0; u < V; u++) v.v[u] = T(0);
-            if (gr < r1) {
+            if (gr < r1 && col < k) {  // pieces wholly past the width stay zero
                 const T *src = M + gr * ld + col;
                 if (vec && col + V <= k) {
                     v = *(const Vec16<T> *)src;
