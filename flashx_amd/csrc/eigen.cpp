@@ -222,7 +222,7 @@ fmx_status Solver<T>::run(const void *X0, int64_t ldx0, fmx_dtype x0_dt, double 
     int jstart = 0;
     int restarts = 0;
     bool converged = false;
-    std::vector<double> hev(m), hS((size_t)m * m), hsel((size_t)m * m), hres(m);
+    std::vector<double> hev(m), hS((size_t)m * m), hsel((size_t)m * m), hres(m), hG((size_t)b * b);
     std::vector<int> order(m);
     const int keep = std::min(m - b, ((nev + b - 1) / b) * b + b);
     double worst = INFINITY;
@@ -234,10 +234,13 @@ fmx_status Solver<T>::run(const void *X0, int64_t ldx0, fmx_dtype x0_dt, double 
             TRY(h_update(ctx_, H_.p, m, cols, j * b, b, C1_.p, C2_.p, s_));
             if (j + 1 < o_.num_blocks) TRY(orth_block(W_.p, cols, (j + 1) * b));
         }
-        // Rayleigh-Ritz
+        // Rayleigh-Ritz; with it the Gram of the last residual block W, for
+        // the residual estimates of the Lanczos relation N V = V H + W E^T
         TRY(small_sym_eig(ctx_, H_.p, m, ev_.p, S_.p, info_.p, s_));
+        TRY(gram(ctx_, dt_of<T>(), FMX_GRAM_F64, n_, b, b, W_.p, b, W_.p, b, G_.p, s_));
         FMX_HIP(ctx_, hipMemcpyAsync(hev.data(), ev_.p, m * sizeof(double), hipMemcpyDeviceToHost, s_));
         FMX_HIP(ctx_, hipMemcpyAsync(hS.data(), S_.p, (size_t)m * m * sizeof(double), hipMemcpyDeviceToHost, s_));
+        FMX_HIP(ctx_, hipMemcpyAsync(hG.data(), G_.p, (size_t)b * b * sizeof(double), hipMemcpyDeviceToHost, s_));
         TRY(sync());
         std::iota(order.begin(), order.end(), 0);
         std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return hev[a] > hev[c]; });
@@ -248,43 +251,63 @@ fmx_status Solver<T>::run(const void *X0, int64_t ldx0, fmx_dtype x0_dt, double 
         FMX_HIP(ctx_, hipMemcpyAsync(Ssel_.p, hsel.data(), (size_t)m * m * sizeof(double),
                                      hipMemcpyHostToDevice, s_));
         FMX_HIP(ctx_, hipMemcpyAsync(theta_.p, th.data(), m * sizeof(double), hipMemcpyHostToDevice, s_));
-        // Ritz vectors (FP64) of the wanted pairs and their FP64 residuals;
-        // Ssel is m x m, the first nev columns are wanted: copy them to m x nev
-        {
-            double *Sw = Scols_.p;  // m x nev, stream-ordered reuse
-            FMX_HIP(ctx_, hipMemcpy2DAsync(Sw, nev * sizeof(double), Ssel_.p, m * sizeof(double),
-                                           nev * sizeof(double), m, hipMemcpyDeviceToDevice, s_));
-            if (std::is_same<T, float>::value)
-                TRY(gemm_tall_f32_f64(ctx_, n_, m, nev, 1.0, (const float *)V_.p, m, Sw, 0.0, X64_.p, nev, s_));
-            else
-                TRY(gemm_tall(ctx_, FMX_F64, n_, m, nev, 1.0, V_.p, m, Sw, 0.0, X64_.p, nev, s_));
-            TRY(apply64(X64_.p, Y64_.p, nev));
-        }
-        TRY(resid_norms(ctx_, Y64_.p, nev, X64_.p, nev, theta_.p, n_, nev, rn_.p, part_.p, 1024, s_));
-        FMX_HIP(ctx_, hipMemcpyAsync(hres.data(), rn_.p, nev * sizeof(double), hipMemcpyDeviceToHost, s_));
-        TRY(sync());
-        worst = 0.0;
+        // Estimated residual of Ritz pair i (y = V s_i): ||N y - theta y|| =
+        // ||W s_i,last|| with s_i,last the coordinates of s_i on the last
+        // block.  The FP64 check below (Ritz vectors, an FP64 SpMM) runs only
+        // once every wanted estimate is within twice the tolerance, and at
+        // the restart limit; it alone decides convergence.
+        double est_worst = 0.0;
         for (int i = 0; i < nev; i++) {
-            hres[i] /= std::max(std::fabs(th[i]), 1e-300);
-            worst = std::max(worst, hres[i]);
+            double q = 0.0;
+            for (int r = 0; r < b; r++)
+                for (int c = 0; c < b; c++)
+                    q += hsel[(size_t)(m - b + r) * m + i] * hG[(size_t)r * b + c] * hsel[(size_t)(m - b + c) * m + i];
+            est_worst = std::max(est_worst, std::sqrt(std::max(q, 0.0)) / std::max(std::fabs(th[i]), 1e-300));
         }
-        if (o_.verbose)
-            fprintf(stderr, "[fmx eigen] restart %d: spmm blocks %d, theta[0]=%.12f theta[nev-1]=%.12f "
-                            "max rel resid %.3e\n", restarts, spmm_blocks_, th[0], th[nev - 1], worst);
-        if (worst <= o_.tol) {
-            converged = true;
-            for (int i = 0; i < nev; i++) {
-                evals[i] = th[i];
-                if (resid) resid[i] = hres[i];
+        if (est_worst > 2.0 * o_.tol && restarts < o_.max_restarts) {
+            worst = est_worst;
+            if (o_.verbose)
+                fprintf(stderr, "[fmx eigen] restart %d: spmm blocks %d, estimated max rel resid %.3e\n",
+                        restarts, spmm_blocks_, est_worst);
+        } else {
+            // Ritz vectors (FP64) of the wanted pairs and their FP64 residuals;
+            // Ssel is m x m, the first nev columns are wanted: copy them to m x nev
+            {
+                double *Sw = Scols_.p;  // m x nev, stream-ordered reuse
+                FMX_HIP(ctx_, hipMemcpy2DAsync(Sw, nev * sizeof(double), Ssel_.p, m * sizeof(double),
+                                               nev * sizeof(double), m, hipMemcpyDeviceToDevice, s_));
+                if (std::is_same<T, float>::value)
+                    TRY(gemm_tall_f32_f64(ctx_, n_, m, nev, 1.0, (const float *)V_.p, m, Sw, 0.0, X64_.p, nev, s_));
+                else
+                    TRY(gemm_tall(ctx_, FMX_F64, n_, m, nev, 1.0, V_.p, m, Sw, 0.0, X64_.p, nev, s_));
+                TRY(apply64(X64_.p, Y64_.p, nev));
             }
-            break;
-        }
-        if (restarts >= o_.max_restarts) {
+            TRY(resid_norms(ctx_, Y64_.p, nev, X64_.p, nev, theta_.p, n_, nev, rn_.p, part_.p, 1024, s_));
+            FMX_HIP(ctx_, hipMemcpyAsync(hres.data(), rn_.p, nev * sizeof(double), hipMemcpyDeviceToHost, s_));
+            TRY(sync());
+            worst = 0.0;
             for (int i = 0; i < nev; i++) {
-                evals[i] = th[i];
-                if (resid) resid[i] = hres[i];
+                hres[i] /= std::max(std::fabs(th[i]), 1e-300);
+                worst = std::max(worst, hres[i]);
             }
-            break;
+            if (o_.verbose)
+                fprintf(stderr, "[fmx eigen] restart %d: spmm blocks %d, theta[0]=%.12f theta[nev-1]=%.12f "
+                                "max rel resid %.3e\n", restarts, spmm_blocks_, th[0], th[nev - 1], worst);
+            if (worst <= o_.tol) {
+                converged = true;
+                for (int i = 0; i < nev; i++) {
+                    evals[i] = th[i];
+                    if (resid) resid[i] = hres[i];
+                }
+                break;
+            }
+            if (restarts >= o_.max_restarts) {
+                for (int i = 0; i < nev; i++) {
+                    evals[i] = th[i];
+                    if (resid) resid[i] = hres[i];
+                }
+                break;
+            }
         }
         // thick restart: V[:, :keep] = V S[:, :keep], H = diag(theta), then the
         // last residual block W (orthogonal to V) becomes block keep/b
