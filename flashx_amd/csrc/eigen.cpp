@@ -17,9 +17,12 @@
 //   C2 = V^T W, W -= V C2                  (FP64 reorthogonalisation)
 //   H[:, j] = C1 + C2
 //   V_{j+1} = W R^-1, R from Cholesky-QR twice (FP64 Gram + Cholesky)
-// When the basis is full: Rayleigh-Ritz by FP64 Jacobi on the device, Ritz
-// vectors X = V S in FP64, residuals ||N X - X Theta|| with an FP64 SpMM.
-// Converged when every wanted pair has ||N x - theta x|| / |theta| <= tol.
+// When the basis is full: Rayleigh-Ritz by FP64 Jacobi on the device, and
+// residual estimates ||W s_last|| from the Lanczos relation N V = V H + W E^T.
+// Once every wanted estimate is within twice the tolerance: Ritz vectors
+// X = V S in FP64 and residuals ||N X - X Theta|| with an FP64 SpMM.
+// Converged when every wanted pair has ||N x - theta x|| / |theta| <= tol in
+// that FP64 check.
 //
 // D^-1/2 := 0 for isolated vertices: the reference's 1/sqrt(deg)
 // (test-eigen.cpp:69-71) would be inf there and turn their rows into NaN.
