@@ -1028,7 +1028,8 @@ fmx_status launch_t(fmx_ctx *ctx, const fmx_spm *spm, const Plan *plan, int k,
     a.nitems = item_end - item_begin;
     a.hs = plan->hs;
     // pad band rows by one element to spread the LDS adds over the banks
-    a.band_stride = (uint32_t)k + ((k >= 4 && getenv("FMX_NO_PAD") == nullptr) ? 1u : 0u);
+    static const bool no_pad = getenv("FMX_NO_PAD") != nullptr;  // read once per process
+    a.band_stride = (uint32_t)k + ((k >= 4 && !no_pad) ? 1u : 0u);
     using A = typename AccOf<T>::type;
     a.band_bytes = (uint32_t)(((uint64_t)plan->hs * a.band_stride * sizeof(A) + 15) & ~15ull);
     a.wave_bytes = 0;
