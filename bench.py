@@ -178,6 +178,7 @@ def main():
     }
 
     out["operand_variants"] = variants_leg(F, torch, spm, n, check=not args.no_cpu)
+    out["scale16"] = scale16_leg(F, torch, check=not args.no_cpu)
     if cols_seq is not None:
         out["gather_floor"] = gather_floor_leg(F, torch, X, cols_seq, ms_per_step)
         del cols_seq
@@ -260,6 +261,60 @@ def variants_leg(F, torch, spm, n, steps=10, check=True):
                                                {k: v for k, v in r.items() if k not in ("ms", "GBps")}))
         out[name] = r
         del X, Y
+    return out
+
+
+def scale16_leg(F, torch, check=True, steps=20):
+    """The CPU-runnable case: R-MAT scale 16, edge factor 16, dedup, times
+    65,536 x 8 FP64 U[0,1) and int32 {0..7} operands, in the reference's
+    default 32768 x 32768 tiles and in 64 x 32768 tiles; int32 bit-exact and
+    FP64 within 1e-12 relative of the oracle's reference-order CPU product.
+    The default tiles leave 2 block rows: each of the 256 LDS bands (hs = 256
+    rows at k = 8 FP64) decodes its whole block row, so that shape shows the
+    cost of tiles much taller than a band, not the kernel's throughput."""
+    import numpy as np
+    from oracle import oracle as O
+    scale, k = 16, 8
+    n = 1 << scale
+    src, dst = F.rmat_edges(scale, EDGE_FACTOR * n, scale)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(scale)
+    Xs = (("f64_k8", torch.rand((n, k), dtype=torch.float64, device="cuda", generator=g)),
+          ("i32_k8", torch.randint(0, 8, (n, k), dtype=torch.int32, device="cuda", generator=g)))
+    out = {"workload": "spmm rmat16_ef16, k=8"}
+    for lh, lw in ((15, 15), (6, 15)):
+        spm = F.SparseMatrix.build(src, dst, n=n, log2_h=lh, log2_w=lw, flags=F.BUILD_DEDUP)
+        tile = "%dx%d" % (1 << lh, 1 << lw)
+        res = {"nnz": int(spm.info.nnz)}
+        img = None
+        if check:
+            data, offs = spm.download()
+            img = O.Image(data, offs, n)
+        for name, X in Xs:
+            Y = torch.empty_like(X)
+            spm.multiply_async(X, Y)
+            torch.cuda.synchronize()
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            a.record()
+            for _ in range(steps):
+                spm.multiply_async(X, Y)
+            b.record()
+            torch.cuda.synchronize()
+            r = {"us": round(a.elapsed_time(b) / steps * 1e3, 1)}
+            if img is not None:
+                ref = O.spmm(img, X.cpu().numpy())
+                got = Y.cpu().numpy()
+                if X.dtype == torch.int32:
+                    r["bit_exact_vs_cpu"] = bool(np.array_equal(got, ref))
+                else:
+                    nz = ref != 0
+                    r["max_rel_err_vs_cpu"] = float((np.abs(got - ref)[nz] / np.abs(ref[nz])).max())
+            log("scale 16, %s tiles, %s: %.1f us %s" % (tile, name, r["us"],
+                                                       {kk: v for kk, v in r.items() if kk != "us"}))
+            res[name] = r
+        out[tile] = res
+        del spm
     return out
 
 
