@@ -155,11 +155,14 @@ struct Stage {
 };
 
 // G = A^T B, ka <= 16*TI, kb <= 16*TJ: per 16 staged rows, four K = 4 steps.
-template <typename T, int TI, int TJ>
+// SYM (B is A: same pointer, stride and width, TI == TJ) stages the rows once
+// and reads both MFMA operands from them.
+template <typename T, int TI, int TJ, bool SYM = false>
 __global__ void __launch_bounds__(kT) gram_stage_kernel(const T *A, int64_t lda, const T *B, int64_t ldb,
                                                         uint64_t n, int ka, int kb, uint64_t rows_per_wave,
                                                         uint64_t nwaves, int veca, int vecb, double *partial)
 {
+    static_assert(!SYM || TI == TJ, "a symmetric Gram has square tiles");
     constexpr int KAP = TI * 16, KBP = TJ * 16;
     constexpr int LA = KAP + 16, LB = KBP + 16;  // padded LDS row strides (elements)
     extern __shared__ __align__(16) unsigned char smg[];
@@ -178,15 +181,15 @@ __global__ void __launch_bounds__(kT) gram_stage_kernel(const T *A, int64_t lda,
     Stage<T, KBP> sb;
     if (r0 < r1) {
         sa.load(A, lda, ka, r0, r1, lane, veca);
-        sb.load(B, ldb, kb, r0, r1, lane, vecb);
+        if constexpr (!SYM) sb.load(B, ldb, kb, r0, r1, lane, vecb);
     }
     for (uint64_t r = r0; r < r1; r += kStageRows) {
         sa.template store<LA>(As, lane);
-        sb.template store<LB>(Bs, lane);
+        if constexpr (!SYM) sb.template store<LB>(Bs, lane);
         wave_lds_sync();
         if (r + kStageRows < r1) {
             sa.load(A, lda, ka, r + kStageRows, r1, lane, veca);
-            sb.load(B, ldb, kb, r + kStageRows, r1, lane, vecb);
+            if constexpr (!SYM) sb.load(B, ldb, kb, r + kStageRows, r1, lane, vecb);
         }
 #pragma unroll
         for (int ks = 0; ks < kStageRows / 4; ks++) {
@@ -195,7 +198,10 @@ __global__ void __launch_bounds__(kT) gram_stage_kernel(const T *A, int64_t lda,
 #pragma unroll
             for (int t = 0; t < TI; t++) a[t] = (double)As[row * LA + t * 16 + li];
 #pragma unroll
-            for (int t = 0; t < TJ; t++) b[t] = (double)Bs[row * LB + t * 16 + li];
+            for (int t = 0; t < TJ; t++) {
+                if constexpr (SYM) b[t] = a[t];
+                else b[t] = (double)Bs[row * LB + t * 16 + li];
+            }
 #pragma unroll
             for (int ti = 0; ti < TI; ti++)
 #pragma unroll
@@ -743,6 +749,8 @@ fmx_status gram(fmx_ctx *ctx, fmx_dtype dt, int mode, uint64_t n, int ka, int kb
         static bool attr_g = false;
         if (!attr_g) {
             const void *ks[] = {
+                (const void *)gram_stage_kernel<float, 1, 1, true>, (const void *)gram_stage_kernel<float, 2, 2, true>,
+                (const void *)gram_stage_kernel<double, 1, 1, true>, (const void *)gram_stage_kernel<double, 2, 2, true>,
                 (const void *)gram_stage_kernel<float, 3, 1>, (const void *)gram_stage_kernel<float, 6, 1>,
                 (const void *)gram_stage_kernel<double, 3, 1>, (const void *)gram_stage_kernel<double, 6, 1>,
                 (const void *)gram_stage_kernel<float, 1, 1>, (const void *)gram_stage_kernel<float, 2, 1>,
@@ -775,7 +783,23 @@ fmx_status gram(fmx_ctx *ctx, fmx_dtype dt, int mode, uint64_t n, int ka, int kb
         else if (tip == 8 || J == 2) FMX_GRAMS(TT, 8, (J == 2 ? 2 : 1));                                      \
         else FMX_GRAMS(TT, 16, 1);                                                                            \
     } while (0)
-        if (dt == FMX_F32) {
+        // A^T A with one staged operand (the Cholesky-QR Grams)
+        const bool sym = A == B && lda == ldb && ka == kb && tip == tjp && tip <= 2;
+        if (sym && dt == FMX_F32) {
+            if (tip == 1) hipLaunchKernelGGL((gram_stage_kernel<float, 1, 1, true>), dim3(nblk), dim3(kT), glds, s,
+                                             (const float *)A, lda, (const float *)B, ldb, n, ka, kb, rpw, waves,
+                                             veca, vecb, (double *)ws);
+            else hipLaunchKernelGGL((gram_stage_kernel<float, 2, 2, true>), dim3(nblk), dim3(kT), glds, s,
+                                    (const float *)A, lda, (const float *)B, ldb, n, ka, kb, rpw, waves, veca,
+                                    vecb, (double *)ws);
+        } else if (sym) {
+            if (tip == 1) hipLaunchKernelGGL((gram_stage_kernel<double, 1, 1, true>), dim3(nblk), dim3(kT), glds, s,
+                                             (const double *)A, lda, (const double *)B, ldb, n, ka, kb, rpw, waves,
+                                             veca, vecb, (double *)ws);
+            else hipLaunchKernelGGL((gram_stage_kernel<double, 2, 2, true>), dim3(nblk), dim3(kT), glds, s,
+                                    (const double *)A, lda, (const double *)B, ldb, n, ka, kb, rpw, waves, veca,
+                                    vecb, (double *)ws);
+        } else if (dt == FMX_F32) {
             if (tjp == 1) FMX_GRAMS_I(float, 1);
             else if (tjp == 2) FMX_GRAMS_I(float, 2);
             else FMX_GRAMS_I(float, 4);

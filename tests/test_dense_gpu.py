@@ -64,6 +64,23 @@ def test_gram_bf16_strided_window(fmx, orc, cols):
     assert np.all(np.abs(G - ref) <= 2e-6 * scale)
 
 
+@pytest.mark.parametrize("n,k,ld", [(100003, 16, 16), (4099, 32, 32), (70001, 16, 96), (777, 32, 40)])
+def test_gram_f64_symmetric(fmx, orc, n, k, ld):
+    """A^T A with B the same operand (one staged copy, the Cholesky-QR Grams),
+    also for column windows of wider arrays; equal to the general kernel's
+    result on a copy of A."""
+    rng = np.random.default_rng(n + k)
+    for dt in (np.float32, np.float64):
+        Afull = rng.standard_normal((n, ld)).astype(dt)
+        A = dev(Afull)[:, :k]
+        G = fmx.gram(A, A).cpu().numpy()
+        Gc = fmx.gram(A, A.clone()).cpu().numpy()  # distinct pointer: the general kernel
+        ref = orc.gram_f64(Afull[:, :k].astype(np.float64), Afull[:, :k].astype(np.float64))
+        scale = np.abs(Afull[:, :k].astype(np.float64)).T @ np.abs(Afull[:, :k].astype(np.float64))
+        assert np.all(np.abs(G - ref) <= 1e-13 * scale)
+        assert np.array_equal(G, Gc)
+
+
 def test_gram_f64_strided(fmx, orc):
     """Operands that are column windows of wider row-major arrays (lda > ka)."""
     import torch
