@@ -594,15 +594,16 @@ extern "C" fmx_status fmx_spm_build_w(fmx_ctx *ctx, const uint32_t *src, const u
 {
     if (!ctx) return FMX_EINVAL;
     if (m && !vals) return fmx::fail(ctx, FMX_EINVAL, "null values");
-    // prim_type of the header (generic_type.h:57-70): P_INTEGER, P_LONG, P_FLOAT, P_DOUBLE
+    // prim_type of the header (generic_type.h:57-70): P_INTEGER, P_LONG, P_ULONG, P_FLOAT, P_DOUBLE
     int32_t prim;
     uint32_t esz;
     switch (dtype) {
     case FMX_I32: prim = 4; esz = 4; break;
     case FMX_I64: prim = 6; esz = 8; break;
+    case FMX_U64: prim = 7; esz = 8; break;
     case FMX_F32: prim = 8; esz = 4; break;
     case FMX_F64: prim = 9; esz = 8; break;
-    default: return fmx::fail(ctx, FMX_ETYPE, "values must be int32, int64, float32 or float64");
+    default: return fmx::fail(ctx, FMX_ETYPE, "values must be int32, int64, uint64, float32 or float64");
     }
     static const uint8_t none = 0;
     return fmx::build_impl(ctx, src, dst, m, n, log2_h, log2_w, flags, vals ? vals : &none, esz, prim, out);

@@ -39,7 +39,7 @@ int elem_size(fmx_dtype dt)
 {
     switch (dt) {
     case FMX_I32: case FMX_F32: return 4;
-    case FMX_I64: case FMX_F64: return 8;
+    case FMX_I64: case FMX_U64: case FMX_F64: return 8;
     case FMX_BF16: return 2;
     }
     return 0;
@@ -84,12 +84,13 @@ fmx_status analyse_image(fmx_ctx *ctx, fmx_spm *s)
     if (s->nrows > (1ULL << 32) || s->ncols > (1ULL << 32))
         return fail(ctx, FMX_EFORMAT, "more than 2^32 rows or columns");
     // binary, or values of the header's primitive type: P_INTEGER / P_FLOAT
-    // (4 bytes), P_LONG / P_DOUBLE (8 bytes) (generic_type.h:57-70)
+    // (4 bytes), P_LONG / P_ULONG / P_DOUBLE (8 bytes) (generic_type.h:57-70)
     if (s->esz != 0) {
         const bool ok4 = s->esz == 4 && (s->prim == 4 || s->prim == 8);
-        const bool ok8 = s->esz == 8 && (s->prim == 6 || s->prim == 9);
+        const bool ok8 = s->esz == 8 && (s->prim == 6 || s->prim == 7 || s->prim == 9);
         if (!ok4 && !ok8)
-            return fail(ctx, FMX_EUNSUPPORTED, "weighted images need 4-byte int/float or 8-byte long/double values");
+            return fail(ctx, FMX_EUNSUPPORTED,
+                        "weighted images need 4-byte int/float or 8-byte long/size_t/double values");
     }
     const uint64_t nbr = (s->nrows + s->H - 1) / s->H;
     if (s->offs.size() != nbr + 1)
@@ -217,9 +218,10 @@ static uint32_t carry_at(const fmx_spm *s, uint64_t br, uint32_t &p, uint32_t &d
     return w[q - 1] & 0x7FFF;
 }
 
-fmx_status get_plan(fmx_ctx *ctx, fmx_spm *s, uint32_t hs, Plan **out)
+fmx_status get_plan(fmx_ctx *ctx, fmx_spm *s, uint32_t hs, bool ordered, Plan **out)
 {
-    auto it = s->plans.find(hs);
+    const uint32_t key = hs | (ordered ? 0x80000000u : 0u);
+    auto it = s->plans.find(key);
     if (it != s->plans.end()) { *out = it->second.get(); return FMX_OK; }
     std::unique_ptr<Plan> plan(new Plan());
     plan->hs = hs;
@@ -242,11 +244,12 @@ fmx_status get_plan(fmx_ctx *ctx, fmx_spm *s, uint32_t hs, Plan **out)
     uint64_t cap = std::max<uint64_t>(8192, 4 * avg);
     const char *e = getenv("FMX_ITEM_CAP");
     if (e) cap = std::max<uint64_t>(64, strtoull(e, nullptr, 10));
+    if (ordered) cap = UINT64_MAX;  // one item per band: its rows are summed in stream order
 
     std::vector<Split> splits;
     uint32_t nslots = 0;
     for (const Band &b : bands) {
-        uint32_t pieces = (uint32_t)((b.words + cap - 1) / cap);
+        uint32_t pieces = (uint32_t)(b.words / cap + (b.words % cap != 0));
         if (pieces < 1) pieces = 1;
         Item base{};
         base.brow_word = (uint64_t)s->offs[b.br] / 2;
@@ -316,7 +319,7 @@ fmx_status get_plan(fmx_ctx *ctx, fmx_spm *s, uint32_t hs, Plan **out)
     // cache footprint (time per non-zero is the same with 1/8 of X), and
     // every rotation rate tried was 5-10% slower than none.
     const char *re = getenv("FMX_ROT_SLOTS");
-    const uint64_t slots = re ? strtoull(re, nullptr, 10) : 0;
+    const uint64_t slots = (re && !ordered) ? strtoull(re, nullptr, 10) : 0;
     for (size_t i = 0; i < plan->items.size(); i++) {
         Item &t = plan->items[i];
         t.rot_begin = t.begin;
@@ -379,7 +382,7 @@ fmx_status get_plan(fmx_ctx *ctx, fmx_spm *s, uint32_t hs, Plan **out)
         FMX_HIP(ctx, hipMemcpy(plan->d_splits, splits.data(), plan->nsplits * sizeof(Split), hipMemcpyHostToDevice));
     }
     *out = plan.get();
-    s->plans[hs] = std::move(plan);
+    s->plans[key] = std::move(plan);
     return FMX_OK;
 }
 
@@ -403,7 +406,29 @@ fmx_status finish_device_image(fmx_ctx *ctx, fmx_spm *s)
     FMX_HIP(ctx, hipMemcpy(s->h_image.data(), s->d_image, s->bytes, hipMemcpyDeviceToHost));
     fmx_status st = analyse_image(ctx, s);
     if (st) return st;
-    return upload_dirs(ctx, s);
+    st = upload_dirs(ctx, s);
+    if (st) return st;
+    return prepare_work(ctx, s);
+}
+
+fmx_status prepare_work(fmx_ctx *ctx, fmx_spm *s)
+{
+    const uint32_t h2 = work_rows_for(s);
+    if (!h2) return FMX_OK;
+    std::unique_ptr<fmx_spm> w(new fmx_spm());
+    w->ctx = ctx;
+    retile_image(s, h2, w->h_image, w->offs);
+    fmx_status st = analyse_image(ctx, w.get());
+    if (st) return fail(ctx, FMX_EINVAL, "internal error: the re-cut work image is invalid: " + ctx->last_error);
+    if (w->nnz != s->nnz) return fail(ctx, FMX_EINVAL, "internal error: the re-cut work image lost non-zeros");
+    const size_t padded = (w->bytes + 4095) & ~size_t(4095);
+    FMX_HIP(ctx, hipMalloc(&w->d_image, padded));
+    FMX_HIP(ctx, hipMemcpy(w->d_image, w->h_image.data(), w->bytes, hipMemcpyHostToDevice));
+    if (padded > w->bytes) FMX_HIP(ctx, hipMemset(w->d_image + w->bytes, 0, padded - w->bytes));
+    st = upload_dirs(ctx, w.get());
+    if (st) return st;
+    s->work = std::move(w);
+    return FMX_OK;
 }
 
 // Out-of-core SpMM (the SAFS-streamed multiply of sparse_matrix.cpp:796-830
@@ -414,7 +439,7 @@ fmx_status finish_device_image(fmx_ctx *ctx, fmx_spm *s)
 // streams), and the copies wait for the caller's earlier work on its stream.
 // Split bands are reduced once, after the last chunk.
 fmx_status spmm_streamed(fmx_ctx *ctx, fmx_spm *spm, Plan *plan, fmx_dtype dtype, int k, const void *X,
-                         int64_t ldx, void *Y, int64_t ldy, hipStream_t s, const double *rscale)
+                         int64_t ldx, void *Y, int64_t ldy, hipStream_t s, const double *rscale, uint32_t flags)
 {
     const size_t nch = spm->chunk_br.size() - 1;
     FMX_HIP(ctx, hipEventRecord(spm->ev_start, s));
@@ -432,7 +457,7 @@ fmx_status spmm_streamed(fmx_ctx *ctx, fmx_spm *spm, Plan *plan, fmx_dtype dtype
         // absolute word w of the image is at staging byte 2w - lo
         const uint16_t *img = (const uint16_t *)(spm->d_stage[b] - lo);
         fmx_status st = launch_spmm(ctx, spm, plan, dtype, k, X, ldx, Y, ldy, spm->partial.ptr, s, img,
-                                    plan->chunk_items[c], plan->chunk_items[c + 1], c + 1 == nch, rscale);
+                                    plan->chunk_items[c], plan->chunk_items[c + 1], c + 1 == nch, rscale, flags);
         if (st) return st;
         FMX_HIP(ctx, hipEventRecord(spm->ev_free[b], s));
         spm->free_recorded[b] = true;
@@ -488,33 +513,55 @@ fmx_status alloc_pinned(fmx_ctx *ctx, fmx_spm *s, uint64_t len)
     return FMX_OK;
 }
 
-// The SpMM entry (fmx_spmm) with an optional per-row output scale.
-fmx_status spmm_scaled(fmx_ctx *ctx, const fmx_spm *cspm, fmx_dtype dtype, int k, const void *X, int64_t ldx,
-                       void *Y, int64_t ldy, const double *rscale, hipStream_t s)
+// Prim type a weighted image must carry for dense data of dtype
+// (sparse_matrix.cpp:754-766): P_INTEGER, P_LONG, P_ULONG, P_FLOAT, P_DOUBLE.
+static int value_prim(fmx_dtype dt)
+{
+    switch (dt) {
+    case FMX_I32: return 4;
+    case FMX_I64: return 6;
+    case FMX_U64: return 7;
+    case FMX_F32: return 8;
+    case FMX_F64: return 9;
+    default: return -1;
+    }
+}
+
+// The SpMM entries (fmx_spmm, fmx_spmm_ex, fmx_spmm_scaled).
+fmx_status spmm_exec(fmx_ctx *ctx, const fmx_spm *cspm, fmx_dtype dtype, int k, const double *pre,
+                     const double *post, const void *X, int64_t ldx, void *Y, int64_t ldy, uint32_t flags,
+                     bool mixed, hipStream_t s)
 {
     if (!ctx || !cspm) return fail(ctx, FMX_EINVAL, "null context or matrix");
     fmx_spm *spm = const_cast<fmx_spm *>(cspm);
     if (spm->ctx != ctx) return fail(ctx, FMX_EINVAL, "matrix belongs to another context");
-    if (dtype != FMX_I32 && dtype != FMX_I64 && dtype != FMX_F32 && dtype != FMX_F64)
-        return fail(ctx, FMX_ETYPE, "wrong matrix type in spmm");
+    if (value_prim(dtype) < 0) return fail(ctx, FMX_ETYPE, "wrong matrix type in spmm");
+    if (flags & ~(FMX_SPMM_SEQ | FMX_SPMM_X_COLMAJOR | FMX_SPMM_Y_COLMAJOR))
+        return fail(ctx, FMX_EINVAL, "unknown SpMM flags");
+    if ((pre || post) && dtype != FMX_F32 && dtype != FMX_F64)
+        return fail(ctx, FMX_ETYPE, "row scaling needs FP32 or FP64 data");
     // a weighted matrix multiplies dense data of its own value type
-    // (sparse_matrix.cpp:754-766)
-    if (spm->esz) {
-        const int want = dtype == FMX_I32 ? 4 : dtype == FMX_I64 ? 6 : dtype == FMX_F32 ? 8 : 9;
-        if (spm->prim != want) return fail(ctx, FMX_ETYPE, "wrong matrix type in spmm");
-    }
+    // (sparse_matrix.cpp:754-766); a custom creator may pair FP64 dense data
+    // with FP32 values (spmm_creator<double, float>)
+    if (spm->esz && spm->prim != value_prim(dtype) && !(mixed && dtype == FMX_F64 && spm->prim == 8))
+        return fail(ctx, FMX_ETYPE, "wrong matrix type in spmm");
     if (k <= 0) return fail(ctx, FMX_EDIM, "k must be positive");
-    if (ldx < k || ldy < k) return fail(ctx, FMX_EDIM, "leading dimension smaller than k");
+    const bool xcol = flags & FMX_SPMM_X_COLMAJOR, ycol = flags & FMX_SPMM_Y_COLMAJOR;
+    if ((xcol ? (uint64_t)ldx < spm->ncols : ldx < k) || (ycol ? (uint64_t)ldy < spm->nrows : ldy < k))
+        return fail(ctx, FMX_EDIM, "leading dimension smaller than the operand's rows (column-major) or k");
     const int es = elem_size(dtype);
-    const int acc = dtype == FMX_F32 ? 8 : es;  // LDS accumulator bytes (FP32 sums in FP64)
+    const bool seq = flags & FMX_SPMM_SEQ;
+    // LDS accumulator bytes: FP32 sums in FP64, except in the sequential mode
+    const int acc = (dtype == FMX_F32 && !seq) ? 8 : es;
     if ((uint64_t)k * acc > 32768) return fail(ctx, FMX_EUNSUPPORTED, "dense rows wider than 32 KiB");
     if ((!X && spm->ncols) || (!Y && spm->nrows)) return fail(ctx, FMX_EINVAL, "null dense operand");
     if (spm->nrows == 0) return FMX_OK;
     FMX_HIP(ctx, hipSetDevice(ctx->device));
+    if (spm->work) spm = spm->work.get();  // tiles taller than a band: the re-cut image
     Plan *plan = nullptr;
-    fmx_status st = get_plan(ctx, spm, band_rows(spm, k, acc), &plan);
+    fmx_status st = get_plan(ctx, spm, band_rows(spm, k, acc), seq, &plan);
     if (st) return st;
-    size_t need = (size_t)plan->nslots * plan->hs * k * acc;
+    size_t need = (size_t)plan->nslots * plan->hs * k * (dtype == FMX_F32 ? 8 : es);
     if (need > spm->partial.bytes) {
         if (spm->partial.ptr) FMX_HIP(ctx, hipFree(spm->partial.ptr));
         spm->partial.ptr = nullptr;
@@ -522,10 +569,33 @@ fmx_status spmm_scaled(fmx_ctx *ctx, const fmx_spm *cspm, fmx_dtype dtype, int k
         FMX_HIP(ctx, hipMalloc(&spm->partial.ptr, need));
         spm->partial.bytes = need;
     }
+    // a column-major or pre-scaled X is restaged row-major first (the
+    // reference converts a non-row-major input, sparse_matrix.cpp:720-736)
+    if ((xcol || pre) && spm->ncols) {
+        const size_t xb = spm->ncols * (size_t)k * es;
+        if (xb > ctx->xstage.bytes) {
+            if (ctx->xstage.ptr) FMX_HIP(ctx, hipFree(ctx->xstage.ptr));
+            ctx->xstage.ptr = nullptr;
+            ctx->xstage.bytes = 0;
+            FMX_HIP(ctx, hipMalloc(&ctx->xstage.ptr, xb));
+            ctx->xstage.bytes = xb;
+        }
+        st = stage_dense_rows(ctx, dtype, spm->ncols, k, X, ldx, xcol, pre, ctx->xstage.ptr, s);
+        if (st) return st;
+        X = ctx->xstage.ptr;
+        ldx = k;
+    }
+    const uint32_t kflags = flags & (FMX_SPMM_SEQ | FMX_SPMM_Y_COLMAJOR);
     if (!spm->streamed())
         return launch_spmm(ctx, spm, plan, dtype, k, X, ldx, Y, ldy, spm->partial.ptr, s,
-                           (const uint16_t *)spm->d_image, 0, plan->nitems, true, rscale);
-    return spmm_streamed(ctx, spm, plan, dtype, k, X, ldx, Y, ldy, s, rscale);
+                           (const uint16_t *)spm->d_image, 0, plan->nitems, true, post, kflags);
+    return spmm_streamed(ctx, spm, plan, dtype, k, X, ldx, Y, ldy, s, post, kflags);
+}
+
+fmx_status spmm_scaled(fmx_ctx *ctx, const fmx_spm *spm, fmx_dtype dtype, int k, const void *X, int64_t ldx,
+                       void *Y, int64_t ldy, const double *rscale, hipStream_t s)
+{
+    return spmm_exec(ctx, spm, dtype, k, nullptr, rscale, X, ldx, Y, ldy, 0, false, s);
 }
 
 }  // namespace fmx
@@ -636,6 +706,8 @@ fmx_status fmx_spm_upload(fmx_ctx *ctx, const void *image, size_t bytes,
     FMX_HIP(ctx, hipMemcpy(s->d_image, s->h_image.data(), s->bytes, hipMemcpyHostToDevice));
     if (padded > s->bytes) FMX_HIP(ctx, hipMemset(s->d_image + s->bytes, 0, padded - s->bytes));
     st = upload_dirs(ctx, s.get());
+    if (st) return st;
+    st = prepare_work(ctx, s.get());
     if (st) return st;
     *out = s.release();
     return FMX_OK;
@@ -832,7 +904,21 @@ fmx_status fmx_spmm_alg_bytes(const fmx_spm *spm, fmx_dtype dtype, int k, uint64
 fmx_status fmx_spmm(fmx_ctx *ctx, const fmx_spm *cspm, fmx_dtype dtype, int k,
                     const void *X, int64_t ldx, void *Y, int64_t ldy, void *stream)
 {
-    return fmx::spmm_scaled(ctx, cspm, dtype, k, X, ldx, Y, ldy, nullptr, (hipStream_t)stream);
+    return fmx::spmm_exec(ctx, cspm, dtype, k, nullptr, nullptr, X, ldx, Y, ldy, 0, false, (hipStream_t)stream);
+}
+
+fmx_status fmx_spmm_ex(fmx_ctx *ctx, const fmx_spm *cspm, fmx_dtype dtype, int k, const void *X, int64_t ldx,
+                       void *Y, int64_t ldy, uint32_t flags, void *stream)
+{
+    return fmx::spmm_exec(ctx, cspm, dtype, k, nullptr, nullptr, X, ldx, Y, ldy, flags, false,
+                          (hipStream_t)stream);
+}
+
+fmx_status fmx_spmm_scaled(fmx_ctx *ctx, const fmx_spm *cspm, fmx_dtype dtype, int k, const double *pre,
+                           const double *post, const void *X, int64_t ldx, void *Y, int64_t ldy, uint32_t flags,
+                           void *stream)
+{
+    return fmx::spmm_exec(ctx, cspm, dtype, k, pre, post, X, ldx, Y, ldy, flags, true, (hipStream_t)stream);
 }
 
 fmx_status fmx_spmm_host(fmx_ctx *ctx, const fmx_spm *spm, fmx_dtype dtype, int k, const void *X, int64_t ldx,

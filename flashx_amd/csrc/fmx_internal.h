@@ -85,6 +85,7 @@ struct fmx_ctx {
     size_t lds_per_block = 0;
     void *ws = nullptr;        // grow-only device workspace (dense-op partials)
     size_t ws_bytes = 0;
+    fmx::Scratch xstage;       // grow-only: a dense operand restaged row-major / pre-scaled
 };
 
 struct fmx_spm {
@@ -116,6 +117,9 @@ struct fmx_spm {
     hipEvent_t ev_free[2] = {nullptr, nullptr};   // last kernel reading each buffer
     bool free_recorded[2] = {false, false};
     hipEvent_t ev_start = nullptr;
+    // Resident images with tiles taller than an SpMM band: the same matrix
+    // re-cut into short tiles, which every SpMM runs on (retile.cpp).
+    std::unique_ptr<fmx_spm> work;
     bool streamed() const { return h_pinned != nullptr; }
     const uint8_t *host_image() const { return h_pinned ? h_pinned : h_image.data(); }
     uint64_t host_bytes() const { return h_pinned ? h_pinned_len : h_image.size(); }
@@ -131,20 +135,40 @@ int elem_size(fmx_dtype dt);
 
 // Validates a host image and fills the matrix fields + directory.
 fmx_status analyse_image(fmx_ctx *ctx, fmx_spm *spm);
-// Builds (or fetches) the work plan for band height hs.
-fmx_status get_plan(fmx_ctx *ctx, fmx_spm *spm, uint32_t hs, Plan **out);
+// The work image of a resident matrix with tiles taller than the work
+// tile height (retile.cpp): built, validated and uploaded into spm->work.
+fmx_status prepare_work(fmx_ctx *ctx, fmx_spm *spm);
+uint32_t work_rows_for(const fmx_spm *spm);
+void retile_image(const fmx_spm *spm, uint32_t h2, std::vector<uint8_t> &img, std::vector<int64_t> &offs);
+// Builds (or fetches) the work plan for band height hs.  An ordered plan
+// neither splits nor rotates items, so every band is one item walked from
+// its first word (the sequential mode needs this).
+fmx_status get_plan(fmx_ctx *ctx, fmx_spm *spm, uint32_t hs, bool ordered, Plan **out);
 // Band height for element size s and width k.
 uint32_t band_rows(const fmx_spm *spm, int k, int s);
 
 // Launches the SpMM kernel over plan items [item_begin, item_end) with the
 // image at img (absolute u16 word w of the image is img[w]) and, if reduce,
 // the in-order reduction of the plan's split bands.
+// the kernels' part of the FMX_SPMM_* flags (FMX_SPMM_SEQ, FMX_SPMM_Y_COLMAJOR).
 fmx_status launch_spmm(fmx_ctx *ctx, const fmx_spm *spm, const Plan *plan,
                        fmx_dtype dt, int k, const void *X, int64_t ldx,
                        void *Y, int64_t ldy, void *partial,
                        hipStream_t stream, const uint16_t *img,
                        uint32_t item_begin, uint32_t item_end, bool reduce,
-                       const double *rscale = nullptr);
+                       const double *rscale, uint32_t flags);
+
+// out (n x k, row-major, leading dimension k) = X, read row- or column-major
+// (xcol), times pre[i] per row when pre is given (FP data only).
+fmx_status stage_dense_rows(fmx_ctx *ctx, fmx_dtype dt, uint64_t n, int k, const void *X, int64_t ldx, bool xcol,
+                            const double *pre, void *out, hipStream_t stream);
+
+// Y = diag(post) A diag(pre) X with FMX_SPMM_* flags; pre / post may be null.
+// The weighted-value type may differ from dtype only when mixed is set
+// (FP64 dense data with FP32 values, spmm_creator<double, float>).
+fmx_status spmm_exec(fmx_ctx *ctx, const fmx_spm *spm, fmx_dtype dtype, int k, const double *pre,
+                     const double *post, const void *X, int64_t ldx, void *Y, int64_t ldy, uint32_t flags,
+                     bool mixed, hipStream_t stream);
 
 // fmx_spmm with each output row r scaled by rscale[r] (FP32/FP64 data; null:
 // none), applied to the FP64 sum before its single rounding.

@@ -58,6 +58,7 @@ struct SpmmArgs {
     const uint32_t *vdir;  // weighted images: values-region word of each tile
     int vwords;            // weighted images: u16 words per value (2 or 4)
     const double *rscale;  // floating-point data: Y row r is scaled by rscale[r] (or null)
+    int ycol;              // Y column-major: element (r, j) at Y[j * ldy + r]
 };
 
 template <typename T, int E>
@@ -382,6 +383,71 @@ __device__ __forceinline__ void accumulate_batch(A *band, uint32_t S, int lane, 
     }
 }
 
+// ---------------------------------------------------------------------------
+// Sequential mode: every output element is the left-to-right sum of its
+// row's products in stream order, one rounding per addition, starting from
+// the zeroed band -- the reference's own order (row_part_func / coo_func,
+// matrix/sparse_matrix.h:219-331, `dest_row[j] += src_row[j] * data` per
+// non-zero, tiles in ascending block column).  Within a tile a row's columns
+// ascend and tiles ascend, so this is each row's ascending-column sum, the
+// same for every tile shape.  Lane groups hold consecutive queue entries, so
+// they add in group order: group g reads the band after group g-1 wrote it
+// (one wave's LDS accesses execute in program order).  A run of one row
+// inside a group is chained in registers, which is the same sequence of
+// roundings.
+template <typename T, int K, int P>
+__device__ __forceinline__ void accumulate_batch_seq(T *band, uint32_t S, int lane, const Vec<T, Lanes<T, K>::E> *v,
+                                                     const uint32_t *rr)
+{
+#pragma clang fp contract(off)
+    using C = Lanes<T, K>;
+    const int q = lane % C::L;
+    const int g = lane / C::L;
+    for (int gg = 0; gg < C::NZP; gg++) {
+        if (g == gg) {
+            T acc[C::E];
+            uint32_t cur = 0xFFFFFFFFu;
+#pragma unroll
+            for (int i = 0; i < C::E; i++) acc[i] = T(0);
+#pragma unroll
+            for (int pp = 0; pp < P; pp++) {
+                if (rr[pp] != cur) {
+                    if (cur != 0xFFFFFFFFu) {
+                        T *dst = band + cur * S + q;
+#pragma unroll
+                        for (int i = 0; i < C::E; i++) dst[i * C::L] = acc[i];
+                    }
+                    cur = rr[pp];
+                    if (cur != 0xFFFFFFFFu) {
+                        const T *src = band + cur * S + q;
+#pragma unroll
+                        for (int i = 0; i < C::E; i++) acc[i] = src[i * C::L];
+                    }
+                }
+                if (cur != 0xFFFFFFFFu) {
+#pragma unroll
+                    for (int i = 0; i < C::E; i++) acc[i] = acc[i] + v[pp].v[i];
+                }
+            }
+            if (cur != 0xFFFFFFFFu) {
+                T *dst = band + cur * S + q;
+#pragma unroll
+                for (int i = 0; i < C::E; i++) dst[i * C::L] = acc[i];
+            }
+        }
+        wave_sync();
+    }
+}
+
+// Sequential mode at runtime widths: lane j owns output column j (of each
+// 64-column slice) and adds the queue's entries in order.  Weighted images
+// (W) multiply each gathered element by the entry's value of type V first,
+// rounded in T, as `src_row[j] * data` is.
+template <typename T, typename V, int P, bool W>
+__device__ __forceinline__ void drain_generic_seq(const SpmmArgs &a, T *band, uint32_t S, const uint32_t *qcol,
+                                                  const uint16_t *qrow, const uint32_t *qval, const uint16_t *vimg,
+                                                  uint32_t qn, int lane);
+
 // Runtime width: flat (entry, element) pairs, one element per lane.
 template <typename T, int P, typename A>
 __device__ __forceinline__ void drain_generic(const SpmmArgs &a, A *band, uint32_t S, const uint32_t *qcol,
@@ -405,6 +471,51 @@ __device__ __forceinline__ void drain_generic(const SpmmArgs &a, A *band, uint32
 #pragma unroll
         for (int pp = 0; pp < P; pp++)
             if (dst[pp] != 0xFFFFFFFFu) lds_add(band + dst[pp], (A)v[pp]);
+    }
+}
+
+// One value of type T from the u16-aligned values region (u16 loads, so no
+// alignment beyond 2 bytes is assumed).
+template <typename T>
+__device__ __forceinline__ T load_value(const uint16_t *p)
+{
+    if constexpr (sizeof(T) == 4) {
+        const uint32_t u = (uint32_t)p[0] | ((uint32_t)p[1] << 16);
+        return __builtin_bit_cast(T, u);
+    } else {
+        const uint64_t u = (uint64_t)p[0] | ((uint64_t)p[1] << 16) | ((uint64_t)p[2] << 32) |
+                           ((uint64_t)p[3] << 48);
+        return __builtin_bit_cast(T, u);
+    }
+}
+
+template <typename T, typename V, int P, bool W>
+__device__ __forceinline__ void drain_generic_seq(const SpmmArgs &a, T *band, uint32_t S, const uint32_t *qcol,
+                                                  const uint16_t *qrow, const uint32_t *qval, const uint16_t *vimg,
+                                                  uint32_t qn, int lane)
+{
+#pragma clang fp contract(off)
+    const T *X = (const T *)a.X;
+    const uint32_t k = (uint32_t)a.k;
+    for (uint32_t j0 = 0; j0 < k; j0 += 64) {
+        const uint32_t j = j0 + (uint32_t)lane;
+        const bool act = j < k;
+        for (uint32_t e0 = 0; e0 < qn; e0 += P) {
+            T v[P];
+            uint32_t dst[P];
+#pragma unroll
+            for (int pp = 0; pp < P; pp++) {
+                const uint32_t e = e0 + pp;
+                const uint32_t ec = e < qn ? e : qn - 1;
+                const uint32_t qr = qrow[ec];
+                v[pp] = X[(uint64_t)qcol[ec] * a.ldx + (act ? j : 0u)];
+                if constexpr (W) v[pp] = v[pp] * (T)load_value<V>(vimg + qval[ec]);
+                dst[pp] = (e < qn && act && qr != 0xFFFFu) ? qr * S + j : 0xFFFFFFFFu;
+            }
+#pragma unroll
+            for (int pp = 0; pp < P; pp++)
+                if (dst[pp] != 0xFFFFFFFFu) band[dst[pp]] = band[dst[pp]] + v[pp];
+        }
     }
 }
 
@@ -453,6 +564,19 @@ __device__ __forceinline__ void write_band(const SpmmArgs &a, const A *band, uin
                 wave_sync();
                 staged = true;
             }
+        }
+        if (a.ycol) {
+            // column-major Y: lanes run down a column, so the stores of one
+            // column are consecutive
+            T *dc = (T *)a.Y + row0;
+            for (uint32_t i = lane; i < rows * k; i += 64) {
+                const uint32_t cc = i / rows, r = i - cc * rows;
+                A v = band[r * S + bcol(cc)];
+                if constexpr (std::is_floating_point<T>::value)
+                    if (a.rscale) v = (A)((double)v * (staged ? scratch[r] : a.rscale[row0 + r]));
+                __builtin_nontemporal_store((T)v, &dc[cc * (uint64_t)a.ldy + r]);
+            }
+            return;
         }
         for (uint32_t i = lane; i < rows * k; i += 64) {
             const uint32_t r = i / k, cc = i - r * k;
@@ -530,7 +654,10 @@ __device__ __forceinline__ ItemState item_state(const SpmmArgs &a, const Item &i
 // the drain, so their latency hides behind the gathers; the gathers of one
 // wave are not overlapped with its own decode, other resident waves fill that
 // gap (four per SIMD where LDS allows).
-template <typename T, int K, int R, int P>
+// SEQ: the sequential mode (accumulate_batch_seq): the band holds T and
+// every element is summed in the reference's order; the plan of such a call
+// neither splits nor rotates items.
+template <typename T, int K, int R, int P, bool SEQ>
 __global__ void __launch_bounds__(64 * kWPB) __attribute__((amdgpu_waves_per_eu(4, 4)))
 spmm_lane(SpmmArgs a)
 {
@@ -545,7 +672,7 @@ spmm_lane(SpmmArgs a)
     if (item_id >= a.nitems) return;
 
     unsigned char *wbase = smem + wv * a.wave_bytes;
-    using A = typename AccOf<T>::type;
+    using A = typename std::conditional<SEQ, T, typename AccOf<T>::type>::type;
     A *band = (A *)wbase;
     uint32_t *qcol = (uint32_t *)(wbase + a.band_bytes);
     uint16_t *qrow = (uint16_t *)(qcol + QS);
@@ -585,10 +712,12 @@ spmm_lane(SpmmArgs a)
                 Vec<T, C::E> v[P];
                 uint32_t rr[P];
                 gather_batch<T, K, P, A>(a, qcol, qrow, base, qn, lane, v, rr);
-                accumulate_batch<T, K, P, A>(band, S, lane, v, rr);
+                if constexpr (SEQ) accumulate_batch_seq<T, K, P>(band, S, lane, v, rr);
+                else accumulate_batch<T, K, P, A>(band, S, lane, v, rr);
             }
-        } else {
-            if (qn) drain_generic<T, P, A>(a, band, S, qcol, qrow, qn, lane);
+        } else if (qn) {
+            if constexpr (SEQ) drain_generic_seq<T, T, P, false>(a, band, S, qcol, qrow, nullptr, nullptr, qn, lane);
+            else drain_generic<T, P, A>(a, band, S, qcol, qrow, qn, lane);
         }
         wave_sync();
         c = cn;
@@ -746,22 +875,7 @@ __device__ __forceinline__ uint32_t decode_chunk_w(const ItemCtx &ic, uint64_t c
     return qn;
 }
 
-// One value of type T from the u16-aligned values region (u16 loads, so no
-// alignment beyond 2 bytes is assumed).
-template <typename T>
-__device__ __forceinline__ T load_value(const uint16_t *p)
-{
-    if constexpr (sizeof(T) == 4) {
-        const uint32_t u = (uint32_t)p[0] | ((uint32_t)p[1] << 16);
-        return __builtin_bit_cast(T, u);
-    } else {
-        const uint64_t u = (uint64_t)p[0] | ((uint64_t)p[1] << 16) | ((uint64_t)p[2] << 32) |
-                           ((uint64_t)p[3] << 48);
-        return __builtin_bit_cast(T, u);
-    }
-}
-
-template <typename T, int K, int R, int P>
+template <typename T, typename V, int K, int R, int P, bool SEQ>
 __global__ void __launch_bounds__(64 * kWPB) __attribute__((amdgpu_waves_per_eu(4, 4)))
 spmm_lane_w(SpmmArgs a)
 {
@@ -776,7 +890,7 @@ spmm_lane_w(SpmmArgs a)
     if (item_id >= a.nitems) return;
 
     unsigned char *wbase = smem + wv * a.wave_bytes;
-    using A = typename AccOf<T>::type;
+    using A = typename std::conditional<SEQ, T, typename AccOf<T>::type>::type;
     A *band = (A *)wbase;
     uint32_t *qcol = (uint32_t *)(wbase + a.band_bytes);
     uint32_t *qval = qcol + QS;
@@ -847,7 +961,7 @@ spmm_lane_w(SpmmArgs a)
             for (uint32_t base = 0; base < qn; base += C::NZP * P) {
                 Vec<T, C::E> v[P];
                 uint32_t rr[P], gcs[P];
-                T vals[P];
+                V vals[P];
 #pragma unroll
                 for (int pp = 0; pp < P; pp++) {
                     const uint32_t e = base + g * P + pp;
@@ -857,11 +971,21 @@ spmm_lane_w(SpmmArgs a)
                     gcs[pp] = ok ? qcol[sl] : 0u;
                     rr[pp] = (ok && qr != 0xFFFFu) ? qr : 0xFFFFFFFFu;
                     const uint32_t vo = ok ? qval[sl] : 0u;  // a safe address when unused
-                    vals[pp] = load_value<T>(vimg + vo);
+                    vals[pp] = load_value<V>(vimg + vo);
                 }
 #pragma unroll
                 for (int pp = 0; pp < P; pp++)
                     v[pp] = *(const Vec<T, C::E> *)(X + (uint64_t)gcs[pp] * a.ldx + q * C::E);
+                if constexpr (SEQ) {
+                    // products rounded in T, then added in order
+#pragma clang fp contract(off)
+#pragma unroll
+                    for (int pp = 0; pp < P; pp++)
+#pragma unroll
+                        for (int i = 0; i < C::E; i++) v[pp].v[i] = v[pp].v[i] * (T)vals[pp];
+                    accumulate_batch_seq<T, K, P>(band, S, lane, v, rr);
+                    continue;
+                }
                 A acc[C::E];
                 uint32_t cur = 0xFFFFFFFFu;
 #pragma unroll
@@ -888,11 +1012,14 @@ spmm_lane_w(SpmmArgs a)
                     for (int i = 0; i < C::E; i++) lds_add(dst + i * C::L, acc[i]);
                 }
             }
+        } else if (SEQ) {
+            if (qn) drain_generic_seq<T, V, P, true>(a, (T *)band, S, qcol, qrow, qval, vimg, qn, lane);
         } else {
             const uint32_t k = (uint32_t)a.k;
             const uint32_t tot = qn * k;
             for (uint32_t base = 0; base < tot; base += 64 * P) {
-                T xv[P], vv[P];
+                T xv[P];
+                V vv[P];
                 uint32_t dst[P];
 #pragma unroll
                 for (int pp = 0; pp < P; pp++) {
@@ -901,7 +1028,7 @@ spmm_lane_w(SpmmArgs a)
                     const uint32_t e = fc / k, jj = fc - e * k;
                     const uint32_t qr = qrow[e];
                     xv[pp] = X[(uint64_t)qcol[e] * a.ldx + jj];
-                    vv[pp] = load_value<T>(vimg + qval[e]);
+                    vv[pp] = load_value<V>(vimg + qval[e]);
                     dst[pp] = (f < tot && qr != 0xFFFFu) ? qr * S + jj : 0xFFFFFFFFu;
                 }
 #pragma unroll
@@ -920,7 +1047,7 @@ spmm_lane_w(SpmmArgs a)
 template <typename T, typename A>
 __global__ void __launch_bounds__(256) spmm_reduce(const Split *splits, const A *partial,
                                                     uint32_t hs, uint32_t k, T *Y, int64_t ldy,
-                                                    const double *rscale)
+                                                    const double *rscale, int ycol)
 {
     const Split sp = splits[blockIdx.x];
     const uint32_t tot = sp.rows * k;
@@ -931,7 +1058,8 @@ __global__ void __launch_bounds__(256) spmm_reduce(const Split *splits, const A 
         uint32_t r = i / k, c = i - r * k;
         if constexpr (std::is_floating_point<T>::value)
             if (rscale) acc = (A)((double)acc * rscale[sp.row0 + r]);
-        Y[(sp.row0 + r) * (uint64_t)ldy + c] = (T)acc;
+        if (ycol) Y[c * (uint64_t)ldy + sp.row0 + r] = (T)acc;
+        else Y[(sp.row0 + r) * (uint64_t)ldy + c] = (T)acc;
     }
 }
 
@@ -951,71 +1079,87 @@ constexpr int passes_for()
 
 constexpr int kLaneR = 8;  // words per lane: 512-word chunks
 
-template <typename T, int K>
-uint32_t wave_bytes_for(uint32_t band_bytes)
+// Raises a kernel's dynamic LDS limit once per process.
+template <void (*KERN)(SpmmArgs)>
+fmx_status allow_lds(fmx_ctx *ctx)
 {
-    constexpr int P = passes_for<T, K>();
-    constexpr int CW = 64 * kLaneR;
-    constexpr int QS = CW + CW / (K ? P : CW) + 1;
-    const uint32_t win_off = (band_bytes + QS * 6 + 7) & ~7u;
-    return (win_off + (LaneCfg<kLaneR>::DWIN + 1) * 8 + 15) & ~15u;
+    static bool done = false;
+    if (!done) {
+        FMX_HIP(ctx, hipFuncSetAttribute((const void *)KERN, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         160 * 1024));
+        done = true;
+    }
+    return FMX_OK;
 }
 
-template <typename T, int K>
-fmx_status launch_kw(fmx_ctx *ctx, SpmmArgs a, const Plan *plan, hipStream_t stream)
+template <void (*KERN)(SpmmArgs)>
+fmx_status launch_items(fmx_ctx *ctx, const SpmmArgs &a, hipStream_t stream)
+{
+    const uint32_t lds = a.wave_bytes * kWPB;
+    if (lds > ctx->lds_per_block) return fail(ctx, FMX_EUNSUPPORTED, "band does not fit in LDS");
+    fmx_status st = allow_lds<KERN>(ctx);
+    if (st) return st;
+    const uint32_t blocks = (a.nitems + kWPB - 1) / kWPB;
+    if (blocks) {
+        hipLaunchKernelGGL(KERN, dim3(blocks), dim3(64 * kWPB), lds, stream, a);
+        FMX_HIP(ctx, hipGetLastError());
+    }
+    return FMX_OK;
+}
+
+// Weighted images; V is the type of the stored values (T, or float values
+// with FP64 dense data: spmm_creator<double, float>).
+template <typename T, typename V, int K, bool SEQ>
+fmx_status launch_kw(fmx_ctx *ctx, SpmmArgs a, hipStream_t stream)
 {
     constexpr int P = passes_for<T, K>();
     constexpr int R = kLaneR;
     constexpr int CW = 64 * R;
     constexpr int QS = CW + CW / (K ? P : CW) + 1;
-    auto kern = spmm_lane_w<T, K, R, P>;
     const uint32_t win_off = (a.band_bytes + QS * 10 + 7) & ~7u;
     a.wave_bytes = (win_off + (LaneCfg<R>::DWIN + 1) * 16 + 15) & ~15u;
-    const uint32_t lds = a.wave_bytes * kWPB;
-    if (lds > ctx->lds_per_block) return fail(ctx, FMX_EUNSUPPORTED, "band does not fit in LDS");
-    static bool attr_set = false;
-    if (!attr_set) {
-        FMX_HIP(ctx, hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                         160 * 1024));
-        attr_set = true;
-    }
-    const uint32_t blocks = (a.nitems + kWPB - 1) / kWPB;
-    if (blocks) {
-        hipLaunchKernelGGL(kern, dim3(blocks), dim3(64 * kWPB), lds, stream, a);
-        FMX_HIP(ctx, hipGetLastError());
-    }
-    return FMX_OK;
+    return launch_items<spmm_lane_w<T, V, K, R, P, SEQ>>(ctx, a, stream);
 }
 
-template <typename T, int K>
-fmx_status launch_k(fmx_ctx *ctx, SpmmArgs a, const Plan *plan, hipStream_t stream)
+template <typename T, int K, bool SEQ>
+fmx_status launch_k(fmx_ctx *ctx, SpmmArgs a, int vprim, hipStream_t stream)
 {
-    if (a.vdir) return launch_kw<T, K>(ctx, a, plan, stream);
+    if (a.vdir) {
+        if constexpr (std::is_same<T, double>::value)
+            if (vprim == 8) return launch_kw<double, float, K, SEQ>(ctx, a, stream);
+        return launch_kw<T, T, K, SEQ>(ctx, a, stream);
+    }
     constexpr int P = passes_for<T, K>();
-    void (*kern)(SpmmArgs) = spmm_lane<T, K, kLaneR, P>;
-    a.wave_bytes = wave_bytes_for<T, K>(a.band_bytes);
-    const uint32_t lds = a.wave_bytes * kWPB;
-    if (lds > ctx->lds_per_block) return fail(ctx, FMX_EUNSUPPORTED, "band does not fit in LDS");
-    static bool attr_set = false;
-    if (!attr_set) {
-        FMX_HIP(ctx, hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                         160 * 1024));
-        attr_set = true;
+    constexpr int CW = 64 * kLaneR;
+    constexpr int QS = CW + CW / (K ? P : CW) + 1;
+    const uint32_t win_off = (a.band_bytes + QS * 6 + 7) & ~7u;
+    a.wave_bytes = (win_off + (LaneCfg<kLaneR>::DWIN + 1) * 8 + 15) & ~15u;
+    return launch_items<spmm_lane<T, K, kLaneR, P, SEQ>>(ctx, a, stream);
+}
+
+template <typename T, bool SEQ>
+fmx_status launch_w(fmx_ctx *ctx, SpmmArgs a, int k, int vprim, hipStream_t stream)
+{
+    switch (k) {
+    case 1: return launch_k<T, 1, SEQ>(ctx, a, vprim, stream);
+    case 2: return launch_k<T, 2, SEQ>(ctx, a, vprim, stream);
+    case 4: return launch_k<T, 4, SEQ>(ctx, a, vprim, stream);
+    case 8: return launch_k<T, 8, SEQ>(ctx, a, vprim, stream);
+    case 16: return launch_k<T, 16, SEQ>(ctx, a, vprim, stream);
+    case 32: return launch_k<T, 32, SEQ>(ctx, a, vprim, stream);
+    case 64: return launch_k<T, 64, SEQ>(ctx, a, vprim, stream);
+    default: return launch_k<T, 0, SEQ>(ctx, a, vprim, stream);
     }
-    const uint32_t blocks = (a.nitems + kWPB - 1) / kWPB;
-    if (blocks) {
-        hipLaunchKernelGGL(kern, dim3(blocks), dim3(64 * kWPB), lds, stream, a);
-        FMX_HIP(ctx, hipGetLastError());
-    }
-    return FMX_OK;
 }
 
 template <typename T>
 fmx_status launch_t(fmx_ctx *ctx, const fmx_spm *spm, const Plan *plan, int k,
                     const void *X, int64_t ldx, void *Y, int64_t ldy, void *partial,
                     hipStream_t stream, const uint16_t *img, uint32_t item_begin, uint32_t item_end,
-                    bool reduce, const double *rscale)
+                    bool reduce, const double *rscale, uint32_t flags)
 {
+    // integer sums are exact in any order: the sequential mode is the default one
+    const bool seq = (flags & FMX_SPMM_SEQ) && std::is_floating_point<T>::value;
     SpmmArgs a;
     a.img = img;
     a.dir = spm->d_dir;
@@ -1030,8 +1174,8 @@ fmx_status launch_t(fmx_ctx *ctx, const fmx_spm *spm, const Plan *plan, int k,
     // pad band rows by one element to spread the LDS adds over the banks
     static const bool no_pad = getenv("FMX_NO_PAD") != nullptr;  // read once per process
     a.band_stride = (uint32_t)k + ((k >= 4 && !no_pad) ? 1u : 0u);
-    using A = typename AccOf<T>::type;
-    a.band_bytes = (uint32_t)(((uint64_t)plan->hs * a.band_stride * sizeof(A) + 15) & ~15ull);
+    const uint64_t acc = seq ? sizeof(T) : sizeof(typename AccOf<T>::type);
+    a.band_bytes = (uint32_t)(((uint64_t)plan->hs * a.band_stride * acc + 15) & ~15ull);
     a.wave_bytes = 0;
     a.log2w = spm->log2w;
     a.k = k;
@@ -1039,24 +1183,21 @@ fmx_status launch_t(fmx_ctx *ctx, const fmx_spm *spm, const Plan *plan, int k,
     a.vdir = spm->esz ? spm->d_vdir : nullptr;
     a.vwords = (int)(spm->esz / 2);
     a.rscale = rscale;
+    a.ycol = (flags & FMX_SPMM_Y_COLMAJOR) ? 1 : 0;
     // 16-byte vector gathers need aligned rows; otherwise the runtime-width path.
     const int vb = (k * (int)sizeof(T) < 16) ? k * (int)sizeof(T) : 16;
     const bool aligned = ((uintptr_t)X % vb == 0) && ((ldx * (int64_t)sizeof(T)) % vb == 0);
+    const int kk = aligned ? k : 0;
     fmx_status st;
-    switch (aligned ? k : 0) {
-    case 1: st = launch_k<T, 1>(ctx, a, plan, stream); break;
-    case 2: st = launch_k<T, 2>(ctx, a, plan, stream); break;
-    case 4: st = launch_k<T, 4>(ctx, a, plan, stream); break;
-    case 8: st = launch_k<T, 8>(ctx, a, plan, stream); break;
-    case 16: st = launch_k<T, 16>(ctx, a, plan, stream); break;
-    case 32: st = launch_k<T, 32>(ctx, a, plan, stream); break;
-    case 64: st = launch_k<T, 64>(ctx, a, plan, stream); break;
-    default: st = launch_k<T, 0>(ctx, a, plan, stream); break;
-    }
+    if constexpr (std::is_floating_point<T>::value)
+        st = seq ? launch_w<T, true>(ctx, a, kk, spm->prim, stream) : launch_w<T, false>(ctx, a, kk, spm->prim, stream);
+    else
+        st = launch_w<T, false>(ctx, a, kk, spm->prim, stream);
     if (st) return st;
     if (reduce && plan->nsplits) {
+        using A = typename AccOf<T>::type;
         hipLaunchKernelGGL((spmm_reduce<T, A>), dim3(plan->nsplits), dim3(256), 0, stream,
-                           plan->d_splits, (const A *)partial, plan->hs, (uint32_t)k, (T *)Y, ldy, rscale);
+                           plan->d_splits, (const A *)partial, plan->hs, (uint32_t)k, (T *)Y, ldy, rscale, a.ycol);
         FMX_HIP(ctx, hipGetLastError());
     }
     return FMX_OK;
@@ -1064,17 +1205,86 @@ fmx_status launch_t(fmx_ctx *ctx, const fmx_spm *spm, const Plan *plan, int k,
 
 }  // namespace
 
+// Row-major copy of a dense operand, optionally scaled by pre[i] per row
+// (one rounding): out[i * k + j] = X(i, j) * pre[i].  A column-major X goes
+// through a 32 x 64 LDS tile so that both the reads (down columns) and the
+// writes (along rows) are consecutive.
+template <typename T>
+__global__ void __launch_bounds__(256) stage_dense(const T *X, int64_t ldx, int xcol, const double *pre,
+                                                    uint64_t n, uint32_t k, T *out)
+{
+    __shared__ T tile[32][65];
+    const uint64_t r0 = (uint64_t)blockIdx.x * 64;
+    const uint32_t c0 = blockIdx.y * 32;
+    const uint32_t tid = threadIdx.x;
+    auto put = [&](uint64_t r, uint32_t c, T v) {
+        if constexpr (std::is_floating_point<T>::value)
+            if (pre) v = (T)((double)v * pre[r]);
+        out[r * k + c] = v;
+    };
+    if (xcol) {
+        for (uint32_t cc = tid / 64; cc < 32; cc += 4) {
+            const uint64_t r = r0 + tid % 64;
+            const uint32_t c = c0 + cc;
+            if (r < n && c < k) tile[cc][tid % 64] = X[c * (uint64_t)ldx + r];
+        }
+        __syncthreads();
+        for (uint32_t rr = tid / 32; rr < 64; rr += 8) {
+            const uint64_t r = r0 + rr;
+            const uint32_t c = c0 + tid % 32;
+            if (r < n && c < k) put(r, c, tile[tid % 32][rr]);
+        }
+    } else {
+        for (uint32_t i = tid; i < 64 * 32; i += 256) {
+            const uint64_t r = r0 + i / 32;
+            const uint32_t c = c0 + i % 32;
+            if (r < n && c < k) put(r, c, X[r * (uint64_t)ldx + c]);
+        }
+    }
+}
+
+fmx_status stage_dense_rows(fmx_ctx *ctx, fmx_dtype dt, uint64_t n, int k, const void *X, int64_t ldx, bool xcol,
+                            const double *pre, void *out, hipStream_t stream)
+{
+    if (n == 0) return FMX_OK;
+    const dim3 grid((unsigned)((n + 63) / 64), (unsigned)((k + 31) / 32));
+    switch (dt) {
+    case FMX_I32:
+        hipLaunchKernelGGL(stage_dense<int32_t>, grid, dim3(256), 0, stream, (const int32_t *)X, ldx, (int)xcol,
+                           nullptr, n, (uint32_t)k, (int32_t *)out);
+        break;
+    case FMX_I64:
+    case FMX_U64:
+        hipLaunchKernelGGL(stage_dense<int64_t>, grid, dim3(256), 0, stream, (const int64_t *)X, ldx, (int)xcol,
+                           nullptr, n, (uint32_t)k, (int64_t *)out);
+        break;
+    case FMX_F32:
+        hipLaunchKernelGGL(stage_dense<float>, grid, dim3(256), 0, stream, (const float *)X, ldx, (int)xcol, pre,
+                           n, (uint32_t)k, (float *)out);
+        break;
+    case FMX_F64:
+        hipLaunchKernelGGL(stage_dense<double>, grid, dim3(256), 0, stream, (const double *)X, ldx, (int)xcol, pre,
+                           n, (uint32_t)k, (double *)out);
+        break;
+    default: return fail(ctx, FMX_ETYPE, "wrong matrix type in spmm");
+    }
+    FMX_HIP(ctx, hipGetLastError());
+    return FMX_OK;
+}
+
 fmx_status launch_spmm(fmx_ctx *ctx, const fmx_spm *spm, const Plan *plan, fmx_dtype dt,
                        int k, const void *X, int64_t ldx, void *Y, int64_t ldy,
                        void *partial, hipStream_t stream, const uint16_t *img,
-                       uint32_t ib, uint32_t ie, bool reduce, const double *rscale)
+                       uint32_t ib, uint32_t ie, bool reduce, const double *rscale, uint32_t flags)
 {
     if (rscale && dt != FMX_F32 && dt != FMX_F64) return fail(ctx, FMX_ETYPE, "row scaling needs FP data");
     switch (dt) {
-    case FMX_I32: return launch_t<int32_t>(ctx, spm, plan, k, X, ldx, Y, ldy, partial, stream, img, ib, ie, reduce, rscale);
-    case FMX_I64: return launch_t<int64_t>(ctx, spm, plan, k, X, ldx, Y, ldy, partial, stream, img, ib, ie, reduce, rscale);
-    case FMX_F32: return launch_t<float>(ctx, spm, plan, k, X, ldx, Y, ldy, partial, stream, img, ib, ie, reduce, rscale);
-    case FMX_F64: return launch_t<double>(ctx, spm, plan, k, X, ldx, Y, ldy, partial, stream, img, ib, ie, reduce, rscale);
+    case FMX_I32: return launch_t<int32_t>(ctx, spm, plan, k, X, ldx, Y, ldy, partial, stream, img, ib, ie, reduce, rscale, flags);
+    // size_t sums wrap exactly as long sums do: one kernel serves both
+    case FMX_I64:
+    case FMX_U64: return launch_t<int64_t>(ctx, spm, plan, k, X, ldx, Y, ldy, partial, stream, img, ib, ie, reduce, rscale, flags);
+    case FMX_F32: return launch_t<float>(ctx, spm, plan, k, X, ldx, Y, ldy, partial, stream, img, ib, ie, reduce, rscale, flags);
+    case FMX_F64: return launch_t<double>(ctx, spm, plan, k, X, ldx, Y, ldy, partial, stream, img, ib, ie, reduce, rscale, flags);
     default: return fail(ctx, FMX_ETYPE, "wrong matrix type in spmm");
     }
 }

@@ -19,8 +19,9 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libfmx.so")
 
 OK, EINVAL, ETYPE, EDIM, EFORMAT, ENOMEM, EHIP, EUNSUPPORTED = range(8)
-I32, I64, F32, F64, BF16 = range(5)
+I32, I64, F32, F64, BF16, U64 = range(6)
 BUILD_DEDUP, BUILD_KEEP_SELF, BUILD_SYMMETRIZE = 1, 2, 4
+SPMM_SEQ, SPMM_X_COLMAJOR, SPMM_Y_COLMAJOR = 1, 2, 4
 
 _vp = ctypes.c_void_p
 _u64 = ctypes.c_uint64
@@ -85,6 +86,10 @@ SIGNATURES = {
     "fmx_spm_open_index": (ctypes.c_int, [_vp, ctypes.c_char_p, _vp, _vp, ctypes.c_size_t, _u64,
                                           ctypes.POINTER(_vp)]),
     "fmx_spmm": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int, _vp, _i64, _vp, _i64, _vp]),
+    "fmx_spmm_ex": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int, _vp, _i64, _vp, _i64, ctypes.c_uint32,
+                                   _vp]),
+    "fmx_spmm_scaled": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, _i64, _vp, _i64,
+                                       ctypes.c_uint32, _vp]),
     "fmx_spmm_alg_bytes": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int,
                                           ctypes.POINTER(_u64)]),
     "fmx_gram": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _u64, ctypes.c_int, ctypes.c_int,
@@ -146,7 +151,7 @@ def _torch():
 def dtype_code(t):
     torch = _torch()
     m = {torch.int32: I32, torch.int64: I64, torch.float32: F32, torch.float64: F64,
-         torch.bfloat16: BF16}
+         torch.bfloat16: BF16, torch.uint64: U64}
     if t not in m:
         raise FmxError(ETYPE, "unsupported dtype %s" % t)
     return m[t]
@@ -325,9 +330,7 @@ class SparseMatrix:
         return out.value
 
     # -- SpMM -------------------------------------------------------------------
-    def multiply_async(self, X, Y, stream=None):
-        """Enqueues Y = A*X on `stream` (default: torch's current stream)."""
-        torch = _torch()
+    def _operands(self, X, Y):
         if X.dim() == 1:
             X = X.view(-1, 1)
         if Y.dim() == 1:
@@ -339,13 +342,51 @@ class SparseMatrix:
         if X.shape[0] != self.info.ncols or Y.shape[0] != self.info.nrows or X.shape[1] != Y.shape[1]:
             raise FmxError(EDIM, "dimension mismatch: A is %dx%d, X %s, Y %s"
                            % (self.info.nrows, self.info.ncols, tuple(X.shape), tuple(Y.shape)))
-        if X.stride(1) != 1 or Y.stride(1) != 1:
-            raise FmxError(EINVAL, "dense operands must be row-major")
+        ldx, xcol = dense_layout(X, "X")
+        ldy, ycol = dense_layout(Y, "Y")
+        flags = (SPMM_X_COLMAJOR if xcol else 0) | (SPMM_Y_COLMAJOR if ycol else 0)
+        return X, Y, ldx, ldy, flags
+
+    def multiply_async(self, X, Y, stream=None, seq=False):
+        """Enqueues Y = A*X on `stream` (default: torch's current stream).
+        X and Y may each be row-major or column-major (any leading
+        dimension); seq=True sums floating-point rows in the reference's
+        sequential order (FMX_SPMM_SEQ: bit-identical to the reference's
+        CPU SpMM)."""
+        X, Y, ldx, ldy, flags = self._operands(X, Y)
         k = X.shape[1]
         sp = ctypes.c_void_p(stream) if stream is not None else self.ctx.stream_ptr()
-        self.ctx.check(lib().fmx_spmm(self.ctx.handle, self.handle, dtype_code(X.dtype), k,
-                                      X.data_ptr(), max(X.stride(0), k), Y.data_ptr(),
-                                      max(Y.stride(0), k), sp), "fmx_spmm")
+        if flags == 0 and not seq:
+            self.ctx.check(lib().fmx_spmm(self.ctx.handle, self.handle, dtype_code(X.dtype), k,
+                                          X.data_ptr(), ldx, Y.data_ptr(), ldy, sp), "fmx_spmm")
+        else:
+            self.ctx.check(lib().fmx_spmm_ex(self.ctx.handle, self.handle, dtype_code(X.dtype), k, X.data_ptr(),
+                                             ldx, Y.data_ptr(), ldy, flags | (SPMM_SEQ if seq else 0), sp),
+                           "fmx_spmm_ex")
+        return Y
+
+    def multiply_scaled_async(self, X, Y, pre=None, post=None, seq=False, stream=None):
+        """Y = diag(post) A diag(pre) X (fmx_spmm_scaled): the SpMM of a
+        custom task creator with row scalings, as the reference's
+        normalised-adjacency operator runs it.  pre (ncols) and post (nrows)
+        are FP64 device vectors or None; an FP32-weighted matrix also
+        multiplies FP64 data (spmm_creator<double, float>)."""
+        torch = _torch()
+        X, Y, ldx, ldy, flags = self._operands(X, Y)
+        vec = []
+        for v, n, what in ((pre, self.info.ncols, "pre"), (post, self.info.nrows, "post")):
+            if v is None:
+                vec.append(None)
+                continue
+            if v.dtype != torch.float64 or v.device.type != "cuda" or v.numel() != n:
+                raise FmxError(EINVAL, "%s must be an FP64 device vector of %d values" % (what, n))
+            vec.append(v.contiguous())
+        sp = ctypes.c_void_p(stream) if stream is not None else self.ctx.stream_ptr()
+        self.ctx.check(lib().fmx_spmm_scaled(self.ctx.handle, self.handle, dtype_code(X.dtype), X.shape[1],
+                                             vec[0].data_ptr() if vec[0] is not None else None,
+                                             vec[1].data_ptr() if vec[1] is not None else None,
+                                             X.data_ptr(), ldx, Y.data_ptr(), ldy,
+                                             flags | (SPMM_SEQ if seq else 0), sp), "fmx_spmm_scaled")
         return Y
 
     def multiply_into(self, X, Y):
@@ -376,7 +417,7 @@ class SparseMatrix:
         if X.ndim == 1:
             X = X.reshape(-1, 1)
         dt = {np.dtype(np.int32): I32, np.dtype(np.int64): I64, np.dtype(np.float32): F32,
-              np.dtype(np.float64): F64}.get(X.dtype)
+              np.dtype(np.float64): F64, np.dtype(np.uint64): U64}.get(X.dtype)
         if dt is None:
             raise FmxError(ETYPE, "wrong matrix type in spmm creator")
         if X.shape[0] != self.info.ncols:
@@ -403,10 +444,28 @@ class SparseMatrix:
 GRAM_F64, GRAM_BF16 = 0, 1
 
 
+def dense_layout(t, what="operand"):
+    """(leading dimension, column-major?) of a 2-D tensor whose rows (or
+    columns) do not overlap: row-major with stride(0) >= columns, or
+    column-major with stride(1) >= rows.  Expanded or overlapping views are
+    refused before anything is launched."""
+    if t.dim() != 2:
+        raise FmxError(EINVAL, "%s must be a 2-D tensor" % what)
+    r, c = t.shape
+    s0, s1 = t.stride()
+    if (c == 1 or s1 == 1) and (r <= 1 or s0 >= c):
+        return (s0 if r > 1 else c), False
+    if (r == 1 or s0 == 1) and (c <= 1 or s1 >= r):
+        return (s1 if c > 1 else r), True
+    raise FmxError(EINVAL, "%s (shape %s, strides %s) is neither row-major nor column-major with "
+                   "non-overlapping rows" % (what, tuple(t.shape), t.stride()))
+
+
 def _ld(t):
-    if t.dim() != 2 or t.stride(1) != 1:
+    ld, col = dense_layout(t)
+    if col and t.shape[1] > 1:
         raise FmxError(EINVAL, "dense operands must be 2-D row-major device tensors")
-    return max(t.stride(0), t.shape[1])
+    return ld
 
 
 def gram(A, B, mode=GRAM_F64, ctx=None):

@@ -130,24 +130,33 @@ public:
     size_t cal_num_block_rows(size_t tot) const { return (tot + get_num_rows() - 1) / get_num_rows(); }
 };
 
-/* A row-major dense matrix in HBM (the in-memory dense_matrix of
- * matrix/dense_matrix.h as the SpMM sees it). */
+/* matrix_layout_t (matrix/generic_type.h:36-43) for dense matrices. */
+enum matrix_layout_t { L_COL = 0, L_ROW = 1 };
+
+/* A dense matrix as the SpMM sees it (matrix/dense_matrix.h): row-major
+ * (L_ROW, element (i, j) at data[i*ld + j]) or column-major (L_COL, at
+ * data[j*ld + i]), in HBM, or -- for operands larger than the HBM given to
+ * them, the reference's external-memory dense matrices -- in pinned host
+ * memory (is_in_mem() false, row-major). */
 class dense_matrix {
     size_t nrow_, ncol_;
     int64_t ld_;
     fmx_dtype type_;
     void *data_;
     bool owned_;
+    matrix_layout_t layout_;
+    bool host_;
 
-    dense_matrix(size_t nrow, size_t ncol, int64_t ld, fmx_dtype t, void *d, bool owned)
-        : nrow_(nrow), ncol_(ncol), ld_(ld), type_(t), data_(d), owned_(owned) {}
+    dense_matrix(size_t nrow, size_t ncol, int64_t ld, fmx_dtype t, void *d, bool owned, matrix_layout_t l,
+                 bool host)
+        : nrow_(nrow), ncol_(ncol), ld_(ld), type_(t), data_(d), owned_(owned), layout_(l), host_(host) {}
 
 public:
     typedef std::shared_ptr<dense_matrix> ptr;
 
     static size_t elem_size(fmx_dtype t) { return (t == FMX_I32 || t == FMX_F32) ? 4 : 8; }
 
-    static ptr create(size_t nrow, size_t ncol, fmx_dtype t)
+    static ptr create(size_t nrow, size_t ncol, fmx_dtype t, matrix_layout_t layout = L_ROW)
     {
         void *d = nullptr;
         size_t bytes = nrow * ncol * elem_size(t);
@@ -155,12 +164,25 @@ public:
             detail::log_error("dense_matrix::create: out of device memory");
             return ptr();
         }
-        return ptr(new dense_matrix(nrow, ncol, (int64_t)ncol, t, d, true));
+        const int64_t ld = (int64_t)std::max<size_t>(1, layout == L_ROW ? ncol : nrow);
+        return ptr(new dense_matrix(nrow, ncol, ld, t, d, true, layout, false));
+    }
+    /* A row-major matrix in pinned host memory: the SpMM streams it through
+     * HBM in column panels (multiply(dense_matrix::ptr, mem_size)). */
+    static ptr create_host(size_t nrow, size_t ncol, fmx_dtype t)
+    {
+        void *d = nullptr;
+        size_t bytes = nrow * ncol * elem_size(t);
+        if (hipHostMalloc(&d, bytes ? bytes : 1, hipHostMallocDefault) != hipSuccess) {
+            detail::log_error("dense_matrix::create_host: out of pinned host memory");
+            return ptr();
+        }
+        return ptr(new dense_matrix(nrow, ncol, (int64_t)std::max<size_t>(1, ncol), t, d, true, L_ROW, true));
     }
     /* Wrap caller-owned device memory (not freed). */
-    static ptr wrap(void *dev, size_t nrow, size_t ncol, int64_t ld, fmx_dtype t)
+    static ptr wrap(void *dev, size_t nrow, size_t ncol, int64_t ld, fmx_dtype t, matrix_layout_t layout = L_ROW)
     {
-        return ptr(new dense_matrix(nrow, ncol, ld, t, dev, false));
+        return ptr(new dense_matrix(nrow, ncol, ld, t, dev, false, layout, false));
     }
     static ptr create_from_host(const void *host, size_t nrow, size_t ncol, fmx_dtype t)
     {
@@ -171,20 +193,80 @@ public:
     }
     ~dense_matrix()
     {
-        if (owned_ && data_) (void)hipFree(data_);
+        if (owned_ && data_) (void)(host_ ? hipHostFree(data_) : hipFree(data_));
     }
+    /* Copies the matrix to `host` as a row-major nrow x ncol array. */
     bool copy_to_host(void *host) const
     {
-        return hipMemcpy2D(host, ncol_ * elem_size(type_), data_, ld_ * elem_size(type_),
-                           ncol_ * elem_size(type_), nrow_, hipMemcpyDeviceToHost) == hipSuccess;
+        const size_t es = elem_size(type_);
+        if (host_) {
+            for (size_t i = 0; i < nrow_; i++)
+                memcpy((char *)host + i * ncol_ * es, (const char *)data_ + i * ld_ * es, ncol_ * es);
+            return true;
+        }
+        if (layout_ == L_ROW)
+            return hipMemcpy2D(host, ncol_ * es, data_, ld_ * es, ncol_ * es, nrow_, hipMemcpyDeviceToHost) ==
+                   hipSuccess;
+        std::vector<char> col(nrow_ * ncol_ * es);
+        if (hipMemcpy2D(col.data(), nrow_ * es, data_, ld_ * es, nrow_ * es, ncol_, hipMemcpyDeviceToHost) !=
+            hipSuccess)
+            return false;
+        for (size_t j = 0; j < ncol_; j++)
+            for (size_t i = 0; i < nrow_; i++)
+                memcpy((char *)host + (i * ncol_ + j) * es, col.data() + (j * nrow_ + i) * es, es);
+        return true;
     }
     size_t get_num_rows() const { return nrow_; }
     size_t get_num_cols() const { return ncol_; }
     int64_t get_ld() const { return ld_; }
     fmx_dtype get_type() const { return type_; }
+    matrix_layout_t store_layout() const { return layout_; }
+    bool is_in_mem() const { return !host_; }
     void *get_raw_arr() { return data_; }
     const void *get_raw_arr() const { return data_; }
 };
+
+namespace detail {
+/* A custom SpMM task creator (task_creator, matrix/sparse_matrix.h:52-62)
+ * as the reference's operators build them (spmm_creator, :377-436,656-693,
+ * with scale_rows around it, matrix/test/test-eigen.cpp:112-127): optional
+ * FP64 row scales of the input (pre, ncols values) and of the output (post,
+ * nrows values), both device arrays; seq sums in the reference's sequential
+ * order (FMX_SPMM_SEQ).  An FP64 input over an FP32-weighted matrix is
+ * spmm_creator<double, float>. */
+class spmm_creator {
+    const double *pre_ = nullptr, *post_ = nullptr;
+    bool seq_ = false;
+
+public:
+    typedef std::shared_ptr<spmm_creator> ptr;
+    static ptr create(const double *pre = nullptr, const double *post = nullptr, bool seq = false)
+    {
+        ptr p(new spmm_creator());
+        p->pre_ = pre;
+        p->post_ = post;
+        p->seq_ = seq;
+        return p;
+    }
+    const double *pre() const { return pre_; }
+    const double *post() const { return post_; }
+    bool seq() const { return seq_; }
+};
+}  // namespace detail
+
+/* Global SpMM options (the role matrix_config plays for the reference,
+ * matrix/matrix_config.h:81-98): seq_fp_sum makes every multiply sum
+ * floating-point rows in the reference's sequential order, bit-identical
+ * to its CPU SpMM (FMX_SPMM_SEQ); the default sums FP32 rows in FP64 and
+ * rounds once. */
+struct matrix_config {
+    bool seq_fp_sum = false;
+};
+inline matrix_config &matrix_conf()
+{
+    static matrix_config conf;
+    return conf;
+}
 
 /* I/O failure while loading (the reference throws safs::io_exception,
  * matrix/sparse_matrix_format.cpp:262-276). */
@@ -441,33 +523,74 @@ public:
         return transpose_;
     }
 
-    /* multiply(in, out) (matrix/sparse_matrix.h:636-637): out must be
-     * allocated; false on a type or dimension mismatch. */
-    bool multiply(const dense_matrix &in, dense_matrix &out) const
+    /* multiply(in, out, creator) (matrix/sparse_matrix.cpp:709-741): in and
+     * out in HBM, each row- or column-major (a column-major input is
+     * converted first, as the reference does, :720-736); false on a type or
+     * dimension mismatch. */
+    bool multiply(const dense_matrix &in, dense_matrix &out, detail::spmm_creator::ptr create) const
     {
-        if (in.get_num_rows() != get_num_cols() || out.get_num_rows() != get_num_rows() ||
-            in.get_num_cols() != out.get_num_cols()) {
-            detail::log_error("the matrix size doesn't match");
+        if (in.get_num_rows() != get_num_cols() || in.get_num_cols() != out.get_num_cols() ||
+            out.get_num_rows() != get_num_rows()) {
+            detail::log_error("the input and output matrix have incompatible dimensions");
             return false;
         }
         if (in.get_type() != out.get_type()) {
-            detail::log_error("wrong matrix type in spmm creator");
+            detail::log_error("input and output matrices need to have the same type");
             return false;
         }
-        if (!detail::check(fmx_spmm(detail::global_ctx(), spm_.get(), in.get_type(), (int)in.get_num_cols(),
-                                    in.get_raw_arr(), in.get_ld(), out.get_raw_arr(), out.get_ld(), nullptr),
-                           "sparse_matrix::multiply"))
+        if (!in.is_in_mem() || !out.is_in_mem()) {
+            detail::log_error("multiply(in, out) needs operands in HBM");
             return false;
+        }
+        uint32_t flags = (in.store_layout() == L_COL ? FMX_SPMM_X_COLMAJOR : 0u) |
+                         (out.store_layout() == L_COL ? FMX_SPMM_Y_COLMAJOR : 0u);
+        fmx_status st;
+        if (create) {
+            if (create->seq() || matrix_conf().seq_fp_sum) flags |= FMX_SPMM_SEQ;
+            st = fmx_spmm_scaled(detail::global_ctx(), spm_.get(), in.get_type(), (int)in.get_num_cols(),
+                                 create->pre(), create->post(), in.get_raw_arr(), in.get_ld(), out.get_raw_arr(),
+                                 out.get_ld(), flags, nullptr);
+        } else {
+            if (matrix_conf().seq_fp_sum) flags |= FMX_SPMM_SEQ;
+            st = fmx_spmm_ex(detail::global_ctx(), spm_.get(), in.get_type(), (int)in.get_num_cols(),
+                             in.get_raw_arr(), in.get_ld(), out.get_raw_arr(), out.get_ld(), flags, nullptr);
+        }
+        if (!detail::check(st, "sparse_matrix::multiply")) return false;
         return hipDeviceSynchronize() == hipSuccess;
     }
 
-    /* multiply(dense_matrix::ptr, mem_size) (matrix/sparse_matrix.h:643-644):
-     * returns a new row-major Y = A * X, or null. */
+    /* multiply(in, out) (matrix/sparse_matrix.h:636-637, .cpp:751-771): out
+     * must be allocated; false on a type or dimension mismatch. */
+    bool multiply(const dense_matrix &in, dense_matrix &out) const
+    {
+        return multiply(in, out, detail::spmm_creator::ptr());
+    }
+
+    /* multiply(dense_matrix::ptr, mem_size) (matrix/sparse_matrix.h:643-644,
+     * .cpp:773-831): returns a new row-major Y = A * X, or null.  An X in
+     * HBM gives a Y in HBM.  An X in host memory (create_host) gives a Y in
+     * host memory and goes through HBM in column panels that fit mem_size
+     * bytes (the reference's panel branch, :796-830; "The memory is too
+     * small for the SpMM" when not even one column fits). */
     dense_matrix::ptr multiply(dense_matrix::ptr right,
                                size_t mem_size = std::numeric_limits<size_t>::max()) const
     {
-        (void)mem_size;  // everything is resident in HBM
         if (!right) return dense_matrix::ptr();
+        if (!right->is_in_mem()) {
+            if (right->get_num_rows() != get_num_cols()) {
+                detail::log_error("the matrix size doesn't match");
+                return dense_matrix::ptr();
+            }
+            dense_matrix::ptr out = dense_matrix::create_host(get_num_rows(), right->get_num_cols(),
+                                                              right->get_type());
+            if (!out ||
+                !detail::check(fmx_spmm_host(detail::global_ctx(), spm_.get(), right->get_type(),
+                                             (int)right->get_num_cols(), right->get_raw_arr(), right->get_ld(),
+                                             out->get_raw_arr(), out->get_ld(), (uint64_t)mem_size, nullptr),
+                               "sparse_matrix::multiply"))
+                return dense_matrix::ptr();
+            return out;
+        }
         dense_matrix::ptr out = dense_matrix::create(get_num_rows(), right->get_num_cols(), right->get_type());
         if (!out || !multiply(*right, *out)) return dense_matrix::ptr();
         return out;

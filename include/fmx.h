@@ -42,7 +42,9 @@ typedef enum {
     FMX_I64 = 1,  /* long     (P_LONG)     */
     FMX_F32 = 2,  /* float    (P_FLOAT)    */
     FMX_F64 = 3,  /* double   (P_DOUBLE)   */
-    FMX_BF16 = 4  /* bfloat16 (dense-op inputs only) */
+    FMX_BF16 = 4, /* bfloat16 (dense-op inputs only) */
+    FMX_U64 = 5   /* size_t   (P_ULONG), SpMM only: spmm_creator<size_t, size_t>,
+                     matrix/sparse_matrix.cpp:568-569 */
 } fmx_dtype;
 
 /* Edge normalisation flags for fmx_spm_build. */
@@ -170,6 +172,53 @@ fmx_status fmx_spm_dump(const fmx_spm *spm, const char *mat_file, const char *id
 fmx_status fmx_spmm(fmx_ctx *ctx, const fmx_spm *spm, fmx_dtype dtype, int k,
                     const void *X, int64_t ldx, void *Y, int64_t ldy,
                     void *stream);
+
+/* fmx_spmm with options (flags, OR-ed):
+ *   FMX_SPMM_SEQ: floating-point sums in the reference's own order.  Each
+ *     output element is the left-to-right sum, one rounding per addition, of
+ *     its row's products in ascending column order starting from 0 -- what
+ *     row_part_func / coo_func compute (`dest_row[j] += src_row[j] * data`,
+ *     matrix/sparse_matrix.h:219-331) with the default sequential tile order
+ *     (seq_exec_order, :161-176), for any tile shape.  FP32 and FP64 results
+ *     are then bit-identical to the reference's in-memory CPU SpMM.  Without
+ *     it (the default), FP32 rows are summed in FP64 and rounded once (the
+ *     correctly rounded sum, which differs from the reference's sequential
+ *     FP32 sum by that sum's own rounding error, about 1e-5 relative on
+ *     power-law hub rows) and FP64 rows in a fixed order of the kernel's own;
+ *     both are deterministic.  Integer results are exact either way.  The
+ *     sequential mode does not split power-law rows over several waves, so
+ *     it is slower on skewed matrices.
+ *   FMX_SPMM_X_COLMAJOR: X is column-major, element (i, j) at X[j*ldx + i]
+ *     (ldx >= ncols); it is converted to row-major in a context workspace
+ *     first, as the reference converts a non-row-major input
+ *     (matrix/sparse_matrix.cpp:720-736).
+ *   FMX_SPMM_Y_COLMAJOR: Y is column-major, element (i, j) at Y[j*ldy + i]
+ *     (ldy >= nrows); written directly (an L_COL output store, as the
+ *     reference's eigen operators pass, matrix/test/test-eigen.cpp:115-118). */
+#define FMX_SPMM_SEQ        1u
+#define FMX_SPMM_X_COLMAJOR 2u
+#define FMX_SPMM_Y_COLMAJOR 4u
+fmx_status fmx_spmm_ex(fmx_ctx *ctx, const fmx_spm *spm, fmx_dtype dtype, int k,
+                       const void *X, int64_t ldx, void *Y, int64_t ldy,
+                       uint32_t flags, void *stream);
+
+/* Y = diag(post) * A * diag(pre) * X: the SpMM of a custom task creator,
+ * sparse_matrix::multiply(in, out, creator) (matrix/sparse_matrix.cpp:
+ * 709-741, task_creator matrix/sparse_matrix.h:52-62) as the reference's
+ * normalised-adjacency operator drives it (scale_rows by D^-1/2 on both
+ * sides around spmm_creator<double, float>, matrix/test/test-eigen.cpp:
+ * 112-127).  pre (ncols) and post (nrows) are DEVICE FP64 vectors or NULL
+ * for none; dtype F32 or F64.  X * pre[i] is rounded to dtype once (staged
+ * in a context workspace); post[r] multiplies row r's unrounded sum, which
+ * is then rounded once (without FMX_SPMM_SEQ; with it, the sequential sum is
+ * rounded, scaled, and rounded again).  A weighted image with FP32 values
+ * multiplies FP64 dense data as spmm_creator<double, float> does: each
+ * product is the FP64 product of the FP64 element and the FP32 value.
+ * flags as for fmx_spmm_ex. */
+fmx_status fmx_spmm_scaled(fmx_ctx *ctx, const fmx_spm *spm, fmx_dtype dtype, int k,
+                           const double *pre, const double *post,
+                           const void *X, int64_t ldx, void *Y, int64_t ldy,
+                           uint32_t flags, void *stream);
 
 /* Y = A * X with X (ncols x k) and Y (nrows x k) in HOST memory, for dense
  * operands larger than the HBM set aside for them: the column-panel branch

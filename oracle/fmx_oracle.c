@@ -735,6 +735,7 @@ static int elem_size(int dtype)
         switch (prim) {                                                          \
         case 4: { int32_t v; memcpy(&v, vals + 4 * i, 4); return (T)v; }          \
         case 6: { int64_t v; memcpy(&v, vals + 8 * i, 8); return (T)v; }          \
+        case 7: { uint64_t v; memcpy(&v, vals + 8 * i, 8); return (T)v; }         \
         case 8: { float v; memcpy(&v, vals + 4 * i, 4); return (T)v; }            \
         case 9: { double v; memcpy(&v, vals + 8 * i, 8); return (T)v; }           \
         default: return (T)1;                                                    \

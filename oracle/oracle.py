@@ -16,7 +16,10 @@ _lib = None
 
 I32, I64, F32, F64 = 0, 1, 2, 3
 DEDUP, KEEP_SELF, SYMMETRIZE = 1, 2, 4
-_NP2DT = {np.dtype(np.int32): I32, np.dtype(np.int64): I64,
+# size_t operands (P_ULONG) reuse the int64 path: two's-complement sums and
+# products modulo 2^64 are the same bits (the C restatement keeps the test
+# values below the int64 range, where no signed overflow occurs)
+_NP2DT = {np.dtype(np.int32): I32, np.dtype(np.int64): I64, np.dtype(np.uint64): I64,
           np.dtype(np.float32): F32, np.dtype(np.float64): F64}
 
 _u8p = ctypes.POINTER(ctypes.c_uint8)
@@ -150,7 +153,8 @@ def build_2d(src, dst, n=0, log2h=15, log2w=15, flags=DEDUP):
     return Image(data, o, n_out.value, nnz.value)
 
 
-_PRIM = {np.dtype(np.int32): 4, np.dtype(np.int64): 6, np.dtype(np.float32): 8, np.dtype(np.float64): 9}
+_PRIM = {np.dtype(np.int32): 4, np.dtype(np.int64): 6, np.dtype(np.uint64): 7, np.dtype(np.float32): 8,
+         np.dtype(np.float64): 9}
 
 
 def build_2d_w(src, dst, vals, n=0, log2h=15, log2w=15, flags=DEDUP):

@@ -146,6 +146,53 @@ int main()
             return 1;
         }
     }
+    // column-major input and output (L_COL; test-eigen.cpp:115-118), an input
+    // in host memory streamed through HBM in column panels (the panel branch
+    // of multiply(dense_matrix::ptr, mem_size), sparse_matrix.cpp:796-830),
+    // and a creator with row scales (multiply(in, out, creator),
+    // sparse_matrix.cpp:709-741): all equal the row-major product exactly on
+    // integer-valued FP64 data
+    {
+        const size_t k = 3;
+        std::vector<double> xr(n * k), xc(n * k);
+        for (uint32_t i = 0; i < n; i++)
+            for (size_t j = 0; j < k; j++) {
+                const double v = (double)((i * 31 + j * 7) % 97);
+                xr[i * k + j] = v;
+                xc[j * n + i] = v;
+            }
+        fm::dense_matrix::ptr Xr = fm::dense_matrix::create_from_host(xr.data(), n, k, FMX_F64);
+        fm::dense_matrix::ptr Xc = fm::dense_matrix::create(n, k, FMX_F64, fm::L_COL);
+        fm::dense_matrix::ptr Yc = fm::dense_matrix::create(n, k, FMX_F64, fm::L_COL);
+        if (!Xr || !Xc || !Yc || Xc->store_layout() != fm::L_COL) return 1;
+        hipMemcpy(Xc->get_raw_arr(), xc.data(), n * k * 8, hipMemcpyHostToDevice);
+        fm::dense_matrix::ptr Yr = A->multiply(Xr);
+        if (!Yr || !A->multiply(*Xc, *Yc)) return 1;
+        std::vector<double> yr(n * k), yc(n * k), yh(n * k), ys(n * k);
+        Yr->copy_to_host(yr.data());
+        Yc->copy_to_host(yc.data());
+        if (yr != yc) return 1;
+        fm::dense_matrix::ptr Xh = fm::dense_matrix::create_host(n, k, FMX_F64);
+        if (!Xh || Xh->is_in_mem()) return 1;
+        memcpy(Xh->get_raw_arr(), xr.data(), n * k * 8);
+        fm::dense_matrix::ptr Yh = A->multiply(Xh, (size_t)2 * n * 8);  // one column per panel
+        if (!Yh || Yh->is_in_mem()) return 1;
+        Yh->copy_to_host(yh.data());
+        if (yh != yr) return 1;
+        if (A->multiply(Xh, 8)) return 1;  // the memory is too small for the SpMM
+        std::vector<double> half(n, 0.5), two(n, 2.0);
+        double *d_half, *d_two;
+        hipMalloc(&d_half, n * 8);
+        hipMalloc(&d_two, n * 8);
+        hipMemcpy(d_half, half.data(), n * 8, hipMemcpyHostToDevice);
+        hipMemcpy(d_two, two.data(), n * 8, hipMemcpyHostToDevice);
+        fm::dense_matrix::ptr Ys = fm::dense_matrix::create(n, k, FMX_F64);
+        if (!Ys || !A->multiply(*Xr, *Ys, fm::detail::spmm_creator::create(d_two, d_half, true))) return 1;
+        Ys->copy_to_host(ys.data());
+        if (ys != yr) return 1;  // 0.5 * A * (2 X) = A X exactly
+        hipFree(d_half);
+        hipFree(d_two);
+    }
     // error convention: a dimension mismatch is false + log, not an exception
     fm::dense_matrix::ptr bad = fm::dense_matrix::create(n + 1, 1, FMX_I64);
     if (A->multiply(*bad, *Y)) return 1;
