@@ -183,13 +183,20 @@ __global__ void __launch_bounds__(64) chol_inv_kernel(const double *G, int b, do
     if (tid == 0) info[0] = bad;
 }
 
-// H[0:rows, c0:c0+b] = C1 + C2 and the mirrored block, H is ld x ld.
+// H[0:rows, c0:c0+b] = C1 + C2 and the mirrored block, H is ld x ld.  In
+// the diagonal block (rows c0 .. c0+b-1) entry (i, j) and its mirror are
+// both set to the mean of C(i, j) and C(j, i), so that every H entry gets
+// one value whichever thread writes it last.
 __global__ void __launch_bounds__(256) h_update_kernel(double *H, int ld, int rows, int c0, int b,
                                                        const double *C1, const double *C2)
 {
     for (int e = threadIdx.x + blockIdx.x * 256; e < rows * b; e += gridDim.x * 256) {
         const int i = e / b, j = e - i * b;
-        const double v = C1[e] + (C2 ? C2[e] : 0.0);
+        double v = C1[e] + (C2 ? C2[e] : 0.0);
+        if (i >= c0 && i < c0 + b) {
+            const int t = (c0 + j) * b + (i - c0);  // C(j, i) of the diagonal block
+            v = 0.5 * (v + (C1[t] + (C2 ? C2[t] : 0.0)));
+        }
         H[i * ld + c0 + j] = v;
         H[(c0 + j) * ld + i] = v;
     }
