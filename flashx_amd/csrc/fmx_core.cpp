@@ -467,6 +467,8 @@ fmx_status spmm_streamed(fmx_ctx *ctx, fmx_spm *spm, Plan *plan, fmx_dtype dtype
 
 // Pinned host image (already holding h_pinned_len bytes) -> validated,
 // chunked streamed matrix.
+fmx_status alloc_pinned(fmx_ctx *ctx, fmx_spm *s, uint64_t len);
+
 fmx_status finish_streamed(fmx_ctx *ctx, fmx_spm *s, uint64_t chunk_bytes, bool whole_file)
 {
     fmx_status st = analyse_image(ctx, s);
@@ -474,6 +476,26 @@ fmx_status finish_streamed(fmx_ctx *ctx, fmx_spm *s, uint64_t chunk_bytes, bool 
     if (whole_file && s->bytes != s->h_pinned_len)
         return fail(ctx, FMX_EFORMAT, "matrix file is longer than its block rows");
     if (chunk_bytes < 4096) return fail(ctx, FMX_EINVAL, "chunk_bytes must be at least 4096");
+    if (const uint32_t h2 = work_rows_for(s)) {
+        // tiles taller than an SpMM band: the re-cut work image (in pinned
+        // host memory too) is what every SpMM streams, as a resident matrix
+        // runs on its re-cut image; the given image stays for downloads
+        std::vector<uint8_t> img;
+        std::vector<int64_t> offs;
+        retile_image(s, h2, img, offs);
+        std::unique_ptr<fmx_spm> w(new fmx_spm());
+        w->ctx = ctx;
+        st = alloc_pinned(ctx, w.get(), img.size());
+        if (st) return st;
+        memcpy(w->h_pinned, img.data(), img.size());
+        std::vector<uint8_t>().swap(img);
+        w->offs = std::move(offs);
+        st = finish_streamed(ctx, w.get(), chunk_bytes, false);  // its tiles are short: no further re-cut
+        if (st) return fail(ctx, FMX_EINVAL, "internal error: the re-cut work image: " + ctx->last_error);
+        if (w->nnz != s->nnz) return fail(ctx, FMX_EINVAL, "internal error: the re-cut work image lost non-zeros");
+        s->work = std::move(w);
+        return FMX_OK;
+    }
     // chunks of whole block rows, each at most chunk_bytes unless one block
     // row alone is larger
     s->chunk_br.clear();
@@ -726,7 +748,8 @@ fmx_status fmx_spm_info_get(const fmx_spm *spm, fmx_spm_info *info)
     info->tile_bytes = spm->bytes - kHeaderBytes;
     info->entry_size = spm->esz;
     info->ntiles = spm->ntiles;
-    info->stream_chunks = spm->streamed() ? spm->chunk_br.size() - 1 : 0;
+    const fmx_spm *run = spm->work ? spm->work.get() : spm;  // the image every SpMM reads
+    info->stream_chunks = spm->streamed() ? run->chunk_br.size() - 1 : 0;
     return FMX_OK;
 }
 

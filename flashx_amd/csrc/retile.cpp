@@ -6,14 +6,17 @@
 // reference's default 32768 x 32768 tiles, flash-graph/utils/fg2fm.cpp:
 // 155-156, against bands of 64-2048 rows), every band walks the whole block
 // row to pick out its own rows.  Such an image is therefore re-cut once, when
-// the matrix is created, into a work image of kWorkRows-row tiles with the
-// same tile width: every row part and single-entry row of a tile stays what
+// the matrix is created, into a work image of 64-row tiles (work_rows_for)
+// with the same tile width: every row part and single-entry row of a tile
+// stays what
 // it is (its columns, its values, SCSR or COO), and moves to the tile of its
 // new block row, in the same block column.  Row r of the matrix then
 // accumulates the same non-zeros in the same ascending-column order, so the
-// sequential mode is bit-identical on both images.  The image given by the
-// caller is kept unchanged: downloads and dumps return it, and the
-// algorithmic bytes are counted on it.
+// sequential mode is bit-identical on both images.  Resident and streamed
+// matrices are re-cut alike (a streamed one keeps its work image in pinned
+// host memory), so the two give the same bits in every mode.  The image
+// given by the caller is kept unchanged: downloads and dumps return it, and
+// the algorithmic bytes are counted on it.
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
@@ -189,7 +192,7 @@ uint32_t work_rows_for(const fmx_spm *s)
 {
     uint64_t h2 = 64;
     if (const char *e = getenv("FMX_WORK_ROWS")) h2 = strtoull(e, nullptr, 10);
-    if (h2 == 0 || (h2 & (h2 - 1)) || s->H <= h2 || s->streamed()) return 0;
+    if (h2 == 0 || (h2 & (h2 - 1)) || s->H <= h2) return 0;
     return (uint32_t)h2;
 }
 
