@@ -6,8 +6,14 @@
 namespace fmx {
 
 fmx_status ctx_scratch(fmx_ctx *ctx, size_t bytes, void **out);
+// Finds this device's copy of the fixed scratch of fmx_gram_noalloc (the
+// context's device must be current) and records it on the context.
+fmx_status noalloc_scratch_lookup(fmx_ctx *ctx);
+// fixed_ws: the partial sums go to this buffer of fixed_bytes (the block
+// count is capped to fit) instead of the context's grow-only workspace.
 fmx_status gram(fmx_ctx *ctx, fmx_dtype dt, int mode, uint64_t n, int ka, int kb, const void *A,
-                int64_t lda, const void *B, int64_t ldb, double *G, hipStream_t s);
+                int64_t lda, const void *B, int64_t ldb, double *G, hipStream_t s, void *fixed_ws = nullptr,
+                size_t fixed_bytes = 0);
 fmx_status gemm_tall(fmx_ctx *ctx, fmx_dtype dt, uint64_t n, int ka, int kc, double alpha,
                      const void *A, int64_t lda, const double *C, double beta, void *Y, int64_t ldy,
                      hipStream_t s);

@@ -663,6 +663,24 @@ uint32_t blocks_for(fmx_ctx *ctx, uint64_t n, uint64_t rows_min)
 
 }  // namespace
 
+namespace {
+// The fixed scratch of fmx_gram_noalloc: per-block partial sums of the Gram
+// kernels.  Every GPU the library runs on holds its own copy until the
+// process ends.  In host code the name is only a stand-in for the device
+// array: kernels get the address that noalloc_scratch_lookup finds.
+constexpr size_t kNoAllocBytes = size_t(4) << 20;
+__device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+}  // namespace
+
+fmx_status noalloc_scratch_lookup(fmx_ctx *ctx)
+{
+    void *p = nullptr;
+    FMX_HIP(ctx, hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    ctx->noalloc_ws = p;
+    ctx->noalloc_bytes = kNoAllocBytes;
+    return FMX_OK;
+}
+
 fmx_status ctx_scratch(fmx_ctx *ctx, size_t bytes, void **out)
 {
     if (bytes > ctx->ws_bytes) {
@@ -676,8 +694,23 @@ fmx_status ctx_scratch(fmx_ctx *ctx, size_t bytes, void **out)
     return FMX_OK;
 }
 
+// Per-block partial sums of the Gram kernels: the context's grow-only
+// workspace, or -- fixed != null -- a caller-given buffer of fixed_bytes,
+// in which case the block count is capped so that the partials fit.
+static fmx_status gram_ws(fmx_ctx *ctx, void *fixed, size_t fixed_bytes, size_t per_block, uint32_t &nblk,
+                          void **ws)
+{
+    if (!fixed) return ctx_scratch(ctx, (size_t)nblk * per_block, ws);
+    const size_t fit = fixed_bytes / per_block;
+    if (fit == 0) return fail(ctx, FMX_EDIM, "Gram output larger than the fixed scratch");
+    if (nblk > fit) nblk = (uint32_t)fit;
+    *ws = fixed;
+    return FMX_OK;
+}
+
 fmx_status gram(fmx_ctx *ctx, fmx_dtype dt, int mode, uint64_t n, int ka, int kb, const void *A,
-                int64_t lda, const void *B, int64_t ldb, double *G, hipStream_t s)
+                int64_t lda, const void *B, int64_t ldb, double *G, hipStream_t s, void *fixed_ws,
+                size_t fixed_bytes)
 {
     if (ka <= 0 || kb <= 0 || ka > 256 || kb > 256) return fail(ctx, FMX_EDIM, "gram widths must be in [1, 256]");
     if (lda < ka || ldb < kb) return fail(ctx, FMX_EDIM, "leading dimension too small");
@@ -686,11 +719,11 @@ fmx_status gram(fmx_ctx *ctx, fmx_dtype dt, int mode, uint64_t n, int ka, int kb
         if (dt != FMX_F32) return fail(ctx, FMX_ETYPE, "BF16 Gram takes FP32 operands");
         const int kap = (ka + 15) & ~15, kbp = (kb + 15) & ~15;
         if ((kap / 16) * (kbp / 16) > 64) return fail(ctx, FMX_EDIM, "BF16 Gram output too large");
-        const uint32_t nblk = blocks_for(ctx, n, 1024);
-        const uint64_t rpb = (n + nblk - 1) / nblk;
+        uint32_t nblk = blocks_for(ctx, n, 1024);
         void *ws;
-        fmx_status st = ctx_scratch(ctx, (size_t)nblk * nout * sizeof(float), &ws);
+        fmx_status st = gram_ws(ctx, fixed_ws, fixed_bytes, (size_t)nout * sizeof(float), nblk, &ws);
         if (st) return st;
+        const uint64_t rpb = (n + nblk - 1) / nblk;
         const size_t lds = (size_t)(kap + kbp) * kBPad * sizeof(unsigned short);
         static bool attr = false;
         if (!attr) {
@@ -733,15 +766,20 @@ fmx_status gram(fmx_ctx *ctx, fmx_dtype dt, int mode, uint64_t n, int ka, int kb
         (dt == FMX_F32 || dt == FMX_F64)) {
         // waves of at least 1024 rows (a multiple of 16), at most 8 per CU
         uint64_t waves = (n + 1023) / 1024;
-        const uint64_t cap = (uint64_t)ctx->num_cus * 8;  // more waves: more partials, no faster
+        uint64_t cap = (uint64_t)ctx->num_cus * 8;  // more waves: more partials, no faster
+        if (fixed_ws) {
+            const uint64_t fit = fixed_bytes / ((size_t)nout * sizeof(double));  // blocks whose partials fit
+            if (fit == 0) return fail(ctx, FMX_EDIM, "Gram output larger than the fixed scratch");
+            cap = std::min<uint64_t>(cap, fit * (kT / 64));
+        }
         if (waves > cap) waves = cap;
         if (waves < 1) waves = 1;
         const uint64_t rpw = ((n + waves - 1) / waves + 15) & ~15ull;
         waves = (n + rpw - 1) / rpw;
         if (waves < 1) waves = 1;
-        const uint32_t nblk = (uint32_t)((waves * 64 + kT - 1) / kT);  // waves past the rows add zeros
+        uint32_t nblk = (uint32_t)((waves * 64 + kT - 1) / kT);  // waves past the rows add zeros
         void *ws;
-        fmx_status st = ctx_scratch(ctx, (size_t)nblk * nout * sizeof(double), &ws);
+        fmx_status st = gram_ws(ctx, fixed_ws, fixed_bytes, (size_t)nout * sizeof(double), nblk, &ws);
         if (st) return st;
         const size_t glds = std::max(stage_lds, (size_t)2 * nout * sizeof(double));
         const int veca = ((uintptr_t)A % 16 == 0) && ((lda * es) % 16 == 0);
@@ -816,11 +854,11 @@ fmx_status gram(fmx_ctx *ctx, fmx_dtype dt, int mode, uint64_t n, int ka, int kb
         FMX_HIP(ctx, hipGetLastError());
         return FMX_OK;
     }
-    const uint32_t nblk = blocks_for(ctx, n, 512);
-    const uint64_t rpb = (n + nblk - 1) / nblk;
+    uint32_t nblk = blocks_for(ctx, n, 512);
     void *ws;
-    fmx_status st = ctx_scratch(ctx, (size_t)nblk * nout * sizeof(double), &ws);
+    fmx_status st = gram_ws(ctx, fixed_ws, fixed_bytes, (size_t)nout * sizeof(double), nblk, &ws);
     if (st) return st;
+    const uint64_t rpb = (n + nblk - 1) / nblk;
     const size_t lds = (size_t)kGRows * (ka + kb) * sizeof(double);
     dim3 grid(nblk, (nout + 4095) / 4096);
     if (dt == FMX_F32)
@@ -1042,6 +1080,16 @@ fmx_status fmx_gram(fmx_ctx *ctx, fmx_dtype dtype, int mode, uint64_t n, int ka,
     FMX_HIP(ctx, hipSetDevice(ctx->device));
     return gram(ctx, dtype, mode, n, ka, kb, A, lda, B, ldb, G,
                 (hipStream_t)stream);
+}
+
+fmx_status fmx_gram_noalloc(fmx_ctx *ctx, fmx_dtype dtype, int mode, uint64_t n, int ka, int kb, const void *A,
+                            int64_t lda, const void *B, int64_t ldb, double *G, void *stream)
+{
+    if (!ctx || (n && (!A || !B)) || !G) return fail(ctx, FMX_EINVAL, "null argument");
+    if (!ctx->noalloc_ws) return fail(ctx, FMX_EINVAL, "the context has no fixed scratch");
+    FMX_HIP(ctx, hipSetDevice(ctx->device));
+    return gram(ctx, dtype, mode, n, ka, kb, A, lda, B, ldb, G, (hipStream_t)stream, ctx->noalloc_ws,
+                ctx->noalloc_bytes);
 }
 
 fmx_status fmx_gemm_tall(fmx_ctx *ctx, fmx_dtype dtype, uint64_t n, int ka, int kc, double alpha,

@@ -8,6 +8,7 @@
 #include <cstring>
 #include <numeric>
 
+#include "dense.h"
 #include "fmx_internal.h"
 
 namespace fmx {
@@ -675,6 +676,11 @@ fmx_status fmx_ctx_create(int device, fmx_ctx **out)
     c->num_cus = prop.multiProcessorCount;
     c->lds_per_block = prop.sharedMemPerBlock;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return FMX_EHIP;
+    // the fixed scratch of fmx_gram_noalloc, looked up with the device current
+    if (noalloc_scratch_lookup(c.get()) != FMX_OK) {
+        (void)hipStreamDestroy(c->stream);
+        return FMX_EHIP;
+    }
     *out = c.release();
     return FMX_OK;
 }

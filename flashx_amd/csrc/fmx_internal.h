@@ -86,6 +86,8 @@ struct fmx_ctx {
     void *ws = nullptr;        // grow-only device workspace (dense-op partials)
     size_t ws_bytes = 0;
     fmx::Scratch xstage;       // grow-only: a dense operand restaged row-major / pre-scaled
+    void *noalloc_ws = nullptr;  // this device's copy of the fixed scratch (fmx_gram_noalloc)
+    size_t noalloc_bytes = 0;
 };
 
 struct fmx_spm {

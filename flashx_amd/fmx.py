@@ -94,6 +94,8 @@ SIGNATURES = {
                                           ctypes.POINTER(_u64)]),
     "fmx_gram": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _u64, ctypes.c_int, ctypes.c_int,
                                 _vp, _i64, _vp, _i64, _vp, _vp]),
+    "fmx_gram_noalloc": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _u64, ctypes.c_int, ctypes.c_int,
+                                        _vp, _i64, _vp, _i64, _vp, _vp]),
     "fmx_gemm_tall": (ctypes.c_int, [_vp, ctypes.c_int, _u64, ctypes.c_int, ctypes.c_int,
                                      ctypes.c_double, _vp, _i64, _vp, ctypes.c_double, _vp, _i64,
                                      _vp]),

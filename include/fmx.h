@@ -250,6 +250,20 @@ fmx_status fmx_gram(fmx_ctx *ctx, fmx_dtype dtype, int mode, uint64_t n, int ka,
                     const void *A, int64_t lda, const void *B, int64_t ldb, double *G,
                     void *stream);
 
+/* No counterpart in the reference.  fmx_gram for callers that cannot
+ * allocate device memory: code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc.  The per-block partial sums go to a fixed
+ * 4 MiB device array that the library holds on every GPU it runs on until
+ * the process ends, not to the context's grow-only workspace, and the block
+ * count is capped so that they fit (a ka * kb * 8-byte output larger than
+ * the array is FMX_EDIM; every ka, kb <= 256 fits).  Arguments, results and
+ * errors are fmx_gram's; no device memory is allocated or freed.  All calls
+ * on one GPU share that array: calls that overlap in time on one GPU (on
+ * different streams or contexts) race, so order them. */
+fmx_status fmx_gram_noalloc(fmx_ctx *ctx, fmx_dtype dtype, int mode, uint64_t n, int ka, int kb,
+                            const void *A, int64_t lda, const void *B, int64_t ldb, double *G,
+                            void *stream);
+
 /* Y = alpha * A * C + beta * Y with A n x ka, C ka x kc (FP64, device,
  * row-major), Y n x kc; FP64 accumulation.  Replaces MvTimesMatAddMv /
  * gemm_op::run (FM_MultiVector.h:389-433, block_dense_matrix.cpp:822-914).
