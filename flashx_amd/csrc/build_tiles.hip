@@ -21,6 +21,7 @@
 //      exclusive scans (fm_utils.cpp:701-712);
 //   5. every key writes its own u16 words; every tile writes its header and
 //      end marker (sparse_block_2d::finalize, sparse_matrix_format.cpp:119-126).
+#include <algorithm>
 #include <cstring>
 
 #include "devprim.h"
@@ -345,19 +346,21 @@ uint32_t grid_for(fmx_ctx *ctx, uint64_t n)
 
 }  // namespace
 
-fmx_status finish_device_image(fmx_ctx *ctx, fmx_spm *s);
+fmx_status finish_device_image(fmx_ctx *ctx, fmx_spm *s, bool internal);
 
 }  // namespace fmx
 
 using namespace fmx;
 
 namespace fmx {
-namespace {
 
 // vals (device, esz bytes per edge) != nullptr builds a weighted image.
+// ncols != 0 builds an nrows = n by ncols matrix (ids below max(n, ncols);
+// the library's own work images); internal skips the work images of the
+// result.
 fmx_status build_impl(fmx_ctx *ctx, const uint32_t *src, const uint32_t *dst, size_t m, uint64_t n,
                       uint32_t log2_h, uint32_t log2_w, uint32_t flags, const void *vals, uint32_t esz,
-                      int32_t prim_type, fmx_spm **out)
+                      int32_t prim_type, fmx_spm **out, uint64_t ncols, bool internal)
 {
     if (!ctx || !out) return fail(ctx, FMX_EINVAL, "null argument");
     *out = nullptr;
@@ -375,13 +378,14 @@ fmx_status build_impl(fmx_ctx *ctx, const uint32_t *src, const uint32_t *dst, si
         fmx_status st = max_u32_pair(ctx, src, dst, m, &mx, s);
         if (st) return st;
         if (n == 0) n = (uint64_t)mx + 1;
-        else if ((uint64_t)mx >= n) return fail(ctx, FMX_EDIM, "vertex id >= n");
+        else if ((uint64_t)mx >= std::max(n, ncols)) return fail(ctx, FMX_EDIM, "vertex id >= n");
     } else if (n == 0) {
         return fail(ctx, FMX_EINVAL, "empty edge list needs an explicit n");
     }
-    if (n > (1ull << 32)) return fail(ctx, FMX_EINVAL, "n exceeds 2^32");
+    if (ncols == 0) ncols = n;
+    if (n > (1ull << 32) || ncols > (1ull << 32)) return fail(ctx, FMX_EINVAL, "n exceeds 2^32");
     const uint64_t H = 1ull << log2_h, W = 1ull << log2_w;
-    const uint64_t nbr = (n + H - 1) / H, nbc = (n + W - 1) / W;
+    const uint64_t nbr = (n + H - 1) / H, nbc = (ncols + W - 1) / W;
     KeyFmt f;
     f.lh = (int)log2_h;
     f.lw = (int)log2_w;
@@ -545,7 +549,7 @@ fmx_status build_impl(fmx_ctx *ctx, const uint32_t *src, const uint32_t *dst, si
         memcpy(hdr + 12, &type, 4);
         memcpy(hdr + 16, &hesz, 8);
         memcpy(hdr + 24, &n, 8);
-        memcpy(hdr + 32, &n, 8);
+        memcpy(hdr + 32, &ncols, 8);
         memcpy(hdr + 40, &layout, 4);
         memcpy(hdr + 44, &prim, 4);
         memcpy(hdr + 48, &H, 8);
@@ -572,20 +576,19 @@ fmx_status build_impl(fmx_ctx *ctx, const uint32_t *src, const uint32_t *dst, si
     spm->offs.resize(nbr + 1);
     for (uint64_t b = 0; b <= nbr; b++) spm->offs[b] = (int64_t)(4096 + oex[b]);
     spm->bytes = bytes;
-    st = finish_device_image(ctx, spm.get());
+    st = finish_device_image(ctx, spm.get(), internal);
     if (st) return st;
     *out = spm.release();
     return FMX_OK;
 }
 
-}  // namespace
 }  // namespace fmx
 
 extern "C" fmx_status fmx_spm_build(fmx_ctx *ctx, const uint32_t *src, const uint32_t *dst,
                                     size_t m, uint64_t n, uint32_t log2_h, uint32_t log2_w,
                                     uint32_t flags, fmx_spm **out)
 {
-    return fmx::build_impl(ctx, src, dst, m, n, log2_h, log2_w, flags, nullptr, 0, 0, out);
+    return fmx::build_impl(ctx, src, dst, m, n, log2_h, log2_w, flags, nullptr, 0, 0, out, 0, false);
 }
 
 extern "C" fmx_status fmx_spm_build_w(fmx_ctx *ctx, const uint32_t *src, const uint32_t *dst, const void *vals,
@@ -606,5 +609,6 @@ extern "C" fmx_status fmx_spm_build_w(fmx_ctx *ctx, const uint32_t *src, const u
     default: return fmx::fail(ctx, FMX_ETYPE, "values must be int32, int64, uint64, float32 or float64");
     }
     static const uint8_t none = 0;
-    return fmx::build_impl(ctx, src, dst, m, n, log2_h, log2_w, flags, vals ? vals : &none, esz, prim, out);
+    return fmx::build_impl(ctx, src, dst, m, n, log2_h, log2_w, flags, vals ? vals : &none, esz, prim, out, 0,
+                           false);
 }

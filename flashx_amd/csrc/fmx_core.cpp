@@ -401,15 +401,17 @@ fmx_status upload_dirs(fmx_ctx *ctx, fmx_spm *s)
     return FMX_OK;
 }
 
-fmx_status finish_device_image(fmx_ctx *ctx, fmx_spm *s)
+fmx_status finish_device_image(fmx_ctx *ctx, fmx_spm *s, bool internal)
 {
     s->h_image.resize(s->bytes);
     FMX_HIP(ctx, hipMemcpy(s->h_image.data(), s->d_image, s->bytes, hipMemcpyDeviceToHost));
     fmx_status st = analyse_image(ctx, s);
     if (st) return st;
     st = upload_dirs(ctx, s);
+    if (st || internal) return st;
+    st = prepare_work(ctx, s);
     if (st) return st;
-    return prepare_work(ctx, s);
+    return prepare_fast(ctx, s);
 }
 
 fmx_status prepare_work(fmx_ctx *ctx, fmx_spm *s)
@@ -440,7 +442,8 @@ fmx_status prepare_work(fmx_ctx *ctx, fmx_spm *s)
 // streams), and the copies wait for the caller's earlier work on its stream.
 // Split bands are reduced once, after the last chunk.
 fmx_status spmm_streamed(fmx_ctx *ctx, fmx_spm *spm, Plan *plan, fmx_dtype dtype, int k, const void *X,
-                         int64_t ldx, void *Y, int64_t ldy, hipStream_t s, const double *rscale, uint32_t flags)
+                         int64_t ldx, void *Y, int64_t ldy, hipStream_t s, const double *rscale, uint32_t flags,
+                         const void *Xhot = nullptr, uint64_t nhot = 0)
 {
     const size_t nch = spm->chunk_br.size() - 1;
     FMX_HIP(ctx, hipEventRecord(spm->ev_start, s));
@@ -458,7 +461,8 @@ fmx_status spmm_streamed(fmx_ctx *ctx, fmx_spm *spm, Plan *plan, fmx_dtype dtype
         // absolute word w of the image is at staging byte 2w - lo
         const uint16_t *img = (const uint16_t *)(spm->d_stage[b] - lo);
         fmx_status st = launch_spmm(ctx, spm, plan, dtype, k, X, ldx, Y, ldy, spm->partial.ptr, s, img,
-                                    plan->chunk_items[c], plan->chunk_items[c + 1], c + 1 == nch, rscale, flags);
+                                    plan->chunk_items[c], plan->chunk_items[c + 1], c + 1 == nch, rscale, flags,
+                                    Xhot, nhot);
         if (st) return st;
         FMX_HIP(ctx, hipEventRecord(spm->ev_free[b], s));
         spm->free_recorded[b] = true;
@@ -468,8 +472,6 @@ fmx_status spmm_streamed(fmx_ctx *ctx, fmx_spm *spm, Plan *plan, fmx_dtype dtype
 
 // Pinned host image (already holding h_pinned_len bytes) -> validated,
 // chunked streamed matrix.
-fmx_status alloc_pinned(fmx_ctx *ctx, fmx_spm *s, uint64_t len);
-
 fmx_status finish_streamed(fmx_ctx *ctx, fmx_spm *s, uint64_t chunk_bytes, bool whole_file)
 {
     fmx_status st = analyse_image(ctx, s);
@@ -477,6 +479,21 @@ fmx_status finish_streamed(fmx_ctx *ctx, fmx_spm *s, uint64_t chunk_bytes, bool 
     if (whole_file && s->bytes != s->h_pinned_len)
         return fail(ctx, FMX_EFORMAT, "matrix file is longer than its block rows");
     if (chunk_bytes < 4096) return fail(ctx, FMX_EINVAL, "chunk_bytes must be at least 4096");
+    st = finish_streamed_chunks(ctx, s, chunk_bytes);
+    if (st) return st;
+    // the gather-order image, when its construction fits on the device (an
+    // image larger than HBM streams in the reference's column order only)
+    st = prepare_fast(ctx, s, chunk_bytes);
+    if (st == FMX_ENOMEM) {
+        s->fast.reset();
+        st = FMX_OK;
+    }
+    return st;
+}
+
+fmx_status finish_streamed_chunks(fmx_ctx *ctx, fmx_spm *s, uint64_t chunk_bytes)
+{
+    fmx_status st = FMX_OK;
     if (const uint32_t h2 = work_rows_for(s)) {
         // tiles taller than an SpMM band: the re-cut work image (in pinned
         // host memory too) is what every SpMM streams, as a resident matrix
@@ -491,7 +508,8 @@ fmx_status finish_streamed(fmx_ctx *ctx, fmx_spm *s, uint64_t chunk_bytes, bool 
         memcpy(w->h_pinned, img.data(), img.size());
         std::vector<uint8_t>().swap(img);
         w->offs = std::move(offs);
-        st = finish_streamed(ctx, w.get(), chunk_bytes, false);  // its tiles are short: no further re-cut
+        st = analyse_image(ctx, w.get());
+        if (!st) st = finish_streamed_chunks(ctx, w.get(), chunk_bytes);  // its tiles are short: no re-cut
         if (st) return fail(ctx, FMX_EINVAL, "internal error: the re-cut work image: " + ctx->last_error);
         if (w->nnz != s->nnz) return fail(ctx, FMX_EINVAL, "internal error: the re-cut work image lost non-zeros");
         s->work = std::move(w);
@@ -580,7 +598,20 @@ fmx_status spmm_exec(fmx_ctx *ctx, const fmx_spm *cspm, fmx_dtype dtype, int k, 
     if ((!X && spm->ncols) || (!Y && spm->nrows)) return fail(ctx, FMX_EINVAL, "null dense operand");
     if (spm->nrows == 0) return FMX_OK;
     FMX_HIP(ctx, hipSetDevice(ctx->device));
-    if (spm->work) spm = spm->work.get();  // tiles taller than a band: the re-cut image
+    // the image the kernels read: in the default mode the gather-order one;
+    // in the sequential mode the reference's column order, re-cut if its
+    // tiles are taller than a band
+    const fmx_spm *orig = spm;
+    // the gather-order image pays where X's rows share 128-byte lines and X
+    // is larger than the XCDs' L2s together (32 MB): a row of a full line
+    // gains nothing from being next to another hot row, and a small X stays
+    // in L2 anyway, so only the restaging would remain
+    // (FMX_HOT_MIN_BYTES, read at every call, moves the X-size threshold)
+    const char *hm = getenv("FMX_HOT_MIN_BYTES");
+    const uint64_t min_x = hm ? strtoull(hm, nullptr, 10) : (32ull << 20);
+    const bool hot = !seq && spm->fast && (uint64_t)k * es < 128 && spm->ncols * (uint64_t)k * es > min_x;
+    if (hot) spm = spm->fast.get();
+    else if (spm->work) spm = spm->work.get();
     Plan *plan = nullptr;
     fmx_status st = get_plan(ctx, spm, band_rows(spm, k, acc), seq, &plan);
     if (st) return st;
@@ -609,6 +640,25 @@ fmx_status spmm_exec(fmx_ctx *ctx, const fmx_spm *cspm, fmx_dtype dtype, int k, 
         ldx = k;
     }
     const uint32_t kflags = flags & (FMX_SPMM_SEQ | FMX_SPMM_Y_COLMAJOR);
+    if (hot) {
+        // X's hot rows, densely, in the gather-order image's column order
+        const size_t hb = orig->nhot * (size_t)k * es;
+        if (hb > ctx->xhot.bytes) {
+            if (ctx->xhot.ptr) FMX_HIP(ctx, hipFree(ctx->xhot.ptr));
+            ctx->xhot.ptr = nullptr;
+            ctx->xhot.bytes = 0;
+            FMX_HIP(ctx, hipMalloc(&ctx->xhot.ptr, hb));
+            ctx->xhot.bytes = hb;
+        }
+        st = stage_hot_rows(ctx, dtype, orig->nhot, k, X, ldx, orig->d_hot, ctx->xhot.ptr, s);
+        if (st) return st;
+        if (spm->streamed())
+            return spmm_streamed(ctx, spm, plan, dtype, k, X, ldx, Y, ldy, s, post, kflags, ctx->xhot.ptr,
+                                 orig->nhot);
+        return launch_spmm(ctx, spm, plan, dtype, k, X, ldx, Y, ldy, spm->partial.ptr, s,
+                           (const uint16_t *)spm->d_image, 0, plan->nitems, true, post, kflags, ctx->xhot.ptr,
+                           orig->nhot);
+    }
     if (!spm->streamed())
         return launch_spmm(ctx, spm, plan, dtype, k, X, ldx, Y, ldy, spm->partial.ptr, s,
                            (const uint16_t *)spm->d_image, 0, plan->nitems, true, post, kflags);
@@ -632,6 +682,7 @@ fmx_spm::~fmx_spm()
     if (d_image) (void)hipFree(d_image);
     if (d_dir) (void)hipFree(d_dir);
     if (d_vdir) (void)hipFree(d_vdir);
+    if (d_hot) (void)hipFree(d_hot);
     for (int b = 0; b < 2; b++) {
         if (d_stage[b]) (void)hipFree(d_stage[b]);
         if (ev_copied[b]) (void)hipEventDestroy(ev_copied[b]);
@@ -736,6 +787,8 @@ fmx_status fmx_spm_upload(fmx_ctx *ctx, const void *image, size_t bytes,
     st = upload_dirs(ctx, s.get());
     if (st) return st;
     st = prepare_work(ctx, s.get());
+    if (st) return st;
+    st = prepare_fast(ctx, s.get());
     if (st) return st;
     *out = s.release();
     return FMX_OK;

@@ -86,6 +86,7 @@ struct fmx_ctx {
     void *ws = nullptr;        // grow-only device workspace (dense-op partials)
     size_t ws_bytes = 0;
     fmx::Scratch xstage;       // grow-only: a dense operand restaged row-major / pre-scaled
+    fmx::Scratch xhot;         // grow-only: the hot rows of X, densely (gather-order images)
     void *noalloc_ws = nullptr;  // this device's copy of the fixed scratch (fmx_gram_noalloc)
     size_t noalloc_bytes = 0;
 };
@@ -122,6 +123,13 @@ struct fmx_spm {
     // Resident images with tiles taller than an SpMM band: the same matrix
     // re-cut into short tiles, which every SpMM runs on (retile.cpp).
     std::unique_ptr<fmx_spm> work;
+    // Gather order (hotcols.cpp): the matrix with its nhot most used columns
+    // relabelled 0..nhot-1 (hot_ids[i] is the original column of new column
+    // i) and every other column c relabelled nhot + c.  Default-mode SpMMs
+    // run on it with X's hot rows restaged densely first.
+    std::unique_ptr<fmx_spm> fast;
+    uint32_t *d_hot = nullptr;
+    uint64_t nhot = 0;
     bool streamed() const { return h_pinned != nullptr; }
     const uint8_t *host_image() const { return h_pinned ? h_pinned : h_image.data(); }
     uint64_t host_bytes() const { return h_pinned ? h_pinned_len : h_image.size(); }
@@ -137,6 +145,21 @@ int elem_size(fmx_dtype dt);
 
 // Validates a host image and fills the matrix fields + directory.
 fmx_status analyse_image(fmx_ctx *ctx, fmx_spm *spm);
+// Device construction (build_tiles.hip); ncols != 0: an n x ncols matrix;
+// internal: the result gets no work images of its own.
+fmx_status build_impl(fmx_ctx *ctx, const uint32_t *src, const uint32_t *dst, size_t m, uint64_t n,
+                      uint32_t log2_h, uint32_t log2_w, uint32_t flags, const void *vals, uint32_t esz,
+                      int32_t prim_type, fmx_spm **out, uint64_t ncols, bool internal);
+// The gather-order image of a resident matrix (hotcols.cpp): columns
+// relabelled so that the most used ones form a dense front block of X.
+// stream_chunk != 0: the image is streamed from pinned host memory in
+// chunks of that many bytes, as the matrix it belongs to.
+fmx_status prepare_fast(fmx_ctx *ctx, fmx_spm *spm, uint64_t stream_chunk = 0);
+fmx_status finish_streamed(fmx_ctx *ctx, fmx_spm *s, uint64_t chunk_bytes, bool whole_file);
+// The chunks, staging buffers and directory of an analysed streamed image
+// (or of its re-cut work image), without a gather-order image.
+fmx_status finish_streamed_chunks(fmx_ctx *ctx, fmx_spm *s, uint64_t chunk_bytes);
+fmx_status alloc_pinned(fmx_ctx *ctx, fmx_spm *s, uint64_t len);
 // The work image of a resident matrix with tiles taller than the work
 // tile height (retile.cpp): built, validated and uploaded into spm->work.
 fmx_status prepare_work(fmx_ctx *ctx, fmx_spm *spm);
@@ -153,12 +176,18 @@ uint32_t band_rows(const fmx_spm *spm, int k, int s);
 // image at img (absolute u16 word w of the image is img[w]) and, if reduce,
 // the in-order reduction of the plan's split bands.
 // the kernels' part of the FMX_SPMM_* flags (FMX_SPMM_SEQ, FMX_SPMM_Y_COLMAJOR).
+// Columns below nhot are read from Xhot (nhot x k, leading dimension k),
+// column c >= nhot from row c - nhot of X.
 fmx_status launch_spmm(fmx_ctx *ctx, const fmx_spm *spm, const Plan *plan,
                        fmx_dtype dt, int k, const void *X, int64_t ldx,
                        void *Y, int64_t ldy, void *partial,
                        hipStream_t stream, const uint16_t *img,
                        uint32_t item_begin, uint32_t item_end, bool reduce,
-                       const double *rscale, uint32_t flags);
+                       const double *rscale, uint32_t flags, const void *Xhot = nullptr,
+                       uint64_t nhot = 0);
+// out[i][0..k) = X[hot[i]][0..k) for i < nhot (out has leading dimension k).
+fmx_status stage_hot_rows(fmx_ctx *ctx, fmx_dtype dt, uint64_t nhot, int k, const void *X, int64_t ldx,
+                          const uint32_t *hot, void *out, hipStream_t stream);
 
 // out (n x k, row-major, leading dimension k) = X, read row- or column-major
 // (xcol), times pre[i] per row when pre is given (FP data only).

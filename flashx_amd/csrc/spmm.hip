@@ -59,7 +59,19 @@ struct SpmmArgs {
     int vwords;            // weighted images: u16 words per value (2 or 4)
     const double *rscale;  // floating-point data: Y row r is scaled by rscale[r] (or null)
     int ycol;              // Y column-major: element (r, j) at Y[j * ldy + r]
+    const void *Xhot;      // gather-order images: rows of columns < nhot (leading dimension k)
+    uint64_t nhot;         // columns c >= nhot are row c - nhot of X
 };
+
+// Row gcol of the gathered operand: with HOT (a gather-order image) the
+// dense hot block for the first nhot columns and X for the others, else X.
+template <typename T, bool HOT>
+__device__ __forceinline__ const T *xrow(const SpmmArgs &a, uint32_t gcol)
+{
+    if constexpr (!HOT) return (const T *)a.X + (uint64_t)gcol * a.ldx;
+    return gcol < a.nhot ? (const T *)a.Xhot + (uint64_t)gcol * (uint32_t)a.k
+                         : (const T *)a.X + (uint64_t)(gcol - a.nhot) * a.ldx;
+}
 
 template <typename T, int E>
 struct alignas(sizeof(T) * E) Vec {
@@ -309,13 +321,12 @@ __device__ __forceinline__ uint32_t window_advance(const uint2 *dreg, const bool
 // ---------------------------------------------------------------------------
 // Drains (compile-time width K): gathers of P consecutive entries per lane
 // group, summed in registers per row run, added to the band at row changes.
-template <typename T, int K, int P, typename A>
+template <typename T, int K, int P, typename A, bool HOT>
 __device__ __forceinline__ void gather_batch(const SpmmArgs &a, const uint32_t *qcol, const uint16_t *qrow,
                                              uint32_t base, uint32_t qn, int lane, Vec<T, Lanes<T, K>::E> *v,
                                              uint32_t *rr)
 {
     using C = Lanes<T, K>;
-    const T *X = (const T *)a.X;
     const int q = lane % C::L;
     const int g = lane / C::L;
     uint32_t gcs[P];
@@ -331,7 +342,7 @@ __device__ __forceinline__ void gather_batch(const SpmmArgs &a, const uint32_t *
     // every queue read before the first gather, every gather before the first use
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int pp = 0; pp < P; pp++) v[pp] = *(const Vec<T, C::E> *)(X + (uint64_t)gcs[pp] * a.ldx + q * C::E);
+    for (int pp = 0; pp < P; pp++) v[pp] = *(const Vec<T, C::E> *)(xrow<T, HOT>(a, gcs[pp]) + q * C::E);
     __builtin_amdgcn_sched_barrier(0);
 }
 
@@ -449,11 +460,10 @@ __device__ __forceinline__ void drain_generic_seq(const SpmmArgs &a, T *band, ui
                                                   uint32_t qn, int lane);
 
 // Runtime width: flat (entry, element) pairs, one element per lane.
-template <typename T, int P, typename A>
+template <typename T, int P, typename A, bool HOT>
 __device__ __forceinline__ void drain_generic(const SpmmArgs &a, A *band, uint32_t S, const uint32_t *qcol,
                                               const uint16_t *qrow, uint32_t qn, int lane)
 {
-    const T *X = (const T *)a.X;
     const uint32_t k = (uint32_t)a.k;
     const uint32_t tot = qn * k;
     for (uint32_t base = 0; base < tot; base += 64 * P) {
@@ -465,7 +475,7 @@ __device__ __forceinline__ void drain_generic(const SpmmArgs &a, A *band, uint32
             const uint32_t fc = f < tot ? f : tot - 1;
             const uint32_t e = fc / k, j = fc - e * k;
             const uint32_t qr = qrow[e];
-            v[pp] = X[(uint64_t)qcol[e] * a.ldx + j];
+            v[pp] = xrow<T, HOT>(a, qcol[e])[j];
             dst[pp] = (f < tot && qr != 0xFFFFu) ? qr * S + j : 0xFFFFFFFFu;
         }
 #pragma unroll
@@ -495,7 +505,6 @@ __device__ __forceinline__ void drain_generic_seq(const SpmmArgs &a, T *band, ui
                                                   uint32_t qn, int lane)
 {
 #pragma clang fp contract(off)
-    const T *X = (const T *)a.X;
     const uint32_t k = (uint32_t)a.k;
     for (uint32_t j0 = 0; j0 < k; j0 += 64) {
         const uint32_t j = j0 + (uint32_t)lane;
@@ -508,7 +517,7 @@ __device__ __forceinline__ void drain_generic_seq(const SpmmArgs &a, T *band, ui
                 const uint32_t e = e0 + pp;
                 const uint32_t ec = e < qn ? e : qn - 1;
                 const uint32_t qr = qrow[ec];
-                v[pp] = X[(uint64_t)qcol[ec] * a.ldx + (act ? j : 0u)];
+                v[pp] = xrow<T, false>(a, qcol[ec])[act ? j : 0u];
                 if constexpr (W) v[pp] = v[pp] * (T)load_value<V>(vimg + qval[ec]);
                 dst[pp] = (e < qn && act && qr != 0xFFFFu) ? qr * S + j : 0xFFFFFFFFu;
             }
@@ -657,7 +666,7 @@ __device__ __forceinline__ ItemState item_state(const SpmmArgs &a, const Item &i
 // SEQ: the sequential mode (accumulate_batch_seq): the band holds T and
 // every element is summed in the reference's order; the plan of such a call
 // neither splits nor rotates items.
-template <typename T, int K, int R, int P, bool SEQ>
+template <typename T, int K, int R, int P, bool SEQ, bool HOT>
 __global__ void __launch_bounds__(64 * kWPB) __attribute__((amdgpu_waves_per_eu(4, 4)))
 spmm_lane(SpmmArgs a)
 {
@@ -711,13 +720,13 @@ spmm_lane(SpmmArgs a)
             for (uint32_t base = 0; base < qn; base += C::NZP * P) {
                 Vec<T, C::E> v[P];
                 uint32_t rr[P];
-                gather_batch<T, K, P, A>(a, qcol, qrow, base, qn, lane, v, rr);
+                gather_batch<T, K, P, A, HOT>(a, qcol, qrow, base, qn, lane, v, rr);
                 if constexpr (SEQ) accumulate_batch_seq<T, K, P>(band, S, lane, v, rr);
                 else accumulate_batch<T, K, P, A>(band, S, lane, v, rr);
             }
         } else if (qn) {
             if constexpr (SEQ) drain_generic_seq<T, T, P, false>(a, band, S, qcol, qrow, nullptr, nullptr, qn, lane);
-            else drain_generic<T, P, A>(a, band, S, qcol, qrow, qn, lane);
+            else drain_generic<T, P, A, HOT>(a, band, S, qcol, qrow, qn, lane);
         }
         wave_sync();
         c = cn;
@@ -875,7 +884,7 @@ __device__ __forceinline__ uint32_t decode_chunk_w(const ItemCtx &ic, uint64_t c
     return qn;
 }
 
-template <typename T, typename V, int K, int R, int P, bool SEQ>
+template <typename T, typename V, int K, int R, int P, bool SEQ, bool HOT>
 __global__ void __launch_bounds__(64 * kWPB) __attribute__((amdgpu_waves_per_eu(4, 4)))
 spmm_lane_w(SpmmArgs a)
 {
@@ -908,7 +917,6 @@ spmm_lane_w(SpmmArgs a)
         win[DWIN] = make_uint2(0xFFFFFFFFu, 0u);
         wvb[DWIN] = 0xFFFFFFFFu;
     }
-    const T *X = (const T *)a.X;
     const uint16_t *vimg = a.img + st.ic.bw;
 
     for (int seg = 0; seg < 2; seg++) {
@@ -975,7 +983,7 @@ spmm_lane_w(SpmmArgs a)
                 }
 #pragma unroll
                 for (int pp = 0; pp < P; pp++)
-                    v[pp] = *(const Vec<T, C::E> *)(X + (uint64_t)gcs[pp] * a.ldx + q * C::E);
+                    v[pp] = *(const Vec<T, C::E> *)(xrow<T, HOT>(a, gcs[pp]) + q * C::E);
                 if constexpr (SEQ) {
                     // products rounded in T, then added in order
 #pragma clang fp contract(off)
@@ -1027,7 +1035,7 @@ spmm_lane_w(SpmmArgs a)
                     const uint32_t fc = f < tot ? f : tot - 1;
                     const uint32_t e = fc / k, jj = fc - e * k;
                     const uint32_t qr = qrow[e];
-                    xv[pp] = X[(uint64_t)qcol[e] * a.ldx + jj];
+                    xv[pp] = xrow<T, HOT>(a, qcol[e])[jj];
                     vv[pp] = load_value<V>(vimg + qval[e]);
                     dst[pp] = (f < tot && qr != 0xFFFFu) ? qr * S + jj : 0xFFFFFFFFu;
                 }
@@ -1109,7 +1117,7 @@ fmx_status launch_items(fmx_ctx *ctx, const SpmmArgs &a, hipStream_t stream)
 
 // Weighted images; V is the type of the stored values (T, or float values
 // with FP64 dense data: spmm_creator<double, float>).
-template <typename T, typename V, int K, bool SEQ>
+template <typename T, typename V, int K, bool SEQ, bool HOT>
 fmx_status launch_kw(fmx_ctx *ctx, SpmmArgs a, hipStream_t stream)
 {
     constexpr int P = passes_for<T, K>();
@@ -1118,37 +1126,40 @@ fmx_status launch_kw(fmx_ctx *ctx, SpmmArgs a, hipStream_t stream)
     constexpr int QS = CW + CW / (K ? P : CW) + 1;
     const uint32_t win_off = (a.band_bytes + QS * 10 + 7) & ~7u;
     a.wave_bytes = (win_off + (LaneCfg<R>::DWIN + 1) * 16 + 15) & ~15u;
-    return launch_items<spmm_lane_w<T, V, K, R, P, SEQ>>(ctx, a, stream);
+    return launch_items<spmm_lane_w<T, V, K, R, P, SEQ, HOT>>(ctx, a, stream);
 }
 
-template <typename T, int K, bool SEQ>
+template <typename T, int K, bool SEQ, bool HOT>
 fmx_status launch_k(fmx_ctx *ctx, SpmmArgs a, int vprim, hipStream_t stream)
 {
     if (a.vdir) {
         if constexpr (std::is_same<T, double>::value)
-            if (vprim == 8) return launch_kw<double, float, K, SEQ>(ctx, a, stream);
-        return launch_kw<T, T, K, SEQ>(ctx, a, stream);
+            if (vprim == 8) return launch_kw<double, float, K, SEQ, HOT>(ctx, a, stream);
+        return launch_kw<T, T, K, SEQ, HOT>(ctx, a, stream);
     }
     constexpr int P = passes_for<T, K>();
     constexpr int CW = 64 * kLaneR;
     constexpr int QS = CW + CW / (K ? P : CW) + 1;
     const uint32_t win_off = (a.band_bytes + QS * 6 + 7) & ~7u;
     a.wave_bytes = (win_off + (LaneCfg<kLaneR>::DWIN + 1) * 8 + 15) & ~15u;
-    return launch_items<spmm_lane<T, K, kLaneR, P, SEQ>>(ctx, a, stream);
+    return launch_items<spmm_lane<T, K, kLaneR, P, SEQ, HOT>>(ctx, a, stream);
 }
 
-template <typename T, bool SEQ>
+// HOT (a gather-order image, rows narrower than a 128-byte line) has kernels
+// for the widths it is used with; other widths take the runtime-width form.
+template <typename T, bool SEQ, bool HOT>
 fmx_status launch_w(fmx_ctx *ctx, SpmmArgs a, int k, int vprim, hipStream_t stream)
 {
+    constexpr int KMAX = HOT ? (int)(64 / sizeof(T)) : 64;
     switch (k) {
-    case 1: return launch_k<T, 1, SEQ>(ctx, a, vprim, stream);
-    case 2: return launch_k<T, 2, SEQ>(ctx, a, vprim, stream);
-    case 4: return launch_k<T, 4, SEQ>(ctx, a, vprim, stream);
-    case 8: return launch_k<T, 8, SEQ>(ctx, a, vprim, stream);
-    case 16: return launch_k<T, 16, SEQ>(ctx, a, vprim, stream);
-    case 32: return launch_k<T, 32, SEQ>(ctx, a, vprim, stream);
-    case 64: return launch_k<T, 64, SEQ>(ctx, a, vprim, stream);
-    default: return launch_k<T, 0, SEQ>(ctx, a, vprim, stream);
+    case 1: return launch_k<T, 1, SEQ, HOT>(ctx, a, vprim, stream);
+    case 2: return launch_k<T, 2, SEQ, HOT>(ctx, a, vprim, stream);
+    case 4: return launch_k<T, 4, SEQ, HOT>(ctx, a, vprim, stream);
+    case 8: return launch_k<T, 8, SEQ, HOT>(ctx, a, vprim, stream);
+    case 16: return launch_k<T, (16 <= KMAX ? 16 : 0), SEQ, HOT>(ctx, a, vprim, stream);
+    case 32: return launch_k<T, (32 <= KMAX ? 32 : 0), SEQ, HOT>(ctx, a, vprim, stream);
+    case 64: return launch_k<T, (64 <= KMAX ? 64 : 0), SEQ, HOT>(ctx, a, vprim, stream);
+    default: return launch_k<T, 0, SEQ, HOT>(ctx, a, vprim, stream);
     }
 }
 
@@ -1156,7 +1167,7 @@ template <typename T>
 fmx_status launch_t(fmx_ctx *ctx, const fmx_spm *spm, const Plan *plan, int k,
                     const void *X, int64_t ldx, void *Y, int64_t ldy, void *partial,
                     hipStream_t stream, const uint16_t *img, uint32_t item_begin, uint32_t item_end,
-                    bool reduce, const double *rscale, uint32_t flags)
+                    bool reduce, const double *rscale, uint32_t flags, const void *Xhot, uint64_t nhot)
 {
     // integer sums are exact in any order: the sequential mode is the default one
     const bool seq = (flags & FMX_SPMM_SEQ) && std::is_floating_point<T>::value;
@@ -1184,15 +1195,21 @@ fmx_status launch_t(fmx_ctx *ctx, const fmx_spm *spm, const Plan *plan, int k,
     a.vwords = (int)(spm->esz / 2);
     a.rscale = rscale;
     a.ycol = (flags & FMX_SPMM_Y_COLMAJOR) ? 1 : 0;
+    a.Xhot = Xhot;
+    a.nhot = Xhot ? nhot : 0;
     // 16-byte vector gathers need aligned rows; otherwise the runtime-width path.
     const int vb = (k * (int)sizeof(T) < 16) ? k * (int)sizeof(T) : 16;
-    const bool aligned = ((uintptr_t)X % vb == 0) && ((ldx * (int64_t)sizeof(T)) % vb == 0);
+    const bool aligned = ((uintptr_t)X % vb == 0) && ((ldx * (int64_t)sizeof(T)) % vb == 0) &&
+                         ((uintptr_t)Xhot % vb == 0) && ((k * (int)sizeof(T)) % vb == 0);
     const int kk = aligned ? k : 0;
     fmx_status st;
-    if constexpr (std::is_floating_point<T>::value)
-        st = seq ? launch_w<T, true>(ctx, a, kk, spm->prim, stream) : launch_w<T, false>(ctx, a, kk, spm->prim, stream);
+    if (a.nhot)
+        st = launch_w<T, false, true>(ctx, a, kk, spm->prim, stream);  // never with the sequential mode
+    else if constexpr (std::is_floating_point<T>::value)
+        st = seq ? launch_w<T, true, false>(ctx, a, kk, spm->prim, stream)
+                 : launch_w<T, false, false>(ctx, a, kk, spm->prim, stream);
     else
-        st = launch_w<T, false>(ctx, a, kk, spm->prim, stream);
+        st = launch_w<T, false, false>(ctx, a, kk, spm->prim, stream);
     if (st) return st;
     if (reduce && plan->nsplits) {
         using A = typename AccOf<T>::type;
@@ -1272,19 +1289,53 @@ fmx_status stage_dense_rows(fmx_ctx *ctx, fmx_dtype dt, uint64_t n, int k, const
     return FMX_OK;
 }
 
+// out[i][0..k) = X[hot[i]][0..k): rows of V-sized pieces, a row's pieces on
+// consecutive lanes.
+template <typename V>
+__global__ void __launch_bounds__(256) stage_hot(const V *X, uint64_t ldx_v, const uint32_t *hot, uint64_t nhot,
+                                                  uint32_t kv, V *out)
+{
+    const uint64_t tot = nhot * kv;
+    for (uint64_t e = blockIdx.x * 256ull + threadIdx.x; e < tot; e += (uint64_t)gridDim.x * 256) {
+        const uint64_t i = e / kv, j = e - i * kv;
+        out[e] = X[(uint64_t)hot[i] * ldx_v + j];
+    }
+}
+
+fmx_status stage_hot_rows(fmx_ctx *ctx, fmx_dtype dt, uint64_t nhot, int k, const void *X, int64_t ldx,
+                          const uint32_t *hot, void *out, hipStream_t stream)
+{
+    if (nhot == 0) return FMX_OK;
+    const uint64_t rb = (uint64_t)k * elem_size(dt), lb = (uint64_t)ldx * elem_size(dt);
+    const uint64_t blocks = std::min<uint64_t>((nhot * k + 255) / 256, (uint64_t)ctx->num_cus * 32);
+    // the widest piece that divides the row, the stride and the base
+    if (rb % 16 == 0 && lb % 16 == 0 && (uintptr_t)X % 16 == 0)
+        hipLaunchKernelGGL(stage_hot<uint4>, dim3((unsigned)blocks), dim3(256), 0, stream, (const uint4 *)X, lb / 16,
+                           hot, nhot, (uint32_t)(rb / 16), (uint4 *)out);
+    else if (rb % 8 == 0 && lb % 8 == 0 && (uintptr_t)X % 8 == 0)
+        hipLaunchKernelGGL(stage_hot<uint2>, dim3((unsigned)blocks), dim3(256), 0, stream, (const uint2 *)X, lb / 8,
+                           hot, nhot, (uint32_t)(rb / 8), (uint2 *)out);
+    else
+        hipLaunchKernelGGL(stage_hot<uint32_t>, dim3((unsigned)blocks), dim3(256), 0, stream, (const uint32_t *)X,
+                           lb / 4, hot, nhot, (uint32_t)(rb / 4), (uint32_t *)out);
+    FMX_HIP(ctx, hipGetLastError());
+    return FMX_OK;
+}
+
 fmx_status launch_spmm(fmx_ctx *ctx, const fmx_spm *spm, const Plan *plan, fmx_dtype dt,
                        int k, const void *X, int64_t ldx, void *Y, int64_t ldy,
                        void *partial, hipStream_t stream, const uint16_t *img,
-                       uint32_t ib, uint32_t ie, bool reduce, const double *rscale, uint32_t flags)
+                       uint32_t ib, uint32_t ie, bool reduce, const double *rscale, uint32_t flags,
+                       const void *Xhot, uint64_t nhot)
 {
     if (rscale && dt != FMX_F32 && dt != FMX_F64) return fail(ctx, FMX_ETYPE, "row scaling needs FP data");
     switch (dt) {
-    case FMX_I32: return launch_t<int32_t>(ctx, spm, plan, k, X, ldx, Y, ldy, partial, stream, img, ib, ie, reduce, rscale, flags);
+    case FMX_I32: return launch_t<int32_t>(ctx, spm, plan, k, X, ldx, Y, ldy, partial, stream, img, ib, ie, reduce, rscale, flags, Xhot, nhot);
     // size_t sums wrap exactly as long sums do: one kernel serves both
     case FMX_I64:
-    case FMX_U64: return launch_t<int64_t>(ctx, spm, plan, k, X, ldx, Y, ldy, partial, stream, img, ib, ie, reduce, rscale, flags);
-    case FMX_F32: return launch_t<float>(ctx, spm, plan, k, X, ldx, Y, ldy, partial, stream, img, ib, ie, reduce, rscale, flags);
-    case FMX_F64: return launch_t<double>(ctx, spm, plan, k, X, ldx, Y, ldy, partial, stream, img, ib, ie, reduce, rscale, flags);
+    case FMX_U64: return launch_t<int64_t>(ctx, spm, plan, k, X, ldx, Y, ldy, partial, stream, img, ib, ie, reduce, rscale, flags, Xhot, nhot);
+    case FMX_F32: return launch_t<float>(ctx, spm, plan, k, X, ldx, Y, ldy, partial, stream, img, ib, ie, reduce, rscale, flags, Xhot, nhot);
+    case FMX_F64: return launch_t<double>(ctx, spm, plan, k, X, ldx, Y, ldy, partial, stream, img, ib, ie, reduce, rscale, flags, Xhot, nhot);
     default: return fail(ctx, FMX_ETYPE, "wrong matrix type in spmm");
     }
 }

@@ -27,7 +27,7 @@ def _time_spmm(spm, X, Y, reps):
     return a.elapsed_time(b) / reps
 
 
-def autotune_tile(src, dst, n, k, dtype, flags=F.BUILD_DEDUP, candidates=DEFAULT_CANDIDATES, reps=3, X=None,
+def autotune_tile(src, dst, n, k, dtype, flags=F.BUILD_DEDUP, candidates=DEFAULT_CANDIDATES, reps=10, X=None,
                   keep=False):
     """Returns ((log2_h, log2_w), {candidate: ms per SpMM}) for the edge list
     src, dst (device tensors) multiplied by an n x k operand of `dtype`.  With
