@@ -75,6 +75,9 @@ def main():
     ap.add_argument("--no-eigen", action="store_true", help="skip the eigensolver legs")
     ap.add_argument("--eigen-only", action="store_true",
                     help="only the scale-22 eigensolve leg (one JSON line), e.g. under a profiler")
+    ap.add_argument("--only-spmm", action="store_true",
+                    help="only the timed SpMM steps (no other legs), e.g. under a profiler; "
+                         "takes the tile shape from --log2-h/--log2-w (default 32 x 32768)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run three 1 GiB device copies (the known-size kernel that scales "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE into bytes)")
@@ -106,6 +109,8 @@ def main():
     src, dst = F.rmat_edges(SCALE, m, SEED)
     torch.cuda.synchronize()
     tune = None
+    if args.only_spmm and args.log2_h is None:
+        args.log2_h = 5
     if args.log2_h is None:
         (args.log2_h, args.log2_w), tt = autotune_tile(src, dst, n, K, torch.float32)
         tune = {"%dx%d" % (1 << a, 1 << b): round(v, 4) for (a, b), v in tt.items()}
@@ -115,7 +120,8 @@ def main():
                                flags=F.BUILD_DEDUP)
     torch.cuda.synchronize()
     t_build = time.perf_counter() - t0
-    cols_seq = None if args.no_floor else stream_columns(torch, src, dst, n, args.log2_h, args.log2_w)
+    cols_seq = None if (args.no_floor or args.only_spmm) else stream_columns(torch, src, dst, n, args.log2_h,
+                                                                             args.log2_w)
     del src, dst
     info = spm.info
     log("built: n=%d nnz=%d tiles=%d image=%d B (%.2f s incl. host validation)"
@@ -177,6 +183,9 @@ def main():
         "cpu_baseline": None,
     }
 
+    if args.only_spmm:
+        print(json.dumps(out), flush=True)
+        return
     out["operand_variants"] = variants_leg(F, torch, spm, n, check=not args.no_cpu)
     out["scale16"] = scale16_leg(F, torch, check=not args.no_cpu)
     if cols_seq is not None:

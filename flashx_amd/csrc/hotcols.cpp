@@ -2,12 +2,12 @@
 //
 // Every non-zero (r, c) of the SpMM gathers row c of X, and on a power-law
 // graph a few columns take most of the gathers (R-MAT scale 22: the 65,536
-// most used columns hold 63% of the non-zeros, the 400,000 most used 90%).
+// most used columns hold 63% of the non-zeros, the 600,000 most used 95%).
 // In the reference's column order those rows lie scattered over X, each
 // sharing its 128-byte cache line with an unrelated row, so the L2 of an XCD
 // holds far fewer of them than it could.  The gather-order image relabels
 // the columns instead: the nhot most used columns (the fewest whose
-// non-zeros make up 90% of the total) become columns 0..nhot-1 in
+// non-zeros make up 95% of the total) become columns 0..nhot-1 in
 // descending order of use, and every other column c becomes nhot + c.  Each
 // default-mode SpMM first copies X's hot rows into a dense nhot x k block
 // (stage_hot_rows; 2 * nhot * k * s bytes, a few percent of the SpMM's
@@ -104,12 +104,14 @@ fmx_status prepare_fast(fmx_ctx *ctx, fmx_spm *s, uint64_t stream_chunk)
         std::vector<uint32_t>().swap(part[t]);
     }
     // the most used columns, descending, ties by column: the fewest whose
-    // non-zeros make up 90% of the total
+    // non-zeros make up 95% of the total (FMX_HOT_PERMILLE: another share)
     std::vector<uint32_t> order(s->ncols);
     std::iota(order.begin(), order.end(), 0u);
     std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return deg[a] > deg[b]; });
+    const char *pm = getenv("FMX_HOT_PERMILLE");
+    const uint64_t permille = std::min<uint64_t>(1000, pm ? strtoull(pm, nullptr, 10) : 950);
     uint64_t nhot = 0, acc = 0;
-    while (nhot < s->ncols && acc * 10 < s->nnz * 9 && deg[order[nhot]] > 0) acc += deg[order[nhot++]];
+    while (nhot < s->ncols && acc * 1000 < s->nnz * permille && deg[order[nhot]] > 0) acc += deg[order[nhot++]];
     if (nhot == 0) return FMX_OK;
     std::vector<uint32_t> newid(s->ncols);
     for (uint64_t c = 0; c < s->ncols; c++) newid[c] = (uint32_t)(nhot + c);
