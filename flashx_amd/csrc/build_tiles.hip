@@ -371,6 +371,19 @@ fmx_status build_impl(fmx_ctx *ctx, const uint32_t *src, const uint32_t *dst, si
     // the edge list may come from any stream: construction is synchronous
     FMX_HIP(ctx, hipDeviceSynchronize());
     hipStream_t s = ctx->stream;
+    // device time of the construction, from the first kernel to the last
+    // write (host waits for totals in between are included)
+    struct EvPair {
+        hipEvent_t a = nullptr, b = nullptr;
+        ~EvPair()
+        {
+            if (a) (void)hipEventDestroy(a);
+            if (b) (void)hipEventDestroy(b);
+        }
+    } ev;
+    FMX_HIP(ctx, hipEventCreate(&ev.a));
+    FMX_HIP(ctx, hipEventCreate(&ev.b));
+    FMX_HIP(ctx, hipEventRecord(ev.a, s));
 
     // n = max id + 1 (fm_utils.cpp:730-739), or a caller-given n above every id
     if (m) {
@@ -570,9 +583,14 @@ fmx_status build_impl(fmx_ctx *ctx, const uint32_t *src, const uint32_t *dst, si
     hipLaunchKernelGGL(write_stubs, dim3(grid_for(ctx, nbr)), dim3(kB), 0, s, had.p, nbr, offs_ex.p,
                        spm->d_image);
     FMX_HIP(ctx, hipGetLastError());
+    FMX_HIP(ctx, hipEventRecord(ev.b, s));
     std::vector<uint64_t> oex(nbr + 1);
     FMX_HIP(ctx, hipMemcpyAsync(oex.data(), offs_ex.p, (nbr + 1) * 8, hipMemcpyDeviceToHost, s));
     FMX_HIP(ctx, hipStreamSynchronize(s));
+    float build_ms = 0.f;
+    FMX_HIP(ctx, hipEventElapsedTime(&build_ms, ev.a, ev.b));
+    spm->build_ns = (uint64_t)((double)build_ms * 1e6);
+    spm->edges_in = m;
     spm->offs.resize(nbr + 1);
     for (uint64_t b = 0; b <= nbr; b++) spm->offs[b] = (int64_t)(4096 + oex[b]);
     spm->bytes = bytes;

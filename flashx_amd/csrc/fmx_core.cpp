@@ -7,6 +7,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <numeric>
+#include <thread>
 
 #include "dense.h"
 #include "fmx_internal.h"
@@ -104,34 +105,34 @@ fmx_status analyse_image(fmx_ctx *ctx, fmx_spm *s)
         if (s->offs[i] <= s->offs[i - 1] || (s->offs[i] & 1))
             return fail(ctx, FMX_EFORMAT, "block-row offsets must be even and strictly increasing");
 
-    s->dir.clear();
-    s->vdir.clear();
-    s->brow_dir.assign(nbr + 1, 0);
-    uint64_t nnz = 0;
-    for (uint64_t br = 0; br < nbr; br++) {
-        s->brow_dir[br] = (uint32_t)s->dir.size();
+    // Each block row is validated on its own (threads over block-row
+    // stripes); the first failing block row's message is reported.  A
+    // second pass fills the directory at each block row's prefix position.
+    std::vector<uint32_t> ntile(nbr, 0);
+    std::vector<uint64_t> rnnz(nbr, 0);
+    std::vector<std::string> err(nbr);
+    auto check_row = [&](uint64_t br) -> std::string {
         const uint8_t *base = img + s->offs[br];
         const uint8_t *p = base, *end = img + s->offs[br + 1];
-        if ((uint64_t)(end - base) / 2 >= (1ULL << 31))
-            return fail(ctx, FMX_EFORMAT, "block row larger than 2^31 words");
+        if ((uint64_t)(end - base) / 2 >= (1ULL << 31)) return "block row larger than 2^31 words";
         const uint64_t rows_here = std::min<uint64_t>(s->H, s->nrows - br * s->H);
         long prev_bc = -1;
-        uint64_t empty_tiles = 0;
+        uint64_t empty_tiles = 0, nnz = 0;
+        uint32_t tiles = 0;
         while (p < end) {
-            if (end - p < 16) return fail(ctx, FMX_EFORMAT, "truncated tile header");
+            if (end - p < 16) return "truncated tile header";
             uint32_t tbr = rd32(p), tbc = rd32(p + 4), tnnz = rd32(p + 8);
             uint32_t tnrow = rd16(p + 12), tncoo = rd16(p + 14);
-            if (tbr != br) return fail(ctx, FMX_EFORMAT, "tile in the wrong block row");
+            if (tbr != br) return "tile in the wrong block row";
             uint64_t size = 16;
             if (!tnnz) empty_tiles++;
             if (tnnz) {
-                if ((long)tbc <= prev_bc) return fail(ctx, FMX_EFORMAT, "tiles not in ascending block column");
-                if ((uint64_t)tbc >= s->nbc) return fail(ctx, FMX_EFORMAT, "block column out of range");
+                if ((long)tbc <= prev_bc) return "tiles not in ascending block column";
+                if ((uint64_t)tbc >= s->nbc) return "block column out of range";
                 prev_bc = tbc;
-                if (tncoo > tnrow || tncoo > tnnz || tnrow > rows_here)
-                    return fail(ctx, FMX_EFORMAT, "inconsistent tile counts");
+                if (tncoo > tnrow || tncoo > tnnz || tnrow > rows_here) return "inconsistent tile counts";
                 size = 16 + 2ULL * tnrow + 2ULL * tnnz + 2 + s->esz * tnnz;
-                if ((uint64_t)(end - p) < size) return fail(ctx, FMX_EFORMAT, "tile runs past its block row");
+                if ((uint64_t)(end - p) < size) return "tile runs past its block row";
                 const uint64_t cols_here = std::min<uint64_t>(s->W, s->ncols - (uint64_t)tbc * s->W);
                 const uint8_t *w = p + 16;
                 const uint8_t *rend = p + 16 + 2ULL * (tnrow - tncoo) + 2ULL * (tnnz - tncoo);
@@ -141,48 +142,80 @@ fmx_status analyse_image(fmx_ctx *ctx, fmx_spm *s)
                     uint16_t h = rd16(w);
                     // 0xFFFF is row 32767 of a 32768-row tile; the region
                     // end is positional (sparse_matrix_format.h:339-347).
-                    if (!(h & 0x8000)) return fail(ctx, FMX_EFORMAT, "bad row-part header");
+                    if (!(h & 0x8000)) return "bad row-part header";
                     long r = h & 0x7FFF;
-                    if (r <= prev_row || (uint64_t)r >= rows_here) return fail(ctx, FMX_EFORMAT, "row-part rows out of order or range");
+                    if (r <= prev_row || (uint64_t)r >= rows_here) return "row-part rows out of order or range";
                     prev_row = r;
                     w += 2;
                     uint64_t len = 0;
                     while (w < rend && !(rd16(w) & 0x8000)) {
-                        if (rd16(w) >= cols_here) return fail(ctx, FMX_EFORMAT, "column out of range");
+                        if (rd16(w) >= cols_here) return "column out of range";
                         w += 2;
                         len++;
                     }
-                    if (len == 0) return fail(ctx, FMX_EFORMAT, "empty row part");
+                    if (len == 0) return "empty row part";
                     nz += len;
                     nr++;
                 }
-                if (rd16(rend) != 0xFFFF) return fail(ctx, FMX_EFORMAT, "missing end-of-row-parts marker");
+                if (rd16(rend) != 0xFFFF) return "missing end-of-row-parts marker";
                 const uint8_t *coo = rend + 2;
                 prev_row = -1;
                 for (uint32_t i = 0; i < tncoo; i++) {
                     uint16_t h = rd16(coo + 4ULL * i), c = rd16(coo + 4ULL * i + 2);
-                    if (!(h & 0x8000) || (c & 0x8000)) return fail(ctx, FMX_EFORMAT, "bad COO entry");
+                    if (!(h & 0x8000) || (c & 0x8000)) return "bad COO entry";
                     long r = h & 0x7FFF;
                     if (r <= prev_row || (uint64_t)r >= rows_here || c >= cols_here)
-                        return fail(ctx, FMX_EFORMAT, "COO rows out of order or range");
+                        return "COO rows out of order or range";
                     prev_row = r;
                 }
-                if (nz + tncoo != tnnz || nr + tncoo != tnrow)
-                    return fail(ctx, FMX_EFORMAT, "tile counts disagree with its contents");
+                if (nz + tncoo != tnnz || nr + tncoo != tnrow) return "tile counts disagree with its contents";
                 nnz += tnnz;
             }
-            s->dir.push_back(make_uint2((uint32_t)((p - base) / 2), tbc));
-            if (s->esz) s->vdir.push_back((uint32_t)((p - base) / 2 + 8 + tnrow + tnnz + 1));
+            tiles++;
             p += size;
         }
-        if (p != end) return fail(ctx, FMX_EFORMAT, "block row size disagrees with its tiles");
-        if (s->dir.size() == s->brow_dir[br]) return fail(ctx, FMX_EFORMAT, "block row without tiles");
+        if (p != end) return "block row size disagrees with its tiles";
+        if (tiles == 0) return "block row without tiles";
         // an empty tile only stands for an empty block row (fm_utils.cpp:
         // 317,340-350); the SpMM decoder relies on every other tile being at
         // least 11 words long
-        if (empty_tiles && s->dir.size() - s->brow_dir[br] > 1)
-            return fail(ctx, FMX_EFORMAT, "empty tile in a non-empty block row");
-    }
+        if (empty_tiles && tiles > 1) return "empty tile in a non-empty block row";
+        ntile[br] = tiles;
+        rnnz[br] = nnz;
+        return std::string();
+    };
+    const unsigned nth = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>({16, nbr / 64 + 1,
+                                                                               std::thread::hardware_concurrency()}));
+    auto for_rows = [&](auto fn) {
+        std::vector<std::thread> th;
+        for (unsigned t = 1; t < nth; t++)
+            th.emplace_back([&, t] {
+                for (uint64_t br = t; br < nbr; br += nth) fn(br);
+            });
+        for (uint64_t br = 0; br < nbr; br += nth) fn(br);
+        for (auto &x : th) x.join();
+    };
+    for_rows([&](uint64_t br) { err[br] = check_row(br); });
+    for (uint64_t br = 0; br < nbr; br++)
+        if (!err[br].empty()) return fail(ctx, FMX_EFORMAT, err[br]);
+    s->brow_dir.assign(nbr + 1, 0);
+    for (uint64_t br = 0; br < nbr; br++) s->brow_dir[br + 1] = s->brow_dir[br] + ntile[br];
+    if ((uint64_t)s->brow_dir[nbr] != std::accumulate(ntile.begin(), ntile.end(), (uint64_t)0))
+        return fail(ctx, FMX_EFORMAT, "more than 2^32 tiles");
+    s->dir.assign(s->brow_dir[nbr], make_uint2(0, 0));
+    s->vdir.assign(s->esz ? s->brow_dir[nbr] : 0, 0);
+    for_rows([&](uint64_t br) {
+        const uint8_t *base = img + s->offs[br], *p = base, *end = img + s->offs[br + 1];
+        uint32_t t = s->brow_dir[br];
+        while (p < end) {
+            const uint32_t tnnz = rd32(p + 8), tnrow = rd16(p + 12);
+            s->dir[t] = make_uint2((uint32_t)((p - base) / 2), rd32(p + 4));
+            if (s->esz) s->vdir[t] = (uint32_t)((p - base) / 2 + 8 + tnrow + tnnz + 1);
+            p += tnnz ? 16 + 2ULL * tnrow + 2ULL * tnnz + 2 + s->esz * tnnz : 16;
+            t++;
+        }
+    });
+    const uint64_t nnz = std::accumulate(rnnz.begin(), rnnz.end(), (uint64_t)0);
     s->brow_dir[nbr] = (uint32_t)s->dir.size();
     s->ntiles = s->dir.size();
     s->nnz = nnz;
@@ -809,6 +842,9 @@ fmx_status fmx_spm_info_get(const fmx_spm *spm, fmx_spm_info *info)
     info->ntiles = spm->ntiles;
     const fmx_spm *run = spm->work ? spm->work.get() : spm;  // the image every SpMM reads
     info->stream_chunks = spm->streamed() ? run->chunk_br.size() - 1 : 0;
+    info->build_ns = spm->build_ns;
+    info->edges_in = spm->edges_in;
+    info->hot_columns = spm->fast ? spm->nhot : 0;
     return FMX_OK;
 }
 

@@ -130,6 +130,8 @@ struct fmx_spm {
     std::unique_ptr<fmx_spm> fast;
     uint32_t *d_hot = nullptr;
     uint64_t nhot = 0;
+    uint64_t build_ns = 0;              // device-built: construction kernels' time
+    uint64_t edges_in = 0;              // device-built: edges of the input list
     bool streamed() const { return h_pinned != nullptr; }
     const uint8_t *host_image() const { return h_pinned ? h_pinned : h_image.data(); }
     uint64_t host_bytes() const { return h_pinned ? h_pinned_len : h_image.size(); }

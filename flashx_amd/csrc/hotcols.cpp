@@ -106,8 +106,14 @@ fmx_status prepare_fast(fmx_ctx *ctx, fmx_spm *s, uint64_t stream_chunk)
     // the most used columns, descending, ties by column: the fewest whose
     // non-zeros make up 95% of the total (FMX_HOT_PERMILLE: another share)
     std::vector<uint32_t> order(s->ncols);
-    std::iota(order.begin(), order.end(), 0u);
-    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return deg[a] > deg[b]; });
+    {
+        // counting sort by descending degree, ascending column among equals
+        const uint32_t maxd = *std::max_element(deg.begin(), deg.end());
+        std::vector<uint64_t> pos((size_t)maxd + 2, 0);
+        for (uint64_t c = 0; c < s->ncols; c++) pos[maxd - deg[c] + 1]++;
+        for (size_t i = 1; i < pos.size(); i++) pos[i] += pos[i - 1];
+        for (uint64_t c = 0; c < s->ncols; c++) order[pos[maxd - deg[c]]++] = (uint32_t)c;
+    }
     const char *pm = getenv("FMX_HOT_PERMILLE");
     const uint64_t permille = std::min<uint64_t>(1000, pm ? strtoull(pm, nullptr, 10) : 950);
     uint64_t nhot = 0, acc = 0;
@@ -126,8 +132,16 @@ fmx_status prepare_fast(fmx_ctx *ctx, fmx_spm *s, uint64_t stream_chunk)
             cnt += rd32(src->host_image() + src->offs[br] + 2ull * src->dir[t].x + 8);
         nzoff[br + 1] = nzoff[br] + cnt;
     }
-    std::vector<uint32_t> rows(s->nnz), cols(s->nnz);
-    std::vector<uint8_t> vals(s->esz * s->nnz);
+    // pinned, so that the uploads run at the host link's rate
+    struct Pinned {
+        void *p = nullptr;
+        ~Pinned() { if (p) (void)hipHostFree(p); }
+    } prow, pcol, pval;
+    FMX_HIP(ctx, hipHostMalloc(&prow.p, s->nnz * 4, hipHostMallocDefault));
+    FMX_HIP(ctx, hipHostMalloc(&pcol.p, s->nnz * 4, hipHostMallocDefault));
+    if (s->esz) FMX_HIP(ctx, hipHostMalloc(&pval.p, s->nnz * s->esz, hipHostMallocDefault));
+    uint32_t *rows = (uint32_t *)prow.p, *cols = (uint32_t *)pcol.p;
+    uint8_t *vals = (uint8_t *)pval.p;
     {
         std::vector<std::thread> th;
         for (unsigned t = 0; t < nth; t++)
@@ -137,7 +151,7 @@ fmx_status prepare_fast(fmx_ctx *ctx, fmx_spm *s, uint64_t stream_chunk)
                     for_nonzeros(src, br, [&](uint64_t r, uint64_t c, const uint8_t *v) {
                         rows[i] = (uint32_t)r;
                         cols[i] = newid[c];
-                        if (s->esz) memcpy(&vals[s->esz * i], v, s->esz);
+                        if (s->esz) memcpy(vals + s->esz * i, v, s->esz);
                         i++;
                     });
                 }
@@ -157,13 +171,10 @@ fmx_status prepare_fast(fmx_ctx *ctx, fmx_spm *s, uint64_t stream_chunk)
         return fail(ctx, FMX_ENOMEM, "gather-order image");
     }
     fmx_status st = FMX_OK;
-    if (hipMemcpy(d_rows, rows.data(), s->nnz * 4, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(d_cols, cols.data(), s->nnz * 4, hipMemcpyHostToDevice) != hipSuccess ||
-        (s->esz && hipMemcpy(d_vals, vals.data(), s->nnz * s->esz, hipMemcpyHostToDevice) != hipSuccess))
+    if (hipMemcpy(d_rows, rows, s->nnz * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_cols, cols, s->nnz * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        (s->esz && hipMemcpy(d_vals, vals, s->nnz * s->esz, hipMemcpyHostToDevice) != hipSuccess))
         st = fail(ctx, FMX_EHIP, "gather-order image upload");
-    std::vector<uint32_t>().swap(rows);
-    std::vector<uint32_t>().swap(cols);
-    std::vector<uint8_t>().swap(vals);
     fmx_spm *f = nullptr;
     if (!st) {
         const uint32_t lh = (uint32_t)std::min(s->log2h, 6), lw = 15;

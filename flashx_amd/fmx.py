@@ -40,7 +40,7 @@ class fmx_spm_info(ctypes.Structure):
     _fields_ = [("nrows", _u64), ("ncols", _u64), ("nnz", _u64), ("block_h", _u64),
                 ("block_w", _u64), ("nbr", _u64), ("image_bytes", _u64),
                 ("tile_bytes", _u64), ("entry_size", _u64), ("ntiles", _u64),
-                ("stream_chunks", _u64)]
+                ("stream_chunks", _u64), ("build_ns", _u64), ("edges_in", _u64), ("hot_columns", _u64)]
 
 
 class fmx_eigen_opts(ctypes.Structure):

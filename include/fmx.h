@@ -66,6 +66,11 @@ typedef struct {
     uint64_t ntiles;           /* stored tiles (including empty stubs)         */
     uint64_t stream_chunks;    /* 0: image in HBM; else block-row chunks the   */
                                /* host-resident image is streamed in per SpMM  */
+    uint64_t build_ns;         /* fmx_spm_build*: device time of the           */
+                               /* construction kernels (0 for other matrices)  */
+    uint64_t edges_in;         /* fmx_spm_build*: m, the edges given            */
+    uint64_t hot_columns;      /* columns in the gather-order image's hot block */
+                               /* (0: none)                                     */
 } fmx_spm_info;
 
 /* ---- context -------------------------------------------------------------
