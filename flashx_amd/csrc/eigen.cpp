@@ -84,6 +84,11 @@ private:
     Dev<double> dinv_, H_, S_, ev_, C1_, C2_, G_, Rinv_, info_, X64_, Y64_, Z64_, Ssel_, Scols_, theta_, rn_, part_;
     Dev<float> deg_;
     int spmm_blocks_ = 0, spmm64_blocks_ = 0;
+    int history_len_ = 0;
+    void record(double r)
+    {
+        if (o_.history && history_len_ < o_.history_cap) o_.history[history_len_++] = r;
+    }
 };
 
 template <typename T>
@@ -269,6 +274,7 @@ fmx_status Solver<T>::run(const void *X0, int64_t ldx0, fmx_dtype x0_dt, double 
         }
         if (est_worst > 2.0 * o_.tol && restarts < o_.max_restarts) {
             worst = est_worst;
+            record(est_worst);
             if (o_.verbose)
                 fprintf(stderr, "[fmx eigen] restart %d: spmm blocks %d, estimated max rel resid %.3e\n",
                         restarts, spmm_blocks_, est_worst);
@@ -293,6 +299,7 @@ fmx_status Solver<T>::run(const void *X0, int64_t ldx0, fmx_dtype x0_dt, double 
                 hres[i] /= std::max(std::fabs(th[i]), 1e-300);
                 worst = std::max(worst, hres[i]);
             }
+            record(worst);
             if (o_.verbose)
                 fprintf(stderr, "[fmx eigen] restart %d: spmm blocks %d, theta[0]=%.12f theta[nev-1]=%.12f "
                                 "max rel resid %.3e\n", restarts, spmm_blocks_, th[0], th[nev - 1], worst);
@@ -341,6 +348,7 @@ fmx_status Solver<T>::run(const void *X0, int64_t ldx0, fmx_dtype x0_dt, double 
         stats->converged = converged ? 1 : 0;
         stats->max_resid = worst;
         stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        stats->history_len = history_len_;
     }
     return FMX_OK;
 }
@@ -361,6 +369,8 @@ extern "C" fmx_status fmx_eigen_opts_init(fmx_eigen_opts *o, int nev)
     o->precision = FMX_EIG_FP32;
     o->seed = 1;
     o->verbose = 0;
+    o->history = nullptr;
+    o->history_cap = 0;
     return FMX_OK;
 }
 

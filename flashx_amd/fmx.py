@@ -46,13 +46,15 @@ class fmx_spm_info(ctypes.Structure):
 class fmx_eigen_opts(ctypes.Structure):
     _fields_ = [("nev", ctypes.c_int), ("block_size", ctypes.c_int), ("num_blocks", ctypes.c_int),
                 ("max_restarts", ctypes.c_int), ("tol", ctypes.c_double),
-                ("precision", ctypes.c_int), ("verbose", ctypes.c_int), ("seed", _u64)]
+                ("precision", ctypes.c_int), ("verbose", ctypes.c_int), ("seed", _u64),
+                ("history", ctypes.POINTER(ctypes.c_double)), ("history_cap", ctypes.c_int)]
 
 
 class fmx_eigen_stats(ctypes.Structure):
     _fields_ = [("spmm_blocks", ctypes.c_int), ("fp64_spmm_blocks", ctypes.c_int),
                 ("restarts", ctypes.c_int), ("converged", ctypes.c_int),
-                ("max_resid", ctypes.c_double), ("seconds", ctypes.c_double)]
+                ("max_resid", ctypes.c_double), ("seconds", ctypes.c_double),
+                ("history_len", ctypes.c_int)]
 
 
 # name -> (restype, argtypes); every symbol include/fmx.h declares.
@@ -527,16 +529,21 @@ EIG_MIXED, EIG_FP32, EIG_FP64 = 0, 1, 2
 
 
 def compute_eigen(spm, nev=16, block_size=16, num_blocks=6, tol=1e-6, max_restarts=100,
-                  precision=EIG_FP32, X0=None, seed=1, verbose=False):
+                  precision=EIG_FP32, X0=None, seed=1, verbose=False, history=False):
     """Top-nev eigenpairs of D^-1/2 A D^-1/2 (fm::eigen::compute_eigen with
     NA_eigen_Operator).  Returns (evals, evecs n x nev FP64 device tensor,
-    relative residuals, stats dict)."""
+    relative residuals, stats dict); with history=True the stats carry the
+    largest relative residual after each restart ("history")."""
     torch = _torch()
     ctx = spm.ctx
     o = fmx_eigen_opts()
     lib().fmx_eigen_opts_init(ctypes.byref(o), nev)
     o.block_size, o.num_blocks, o.tol = block_size, num_blocks, tol
     o.max_restarts, o.precision, o.seed, o.verbose = max_restarts, precision, seed, int(verbose)
+    hist = (ctypes.c_double * (max_restarts + 2))() if history else None
+    if hist is not None:
+        o.history = ctypes.cast(hist, ctypes.POINTER(ctypes.c_double))
+        o.history_cap = max_restarts + 2
     n = spm.get_num_rows()
     evals = np.zeros(nev, np.float64)
     resid = np.zeros(nev, np.float64)
@@ -553,6 +560,8 @@ def compute_eigen(spm, nev=16, block_size=16, num_blocks=6, tol=1e-6, max_restar
                                          resid.ctypes.data, ctypes.byref(st)),
               "fmx_eigen_normalized")
     stats = {f: getattr(st, f) for f, _ in fmx_eigen_stats._fields_}
+    if hist is not None:
+        stats["history"] = [hist[i] for i in range(st.history_len)]
     return evals, vecs, resid, stats
 
 

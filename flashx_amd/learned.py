@@ -14,6 +14,7 @@ random block (matrix/eigensolver/FM_MultiVector.h:164-168).
 
     python -m flashx_amd.learned train --graphs 200 --out flashx_amd/weights/warmstart.npy
     python -m flashx_amd.learned eval --weights flashx_amd/weights/warmstart.npy --graphs 20
+    python -m flashx_amd.learned analyse --graphs 3 --report gpurun_out/warmstart_analysis.json
 """
 import argparse
 import ctypes
@@ -221,9 +222,49 @@ def evaluate(net, count=20, n=1 << 20, seed0=900000, tol=1e-6, log=print):
     return summary, rows
 
 
+def analyse(net, count=3, n=1 << 20, seed0=900000, tol=1e-6, log=print):
+    """Why the learned start gains what it gains, per held-out graph: the
+    residual history of both solves (largest relative residual after each
+    restart), and for each of the 16 converged eigenvectors v_i how
+    localised it is (n * sum v_i^4: about 3 for a vector spread over the
+    graph, n / (sites) for one concentrated on a few vertices) and how much
+    of it the learned start spans (||Q^T v_i||^2 with Q an orthonormal basis
+    of the start; a random start spans about 16/n of any vector)."""
+    torch = _torch()
+    rows = []
+    for i in range(count):
+        g = Graph(n, seed0 + i)
+        g.warmup()
+        ev, V, res_r, st_r = F.compute_eigen(g.spm, nev=NEV, block_size=16, num_blocks=6, tol=tol, seed=seed0 + i,
+                                             precision=F.EIG_FP32, max_restarts=400, history=True)
+        Y = net.forward(g).double()
+        _, _, res_l, st_l = F.compute_eigen(g.spm, nev=NEV, block_size=16, num_blocks=6, tol=tol, seed=seed0 + i,
+                                            precision=F.EIG_FP32, max_restarts=400, history=True, X0=Y.float())
+        Q, _ = torch.linalg.qr(Y)
+        cap = (Q.T.contiguous() @ V).pow(2).sum(0).cpu().numpy()
+        ipr = (V.pow(4).sum(0) * n).cpu().numpy()
+        loc = ipr > 1000.0
+        row = {"seed": seed0 + i, "n": n, "eigenvalues": [round(float(x), 6) for x in ev],
+               "n_ipr": [round(float(x), 1) for x in ipr], "learned_capture": [round(float(x), 3) for x in cap],
+               "localised": int(loc.sum()),
+               "capture_localised": round(float(cap[loc].mean()), 3) if loc.any() else None,
+               "capture_spread": round(float(cap[~loc].mean()), 3) if (~loc).any() else None,
+               "subspace_loss_of_start": round(float(NEV - cap.sum()), 3),
+               "random": {"spmm_blocks": st_r["spmm_blocks"], "history": [float("%.3g" % x) for x in st_r["history"]]},
+               "learned": {"spmm_blocks": st_l["spmm_blocks"],
+                           "history": [float("%.3g" % x) for x in st_l["history"]]}}
+        log("  held-out %d: %d of 16 eigenvectors localised (n*IPR > 1000); the learned start spans %.2f of the "
+            "localised ones and %.2f of the spread ones on average; random %d blocks, learned %d" % (
+                i, row["localised"], row["capture_localised"] or 0, row["capture_spread"] or 0,
+                st_r["spmm_blocks"], st_l["spmm_blocks"]))
+        rows.append(row)
+        del g
+    return rows
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("cmd", choices=["train", "eval"])
+    ap.add_argument("cmd", choices=["train", "eval", "analyse"])
     ap.add_argument("--graphs", type=int, default=200)
     ap.add_argument("--epochs", type=int, default=40)
     ap.add_argument("--lr", type=float, default=2e-3)
@@ -245,6 +286,12 @@ def main(argv=None):
         if args.report:
             json.dump(rep, open(args.report, "w"), indent=1)
         log(json.dumps(rep))
+    elif args.cmd == "analyse":
+        net = WarmStartNet(params=np.load(args.weights))
+        rows = analyse(net, args.graphs, args.eval_n, log=log)
+        if args.report:
+            json.dump(rows, open(args.report, "w"), indent=1)
+        log(json.dumps(rows))
     else:
         net = WarmStartNet(params=np.load(args.weights))
         summary, rows = evaluate(net, args.graphs, args.eval_n, log=log)

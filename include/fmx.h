@@ -303,6 +303,9 @@ typedef struct {
     int precision;     /* FMX_EIG_*                                          */
     int verbose;
     uint64_t seed;     /* random start block, uniform [-1, 1)               */
+    double *history;   /* host array (or NULL): the largest relative        */
+    int history_cap;   /* residual after each restart, estimated from the   */
+                       /* Lanczos relation, or the FP64 one where checked   */
 } fmx_eigen_opts;
 
 typedef struct {
@@ -312,6 +315,7 @@ typedef struct {
     int converged;
     double max_resid;
     double seconds;
+    int history_len;       /* entries written to opts->history               */
 } fmx_eigen_stats;
 
 fmx_status fmx_eigen_opts_init(fmx_eigen_opts *opts, int nev);
