@@ -151,11 +151,11 @@ def training_sizes(count, lo=16, hi=20):
     return [1 << int(x) for x in np.sort(e)]
 
 
-def build_training_set(count, seed0=1000, log=print):
+def build_training_set(count, seed0=1000, log=print, lo=16, hi=20):
     torch = _torch()
     graphs = []
     t0 = time.time()
-    for i, n in enumerate(training_sizes(count)):
+    for i, n in enumerate(training_sizes(count, lo, hi)):
         g = Graph(n, seed0 + i)
         ev, vecs, res, st = g.solve(tol=1e-5, seed=seed0 + i)
         g.U = vecs.to(torch.float32).contiguous()
@@ -268,6 +268,9 @@ def main(argv=None):
     ap.add_argument("--graphs", type=int, default=200)
     ap.add_argument("--epochs", type=int, default=40)
     ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--halve-every", type=int, default=10, help="epochs between learning-rate halvings")
+    ap.add_argument("--min-log2", type=int, default=16, help="smallest training graph, log2 of its vertices")
+    ap.add_argument("--max-log2", type=int, default=20, help="largest training graph, log2 of its vertices")
     ap.add_argument("--weights", default=DEFAULT_WEIGHTS)
     ap.add_argument("--out", default=DEFAULT_WEIGHTS)
     ap.add_argument("--eval-n", type=int, default=1 << 20)
@@ -276,12 +279,13 @@ def main(argv=None):
     log = lambda *a: print(*a, flush=True)
     if args.cmd == "train":
         t0 = time.time()
-        graphs = build_training_set(args.graphs, log=log)
+        graphs = build_training_set(args.graphs, log=log, lo=args.min_log2, hi=args.max_log2)
         log("targets ready in %.1fs" % (time.time() - t0))
-        net, hist = train(graphs, args.epochs, args.lr, log=log)
+        net, hist = train(graphs, args.epochs, args.lr, log=log, halve_every=args.halve_every)
         os.makedirs(os.path.dirname(args.out), exist_ok=True)
         np.save(args.out, net.params())
-        rep = {"graphs": args.graphs, "epochs": args.epochs, "loss_history": hist,
+        rep = {"graphs": args.graphs, "epochs": args.epochs, "sizes_log2": [args.min_log2, args.max_log2],
+               "halve_every": args.halve_every, "lr": args.lr, "loss_history": hist,
                "seconds": time.time() - t0}
         if args.report:
             json.dump(rep, open(args.report, "w"), indent=1)
