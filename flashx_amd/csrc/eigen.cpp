@@ -1,17 +1,25 @@
-// eigen.cpp -- block Krylov eigensolver for the normalised adjacency
-// N = D^-1/2 A D^-1/2 of a symmetric binary matrix, replacing
+// eigen.cpp -- block Krylov eigensolver for the three operators of the
+// reference's eigensolver driver (matrix/test/test-eigen.cpp), replacing
 // fm::eigen::compute_eigen with Anasazi's BlockKrylovSchur
-// (matrix/eigensolver/eigensolver.cpp:118-308) over NA_eigen_Operator
-// (matrix/test/test-eigen.cpp:62-136).
+// (matrix/eigensolver/eigensolver.cpp:118-308):
+//   NORMALIZED  N = D^-1/2 A D^-1/2 of a symmetric binary matrix
+//               (NA_eigen_Operator, test-eigen.cpp:62-136)
+//   PLAIN       A itself, symmetric (eigen_Operator, test-eigen.cpp:32-58)
+//   SVD         A^T A, x -> A^T (A x) through the matrix and its transpose
+//               (SVD_Operator, test-eigen.cpp:138-177); the eigenvalues are
+//               the squared singular values of A, the vectors its right
+//               singular vectors
 //
 // Method: thick-restart block Lanczos with full reorthogonalisation, the
 // symmetric form of block Krylov-Schur.  The basis V (n x m, m = blocks *
 // block_size) lives in HBM; the projected matrix H = V^T N V (m x m, FP64) is
 // assembled from the projection coefficients, so after a restart it simply
 // becomes diag(kept Ritz values) and fills again as the next blocks are
-// expanded.  Each expansion:
-//   W = N V_j                              (row scaling, then the SpMM with the
-//                                           outer scaling applied to its sums)
+// expanded.  Each expansion (N the operator):
+//   W = N V_j                              (NORMALIZED: row scaling, then the
+//                                           SpMM with the outer scaling
+//                                           applied to its sums; SVD: two
+//                                           SpMMs through an m_rows x b block)
 //   C1 = V^T W, W -= V C1                  (first pass; BF16 MFMA Gram in the
 //                                           mixed mode, FP64 otherwise)
 //   C2 = V^T W, W -= V C2                  (FP64 reorthogonalisation)
@@ -20,7 +28,7 @@
 // When the basis is full: Rayleigh-Ritz by FP64 Jacobi on the device, and
 // residual estimates ||W s_last|| from the Lanczos relation N V = V H + W E^T.
 // Once every wanted estimate is within twice the tolerance: Ritz vectors
-// X = V S in FP64 and residuals ||N X - X Theta|| with an FP64 SpMM.
+// X = V S in FP64 and residuals ||N X - X Theta|| with FP64 SpMMs.
 // Converged when every wanted pair has ||N x - theta x|| / |theta| <= tol in
 // that FP64 check.
 //
@@ -56,8 +64,9 @@ struct Dev {
 template <typename T>
 class Solver {
 public:
-    Solver(fmx_ctx *ctx, fmx_spm *spm, const fmx_eigen_opts &o, hipStream_t s)
-        : ctx_(ctx), spm_(spm), o_(o), s_(s), n_(spm->nrows), b_(o.block_size),
+    Solver(fmx_ctx *ctx, int op, fmx_spm *spm, fmx_spm *spm_t, const fmx_eigen_opts &o, hipStream_t s)
+        : ctx_(ctx), op_(op), spm_(spm), spm_t_(spm_t), o_(o), s_(s),
+          n_(op == FMX_EIG_OP_SVD ? spm->ncols : spm->nrows), nmid_(spm->nrows), b_(o.block_size),
           m_(o.block_size * o.num_blocks)
     {
     }
@@ -74,14 +83,24 @@ private:
     fmx_status orth_block(T *W, int cols, int dst_col);
     fmx_status sync() { FMX_HIP(ctx_, hipStreamSynchronize(s_)); return FMX_OK; }
 
+    // one SpMM of the operator's matrix (or its transpose) on FP32 or FP64
+    // data; FP64 data over FP32 values is spmm_creator<double, float>
+    fmx_status spmm(const fmx_spm *a, fmx_dtype dt, int k, const void *X, int64_t ldx, void *Y, int64_t ldy,
+                    const double *post)
+    {
+        return spmm_exec(ctx_, a, dt, k, nullptr, post, X, ldx, Y, ldy, 0, dt == FMX_F64, s_);
+    }
+
     fmx_ctx *ctx_;
-    fmx_spm *spm_;
+    int op_;
+    fmx_spm *spm_, *spm_t_;
     fmx_eigen_opts o_;
     hipStream_t s_;
-    uint64_t n_;
+    uint64_t n_, nmid_;  // basis rows; rows of A (the SVD's middle product)
     int b_, m_;
-    Dev<T> V_, W_, T1_;
-    Dev<double> dinv_, H_, S_, ev_, C1_, C2_, G_, Rinv_, info_, X64_, Y64_, Z64_, Ssel_, Scols_, theta_, rn_, part_;
+    Dev<T> V_, W_, T1_, T2_;
+    Dev<double> dinv_, H_, S_, ev_, C1_, C2_, G_, Rinv_, info_, X64_, Y64_, Z64_, M64_, Ssel_, Scols_, theta_, rn_,
+        part_;
     Dev<float> deg_;
     int spmm_blocks_ = 0, spmm64_blocks_ = 0;
     int history_len_ = 0;
@@ -116,29 +135,52 @@ fmx_status Solver<T>::alloc()
     FMX_HIP(ctx_, hipMalloc(&theta_.p, (size_t)m_ * sizeof(double)));
     FMX_HIP(ctx_, hipMalloc(&rn_.p, (size_t)m_ * sizeof(double)));
     FMX_HIP(ctx_, hipMalloc(&part_.p, (size_t)1024 * 256 * sizeof(double)));
+    if (op_ == FMX_EIG_OP_SVD) {
+        const uint64_t nm = nmid_ ? nmid_ : 1;
+        FMX_HIP(ctx_, hipMalloc(&T2_.p, nm * b_ * sizeof(T)));
+        FMX_HIP(ctx_, hipMalloc(&M64_.p, nm * o_.nev * sizeof(double)));
+    }
     return FMX_OK;
 }
 
-// Y = D^-1/2 A D^-1/2 X for one block (NA_eigen_Operator::run)
+// Y = N X for one block: NA_eigen_Operator::run, eigen_Operator::run or
+// SVD_Operator::run
 template <typename T>
 fmx_status Solver<T>::apply(const T *X, int64_t ldx, T *Y, int64_t ldy)
 {
-    // D^-1/2 X, then A (D^-1/2 X) with the outer D^-1/2 applied to each
-    // row's FP64 sum before its rounding
-    TRY(row_scale(ctx_, dt_of<T>(), n_, b_, dinv_.p, X, ldx, T1_.p, b_, s_));
-    TRY(spmm_scaled(ctx_, spm_, dt_of<T>(), b_, T1_.p, b_, Y, ldy, dinv_.p, s_));
+    if (op_ == FMX_EIG_OP_NORMALIZED) {
+        // D^-1/2 X, then A (D^-1/2 X) with the outer D^-1/2 applied to each
+        // row's FP64 sum before its rounding
+        TRY(row_scale(ctx_, dt_of<T>(), n_, b_, dinv_.p, X, ldx, T1_.p, b_, s_));
+        TRY(spmm(spm_, dt_of<T>(), b_, T1_.p, b_, Y, ldy, dinv_.p));
+    } else if (op_ == FMX_EIG_OP_PLAIN) {
+        TRY(spmm(spm_, dt_of<T>(), b_, X, ldx, Y, ldy, nullptr));
+    } else {
+        // A X (rows of A), then A^T (A X)
+        TRY(spmm(spm_, dt_of<T>(), b_, X, ldx, T2_.p, b_, nullptr));
+        TRY(spmm(spm_t_, dt_of<T>(), b_, T2_.p, b_, Y, ldy, nullptr));
+        spmm_blocks_++;
+    }
     spmm_blocks_++;
     return FMX_OK;
 }
 
 // FP64 operator on k <= nev columns (residual check) into Y, through the
-// persistent Z64_ (the scaled input)
+// persistent Z64_ (the scaled input) and M64_ (the SVD's A X)
 template <typename T>
 fmx_status Solver<T>::apply64(const double *X, double *Y, int k)
 {
     if (k > o_.nev) return fail(ctx_, FMX_EDIM, "residual block wider than nev");
-    TRY(row_scale(ctx_, FMX_F64, n_, k, dinv_.p, X, k, Z64_.p, k, s_));
-    TRY(spmm_scaled(ctx_, spm_, FMX_F64, k, Z64_.p, k, Y, k, dinv_.p, s_));
+    if (op_ == FMX_EIG_OP_NORMALIZED) {
+        TRY(row_scale(ctx_, FMX_F64, n_, k, dinv_.p, X, k, Z64_.p, k, s_));
+        TRY(spmm(spm_, FMX_F64, k, Z64_.p, k, Y, k, dinv_.p));
+    } else if (op_ == FMX_EIG_OP_PLAIN) {
+        TRY(spmm(spm_, FMX_F64, k, X, k, Y, k, nullptr));
+    } else {
+        TRY(spmm(spm_, FMX_F64, k, X, k, M64_.p, k, nullptr));
+        TRY(spmm(spm_t_, FMX_F64, k, M64_.p, k, Y, k, nullptr));
+        spmm64_blocks_++;
+    }
     spmm64_blocks_++;
     return FMX_OK;
 }
@@ -200,7 +242,7 @@ fmx_status Solver<T>::run(const void *X0, int64_t ldx0, fmx_dtype x0_dt, double 
     TRY(alloc());
     const int nev = o_.nev, b = b_, m = m_;
     // degrees and D^-1/2 (NA_eigen_Operator ctor, test-eigen.cpp:93-110)
-    {
+    if (op_ == FMX_EIG_OP_NORMALIZED) {
         Dev<float> ones;
         FMX_HIP(ctx_, hipMalloc(&ones.p, (n_ ? n_ : 1) * sizeof(float)));
         std::vector<float> h1(n_, 1.0f);
@@ -374,15 +416,28 @@ extern "C" fmx_status fmx_eigen_opts_init(fmx_eigen_opts *o, int nev)
     return FMX_OK;
 }
 
-extern "C" fmx_status fmx_eigen_normalized(fmx_ctx *ctx, const fmx_spm *cspm, const fmx_eigen_opts *o,
-                                           const void *X0, int64_t ldx0, fmx_dtype x0_dtype,
-                                           double *evals, double *evecs, int64_t ldv, double *resid,
-                                           fmx_eigen_stats *stats)
+namespace {
+
+fmx_status eigen_run(fmx_ctx *ctx, int op, const fmx_spm *ca, const fmx_spm *cat, const fmx_eigen_opts *o,
+                     const void *X0, int64_t ldx0, fmx_dtype x0_dtype, double *evals, double *evecs, int64_t ldv,
+                     double *resid, fmx_eigen_stats *stats)
 {
-    if (!ctx || !cspm || !o || !evals) return fail(ctx, FMX_EINVAL, "null argument");
-    fmx_spm *spm = const_cast<fmx_spm *>(cspm);
-    if (spm->ctx != ctx) return fail(ctx, FMX_EINVAL, "matrix belongs to another context");
-    if (spm->nrows != spm->ncols) return fail(ctx, FMX_EDIM, "the operator needs a square matrix");
+    if (!ctx || !ca || !o || !evals) return fail(ctx, FMX_EINVAL, "null argument");
+    fmx_spm *a = const_cast<fmx_spm *>(ca), *at = const_cast<fmx_spm *>(cat);
+    if (a->ctx != ctx || (at && at->ctx != ctx)) return fail(ctx, FMX_EINVAL, "matrix belongs to another context");
+    if (op == FMX_EIG_OP_SVD) {
+        // a symmetric matrix is its own transpose (sparse_matrix::transpose)
+        if (!at && a->nrows != a->ncols)
+            return fail(ctx, FMX_EINVAL, "the SVD operator of a non-square matrix needs its transpose");
+        if (!at) at = a;
+        if (at->nrows != a->ncols || at->ncols != a->nrows)
+            return fail(ctx, FMX_EDIM, "the transpose's shape does not match the matrix");
+    } else if (op == FMX_EIG_OP_NORMALIZED || op == FMX_EIG_OP_PLAIN) {
+        if (at) return fail(ctx, FMX_EINVAL, "only the SVD operator takes a transpose");
+        if (a->nrows != a->ncols) return fail(ctx, FMX_EDIM, "the operator needs a square matrix");
+    } else {
+        return fail(ctx, FMX_EINVAL, "unknown eigen operator");
+    }
     if (o->nev <= 0 || o->block_size <= 0 || o->block_size > 32 || o->num_blocks < 2)
         return fail(ctx, FMX_EINVAL, "bad eigen options");
     const int m = o->block_size * o->num_blocks;
@@ -397,11 +452,29 @@ extern "C" fmx_status fmx_eigen_normalized(fmx_ctx *ctx, const fmx_spm *cspm, co
     FMX_HIP(ctx, hipDeviceSynchronize());
     hipStream_t s = ctx->stream;
     if (o->precision == FMX_EIG_FP64) {
-        Solver<double> sv(ctx, spm, *o, s);
+        Solver<double> sv(ctx, op, a, at, *o, s);
         return sv.run(X0, ldx0, x0_dtype, evals, evecs, ldv, resid, stats);
     }
     if (o->precision != FMX_EIG_MIXED && o->precision != FMX_EIG_FP32)
         return fail(ctx, FMX_EINVAL, "unknown precision");
-    Solver<float> sv(ctx, spm, *o, s);
+    Solver<float> sv(ctx, op, a, at, *o, s);
     return sv.run(X0, ldx0, x0_dtype, evals, evecs, ldv, resid, stats);
+}
+
+}  // namespace
+
+extern "C" fmx_status fmx_eigen(fmx_ctx *ctx, int op, const fmx_spm *A, const fmx_spm *At,
+                                const fmx_eigen_opts *o, const void *X0, int64_t ldx0, fmx_dtype x0_dtype,
+                                double *evals, double *evecs, int64_t ldv, double *resid, fmx_eigen_stats *stats)
+{
+    return eigen_run(ctx, op, A, At, o, X0, ldx0, x0_dtype, evals, evecs, ldv, resid, stats);
+}
+
+extern "C" fmx_status fmx_eigen_normalized(fmx_ctx *ctx, const fmx_spm *cspm, const fmx_eigen_opts *o,
+                                           const void *X0, int64_t ldx0, fmx_dtype x0_dtype,
+                                           double *evals, double *evecs, int64_t ldv, double *resid,
+                                           fmx_eigen_stats *stats)
+{
+    return eigen_run(ctx, FMX_EIG_OP_NORMALIZED, cspm, nullptr, o, X0, ldx0, x0_dtype, evals, evecs, ldv, resid,
+                     stats);
 }

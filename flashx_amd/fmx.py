@@ -108,6 +108,8 @@ SIGNATURES = {
     "fmx_eigen_normalized": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(fmx_eigen_opts), _vp, _i64,
                                             ctypes.c_int, _vp, _vp, _i64, _vp,
                                             ctypes.POINTER(fmx_eigen_stats)]),
+    "fmx_eigen": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, ctypes.POINTER(fmx_eigen_opts), _vp, _i64,
+                                 ctypes.c_int, _vp, _vp, _i64, _vp, ctypes.POINTER(fmx_eigen_stats)]),
     "fmx_gnn_param_count": (_u64, []),
     "fmx_gnn_acts_floats": (_u64, [_u64]),
     "fmx_gnn_work_floats": (_u64, [_u64]),
@@ -526,14 +528,20 @@ def row_scale(d, X, Y=None, ctx=None):
 
 
 EIG_MIXED, EIG_FP32, EIG_FP64 = 0, 1, 2
+EIG_OP_NORMALIZED, EIG_OP_PLAIN, EIG_OP_SVD = 0, 1, 2
+_EIG_OPS = {"normalized": EIG_OP_NORMALIZED, "plain": EIG_OP_PLAIN, "svd": EIG_OP_SVD}
 
 
 def compute_eigen(spm, nev=16, block_size=16, num_blocks=6, tol=1e-6, max_restarts=100,
-                  precision=EIG_FP32, X0=None, seed=1, verbose=False, history=False):
-    """Top-nev eigenpairs of D^-1/2 A D^-1/2 (fm::eigen::compute_eigen with
-    NA_eigen_Operator).  Returns (evals, evecs n x nev FP64 device tensor,
-    relative residuals, stats dict); with history=True the stats carry the
-    largest relative residual after each restart ("history")."""
+                  precision=EIG_FP32, X0=None, seed=1, verbose=False, history=False, op="normalized", At=None):
+    """Top-nev eigenpairs of an operator of spm (fm::eigen::compute_eigen):
+    op "normalized" is D^-1/2 A D^-1/2 (NA_eigen_Operator), "plain" A itself
+    (eigen_Operator), "svd" A^T A (SVD_Operator; At is A's transpose, None
+    for a symmetric A; the eigenvalues are the squared singular values and
+    the vectors, ncols x nev, the right singular vectors).  Returns (evals,
+    evecs n x nev FP64 device tensor, relative residuals, stats dict); with
+    history=True the stats carry the largest relative residual after each
+    restart ("history")."""
     torch = _torch()
     ctx = spm.ctx
     o = fmx_eigen_opts()
@@ -544,7 +552,9 @@ def compute_eigen(spm, nev=16, block_size=16, num_blocks=6, tol=1e-6, max_restar
     if hist is not None:
         o.history = ctypes.cast(hist, ctypes.POINTER(ctypes.c_double))
         o.history_cap = max_restarts + 2
-    n = spm.get_num_rows()
+    if op not in _EIG_OPS:
+        raise FmxError(EINVAL, "unknown eigen operator %r" % (op,))
+    n = spm.get_num_cols() if op == "svd" else spm.get_num_rows()
     evals = np.zeros(nev, np.float64)
     resid = np.zeros(nev, np.float64)
     vecs = torch.empty((n, nev), dtype=torch.float64, device="cuda:%d" % ctx.device)
@@ -555,10 +565,10 @@ def compute_eigen(spm, nev=16, block_size=16, num_blocks=6, tol=1e-6, max_restar
             raise FmxError(EDIM, "warm start must be n x block_size row-major")
         x0p, ldx0, x0dt = X0.data_ptr(), X0.stride(0), dtype_code(X0.dtype)
     torch.cuda.synchronize()
-    ctx.check(lib().fmx_eigen_normalized(ctx.handle, spm.handle, ctypes.byref(o), x0p, ldx0, x0dt,
-                                         evals.ctypes.data, vecs.data_ptr(), nev,
-                                         resid.ctypes.data, ctypes.byref(st)),
-              "fmx_eigen_normalized")
+    ctx.check(lib().fmx_eigen(ctx.handle, _EIG_OPS[op], spm.handle, At.handle if At is not None else None,
+                              ctypes.byref(o), x0p, ldx0, x0dt, evals.ctypes.data, vecs.data_ptr(), nev,
+                              resid.ctypes.data, ctypes.byref(st)),
+              "fmx_eigen")
     stats = {f: getattr(st, f) for f, _ in fmx_eigen_stats._fields_}
     if hist is not None:
         stats["history"] = [hist[i] for i in range(st.history_len)]

@@ -309,8 +309,8 @@ typedef struct {
 } fmx_eigen_opts;
 
 typedef struct {
-    int spmm_blocks;       /* block applications of the operator            */
-    int fp64_spmm_blocks;  /* FP64 residual applications                    */
+    int spmm_blocks;       /* block SpMMs (two per application of A^T A)     */
+    int fp64_spmm_blocks;  /* FP64 block SpMMs of the residual checks        */
     int restarts;
     int converged;
     double max_resid;
@@ -331,6 +331,26 @@ fmx_status fmx_eigen_normalized(fmx_ctx *ctx, const fmx_spm *spm, const fmx_eige
                                 const void *X0, int64_t ldx0, fmx_dtype x0_dtype,
                                 double *evals, double *evecs, int64_t ldv, double *resid,
                                 fmx_eigen_stats *stats);
+
+/* The operators of the reference's eigensolver driver
+ * (matrix/test/test-eigen.cpp:32-177), each solved as above: */
+#define FMX_EIG_OP_NORMALIZED 0  /* D^-1/2 A D^-1/2, A symmetric (NA_eigen_Operator)  */
+#define FMX_EIG_OP_PLAIN      1  /* A, symmetric (eigen_Operator, :32-58)               */
+#define FMX_EIG_OP_SVD        2  /* A^T A, x -> A^T (A x) (SVD_Operator, :138-177)      */
+
+/* compute_eigen(op, true, opts) (matrix/eigensolver/eigensolver.cpp:118-308)
+ * for op = FMX_EIG_OP_*.  For the SVD operator A is nrows x ncols and At its
+ * transpose (built from the swapped edge list, or loaded from the
+ * reference's transpose image; NULL when A is symmetric, which is its own
+ * transpose); evals are then the squared singular values of A, descending,
+ * and evecs (ncols x nev) the right singular vectors.  The other operators
+ * take At = NULL.  FP32 and mixed precision run the operator's SpMMs on FP32
+ * data (a weighted matrix then needs FP32 values); the residual check runs
+ * them on FP64 data. */
+fmx_status fmx_eigen(fmx_ctx *ctx, int op, const fmx_spm *A, const fmx_spm *At, const fmx_eigen_opts *opts,
+                     const void *X0, int64_t ldx0, fmx_dtype x0_dtype,
+                     double *evals, double *evecs, int64_t ldv, double *resid,
+                     fmx_eigen_stats *stats);
 
 /* ---- SpMV callers ------------------------------------------------------------
  * PageRank as matrix/libmatrix-algs/PageRank.cpp:49-137 computes it (FP32):

@@ -11,6 +11,7 @@
 #include <unistd.h>
 #include <vector>
 
+#include "fm/eigensolver.h"
 #include "fm/sparse_matrix.h"
 
 int main()
@@ -192,6 +193,78 @@ int main()
         if (ys != yr) return 1;  // 0.5 * A * (2 X) = A X exactly
         hipFree(d_half);
         hipFree(d_two);
+    }
+    // the eigensolver front end (matrix/eigensolver/eigensolver.h) with the
+    // reference driver's operators (matrix/test/test-eigen.cpp): each pair
+    // returned satisfies ||run(x) - theta x|| <= tol |theta| under the
+    // operator's own FP64 run(), and the top eigenvalue of D^-1/2 S D^-1/2
+    // is 1.  The graph: vertex i has 1 + i % 8 out-edges, to targets skewed
+    // towards low ids, so that the top eigenvalues are well separated (the
+    // graph above is nearly regular: its top eigenvalues of A^T A agree to
+    // 1e-6)
+    {
+        std::vector<uint32_t> s2, d2;
+        auto h = [](uint64_t x) { return (uint32_t)((x * 2654435761ull) & 0xFFFFFFFFull); };
+        for (uint32_t i = 0; i < n; i++)
+            for (uint32_t j = 0; j <= i % 8; j++) {
+                const uint64_t x = (uint64_t)i * 16 + j;
+                s2.push_back(i);
+                d2.push_back(h(x) % (1 + h(x + 7) % n));
+            }
+        uint32_t *ds2, *dd2;
+        if (hipMalloc(&ds2, s2.size() * 4) != hipSuccess || hipMalloc(&dd2, d2.size() * 4) != hipSuccess) return 1;
+        if (hipMemcpy(ds2, s2.data(), s2.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(dd2, d2.data(), d2.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
+            return 1;
+        fm::sparse_matrix::ptr A = fm::sparse_matrix::create_2d_matrix(ds2, dd2, s2.size(),
+                                                                       fm::block_2d_size(256, 1024), false, n);
+        fm::sparse_matrix::ptr S = fm::sparse_matrix::create_2d_matrix(
+            ds2, dd2, s2.size(), fm::block_2d_size(256, 1024), true, n, FMX_BUILD_DEDUP | FMX_BUILD_SYMMETRIZE);
+        (void)hipFree(ds2);
+        (void)hipFree(dd2);
+        if (!A || !A->transpose() || !S || !S->is_symmetric()) return 1;
+        const int nev = 4;
+        const double tol = 1e-8;
+        auto check = [&](fm::eigen::spm_function *op, const fm::eigen::spm_function &again, const char *name) {
+            fm::eigen::eigen_options opts;
+            if (!opts.init(nev)) return false;
+            opts.tol = tol;
+            opts.precision = FMX_EIG_FP64;
+            fm::eigen::eigen_res res = fm::eigen::compute_eigen(op, true, opts);
+            if (!res.converged || (int)res.vals.size() != nev || !res.vecs) {
+                printf("%s: no convergence\n", name);
+                return false;
+            }
+            const size_t rows = res.vecs->get_num_rows();
+            std::vector<double> v(rows * nev), y(rows * nev);
+            res.vecs->copy_to_host(v.data());
+            fm::dense_matrix::ptr x = fm::dense_matrix::create_from_host(v.data(), rows, nev, FMX_F64);
+            fm::dense_matrix::ptr ny = again.run(x);
+            if (!ny || x) return false;  // run() releases its input, as the reference's does
+            ny->copy_to_host(y.data());
+            for (int i = 0; i < nev; i++) {
+                double r = 0.0;
+                for (size_t j = 0; j < rows; j++) {
+                    const double d = y[j * nev + i] - res.vals[i] * v[j * nev + i];
+                    r += d * d;
+                }
+                if (std::sqrt(r) > 1.01 * tol * std::fabs(res.vals[i]) || (i && res.vals[i] > res.vals[i - 1])) {
+                    printf("%s: pair %d residual %.3e, value %.12f\n", name, i, std::sqrt(r) / std::fabs(res.vals[i]),
+                           res.vals[i]);
+                    return false;
+                }
+            }
+            if (std::string(name) == "NA_eigen" && std::fabs(res.vals[0] - 1.0) > 1e-8) return false;
+            return true;
+        };
+        if (!check(new fm::eigen::NA_eigen_Operator(S), fm::eigen::NA_eigen_Operator(S), "NA_eigen")) return 1;
+        if (!check(new fm::eigen::eigen_Operator(S), fm::eigen::eigen_Operator(S), "eigen")) return 1;
+        if (!check(new fm::eigen::SVD_Operator(A), fm::eigen::SVD_Operator(A), "SVD")) return 1;
+        if (!check(new fm::eigen::SVD_Operator(S), fm::eigen::SVD_Operator(S), "SVD (symmetric)")) return 1;
+        fm::eigen::eigen_options lm;
+        lm.init(nev);
+        lm.which = "LM";
+        if (!fm::eigen::compute_eigen(new fm::eigen::eigen_Operator(S), true, lm).vals.empty()) return 1;
     }
     // error convention: a dimension mismatch is false + log, not an exception
     fm::dense_matrix::ptr bad = fm::dense_matrix::create(n + 1, 1, FMX_I64);
