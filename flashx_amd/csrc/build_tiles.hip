@@ -242,6 +242,43 @@ __global__ void __launch_bounds__(kB) first_tile_of_row(const uint64_t *k, KeyFm
     }
 }
 
+// The tile directory the SpMM plans and kernels read, straight from the
+// construction (a resident image built here is never copied to the host):
+// per block row its first directory index (bdir, exclusive counts: its
+// tiles, or 1 for the empty tile of an empty block row), per directory entry
+// the tile's word offset inside its block row and its block column, and for
+// weighted images the word where the tile's values start.
+__global__ void __launch_bounds__(kB) dir_counts(const uint64_t *k, KeyFmt f, const uint32_t *run_start,
+                                                 const uint32_t *tile_first_run, uint32_t T, const uint8_t *had,
+                                                 uint64_t nbr, uint32_t *cnt)
+{
+    for (uint64_t t = gtid(); t < T; t += gstride()) atomicAdd(&cnt[f.br(k[run_start[tile_first_run[t]]])], 1u);
+    for (uint64_t b = gtid(); b < nbr; b += gstride())
+        if (!had[b]) cnt[b] = 1;
+}
+
+__global__ void __launch_bounds__(kB) fill_dir(const uint64_t *k, KeyFmt f, const uint32_t *run_start,
+                                               const uint32_t *tile_first_run, uint32_t T, const uint64_t *offs_ex,
+                                               const uint32_t *br_first, const uint64_t *tile_off,
+                                               const uint32_t *bdir, const uint8_t *had, uint64_t nbr, int weighted,
+                                               uint2 *dir, uint32_t *vdir)
+{
+    for (uint64_t t = gtid(); t < T; t += gstride()) {
+        const uint32_t fr = tile_first_run[t], lr = tile_first_run[t + 1];
+        const uint64_t key0 = k[run_start[fr]];
+        const uint32_t b = f.br(key0);
+        const uint32_t x = (uint32_t)((tile_off[t] - 4096 - offs_ex[b]) / 2);
+        const uint32_t i = bdir[b] + (uint32_t)(t - br_first[b]);
+        dir[i] = make_uint2(x, f.bc(key0));
+        if (weighted) vdir[i] = x + 8 + (lr - fr) + (run_start[lr] - run_start[fr]) + 1;
+    }
+    for (uint64_t b = gtid(); b < nbr; b += gstride())
+        if (!had[b]) {
+            dir[bdir[b]] = make_uint2(0, 0);
+            if (weighted) vdir[bdir[b]] = 9;  // the empty tile: 8 header words, no rows, no entries, end marker
+        }
+}
+
 __device__ __forceinline__ void put16(uint8_t *img, uint64_t off, uint32_t v)
 {
     *(uint16_t *)(img + off) = (uint16_t)v;
@@ -346,7 +383,6 @@ uint32_t grid_for(fmx_ctx *ctx, uint64_t n)
 
 }  // namespace
 
-fmx_status finish_device_image(fmx_ctx *ctx, fmx_spm *s, bool internal);
 
 }  // namespace fmx
 
@@ -583,9 +619,35 @@ fmx_status build_impl(fmx_ctx *ctx, const uint32_t *src, const uint32_t *dst, si
     hipLaunchKernelGGL(write_stubs, dim3(grid_for(ctx, nbr)), dim3(kB), 0, s, had.p, nbr, offs_ex.p,
                        spm->d_image);
     FMX_HIP(ctx, hipGetLastError());
+
+    // 6. the directory, in HBM where the SpMM reads it and on the host for
+    // the plans
+    DevBuf<uint32_t> bdir;
+    FMX_HIP(ctx, bdir.alloc(nbr + 1));
+    FMX_HIP(ctx, hipMemsetAsync(bdir.p, 0, (nbr + 1) * sizeof(uint32_t), s));
+    hipLaunchKernelGGL(dir_counts, dim3(grid_for(ctx, std::max<uint64_t>(T, nbr))), dim3(kB), 0, s, K, f,
+                       run_start.p, tile_first_run.p, T, had.p, nbr, bdir.p);
+    FMX_HIP(ctx, hipGetLastError());
+    uint32_t ndir = 0;
+    st = scan_exclusive<uint32_t>(ctx, bdir.p, bdir.p, nbr + 1, &ndir, s);
+    if (st) return st;
+    FMX_HIP(ctx, hipMalloc(&spm->d_dir, std::max<uint64_t>(1, ndir) * sizeof(uint2)));
+    if (vals) FMX_HIP(ctx, hipMalloc(&spm->d_vdir, std::max<uint64_t>(1, ndir) * sizeof(uint32_t)));
+    hipLaunchKernelGGL(fill_dir, dim3(grid_for(ctx, std::max<uint64_t>(T, nbr))), dim3(kB), 0, s, K, f,
+                       run_start.p, tile_first_run.p, T, offs_ex.p, br_first.p, tile_off.p, bdir.p, had.p, nbr,
+                       vals ? 1 : 0, spm->d_dir, spm->d_vdir);
+    FMX_HIP(ctx, hipGetLastError());
     FMX_HIP(ctx, hipEventRecord(ev.b, s));
     std::vector<uint64_t> oex(nbr + 1);
+    spm->brow_dir.resize(nbr + 1);
+    spm->dir.resize(ndir);
+    if (vals) spm->vdir.resize(ndir);
     FMX_HIP(ctx, hipMemcpyAsync(oex.data(), offs_ex.p, (nbr + 1) * 8, hipMemcpyDeviceToHost, s));
+    FMX_HIP(ctx, hipMemcpyAsync(spm->brow_dir.data(), bdir.p, (nbr + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    FMX_HIP(ctx, hipMemcpyAsync(spm->dir.data(), spm->d_dir, (size_t)ndir * sizeof(uint2), hipMemcpyDeviceToHost, s));
+    if (vals)
+        FMX_HIP(ctx, hipMemcpyAsync(spm->vdir.data(), spm->d_vdir, (size_t)ndir * sizeof(uint32_t),
+                                    hipMemcpyDeviceToHost, s));
     FMX_HIP(ctx, hipStreamSynchronize(s));
     float build_ms = 0.f;
     FMX_HIP(ctx, hipEventElapsedTime(&build_ms, ev.a, ev.b));
@@ -594,8 +656,25 @@ fmx_status build_impl(fmx_ctx *ctx, const uint32_t *src, const uint32_t *dst, si
     spm->offs.resize(nbr + 1);
     for (uint64_t b = 0; b <= nbr; b++) spm->offs[b] = (int64_t)(4096 + oex[b]);
     spm->bytes = bytes;
-    st = finish_device_image(ctx, spm.get(), internal);
+    spm->nrows = n;
+    spm->ncols = ncols;
+    spm->nnz = nk;
+    spm->H = H;
+    spm->W = W;
+    spm->log2h = (int)log2_h;
+    spm->log2w = (int)log2_w;
+    spm->nbr = nbr;
+    spm->nbc = nbc;
+    spm->esz = vals ? esz : 0;
+    spm->prim = vals ? prim_type : 0;
+    spm->ntiles = ndir;
+    st = check_device_dir(ctx, spm.get());
     if (st) return st;
+    if (!internal) {
+        st = prepare_work(ctx, spm.get());
+        if (!st) st = prepare_fast(ctx, spm.get());
+        if (st) return st;
+    }
     *out = spm.release();
     return FMX_OK;
 }

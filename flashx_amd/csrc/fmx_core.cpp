@@ -223,6 +223,45 @@ fmx_status analyse_image(fmx_ctx *ctx, fmx_spm *s)
     return FMX_OK;
 }
 
+// The directory of an image built on the device (no host copy to analyse):
+// every block row has tiles, its first at word 0, word offsets strictly
+// increasing and leaving room for a tile header inside the block row, block
+// columns in range, value words (weighted) past their tile's header (one
+// past the block row for the empty tile of an empty block row).  The
+// SpMM plans and kernels trust it.
+fmx_status check_device_dir(fmx_ctx *ctx, const fmx_spm *s)
+{
+    const uint64_t nbr = s->nbr;
+    if (s->brow_dir.size() != nbr + 1 || s->offs.size() != nbr + 1 || s->brow_dir[0] != 0 ||
+        s->brow_dir[nbr] != s->dir.size() || (s->esz && s->vdir.size() != s->dir.size()))
+        return fail(ctx, FMX_EINVAL, "internal error: device-built directory has the wrong size");
+    const unsigned nth = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>({16, nbr / 4096 + 1,
+                                                                               std::thread::hardware_concurrency()}));
+    std::vector<int> bad(nth, 0);
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < nth; t++)
+        th.emplace_back([&, t] {
+            for (uint64_t br = t; br < nbr && !bad[t]; br += nth) {
+                const uint32_t lo = s->brow_dir[br], hi = s->brow_dir[br + 1];
+                const int64_t words = (s->offs[br + 1] - s->offs[br]) / 2;
+                if (hi <= lo || s->dir[lo].x != 0 || words < 8) { bad[t] = 1; break; }
+                for (uint32_t i = lo; i < hi; i++) {
+                    const uint2 d = s->dir[i];
+                    if ((int64_t)d.x + 8 > words || d.y >= std::max<uint64_t>(1, s->nbc) ||
+                        (i > lo && (d.x <= s->dir[i - 1].x || d.y <= s->dir[i - 1].y)) ||
+                        (s->esz && (s->vdir[i] < d.x + 8 || (int64_t)s->vdir[i] > words + 1))) {
+                        bad[t] = 1;
+                        break;
+                    }
+                }
+            }
+        });
+    for (auto &x : th) x.join();
+    for (int b : bad)
+        if (b) return fail(ctx, FMX_EINVAL, "internal error: inconsistent device-built directory");
+    return FMX_OK;
+}
+
 uint32_t band_rows(const fmx_spm *spm, int k, int s)
 {
     const char *e = getenv("FMX_BAND_BYTES");
@@ -234,11 +273,10 @@ uint32_t band_rows(const fmx_spm *spm, int k, int s)
     return (uint32_t)std::min<uint64_t>(hs, spm->H);
 }
 
-// Tile-relative row in force at word `p` of block row `br` (kNone if `p`
-// is a row-part or COO header); `p` is first moved out of a tile header.
-static uint32_t carry_at(const fmx_spm *s, uint64_t br, uint32_t &p, uint32_t &dir_idx)
+// Moves a cut at word `p` of block row `br` out of a tile header and names
+// the tile it falls in.
+static void split_point(const fmx_spm *s, uint64_t br, uint32_t &p, uint32_t &dir_idx)
 {
-    const uint16_t *w = reinterpret_cast<const uint16_t *>(s->host_image() + s->offs[br]);
     uint32_t lo = s->brow_dir[br], hi = s->brow_dir[br + 1];
     // last tile whose start <= p
     uint32_t t = (uint32_t)(std::upper_bound(s->dir.begin() + lo, s->dir.begin() + hi, p,
@@ -246,6 +284,15 @@ static uint32_t carry_at(const fmx_spm *s, uint64_t br, uint32_t &p, uint32_t &d
                             - s->dir.begin()) - 1;
     if (p < s->dir[t].x + 8) p = s->dir[t].x + 8;
     dir_idx = t;
+}
+
+// Tile-relative row in force at word `p` of the block row whose first word
+// is `base` in a host image (kNone if `p` is a row-part or COO header);
+// plan_carries computes the same on the device for an image without a host
+// copy.
+static uint32_t carry_at(const fmx_spm *s, uint64_t base, uint32_t p)
+{
+    const uint16_t *w = reinterpret_cast<const uint16_t *>(s->host_image()) + base;
     if (w[p] & 0x8000) return kNone;
     uint32_t q = p;
     while (!(w[q - 1] & 0x8000)) q--;
@@ -282,6 +329,11 @@ fmx_status get_plan(fmx_ctx *ctx, fmx_spm *s, uint32_t hs, bool ordered, Plan **
 
     std::vector<Split> splits;
     uint32_t nslots = 0;
+    // carries of binary split items, resolved after the loop: item index,
+    // block row's first word, cut word
+    std::vector<size_t> carry_item;
+    std::vector<uint64_t> carry_base;
+    std::vector<uint32_t> carry_word;
     for (const Band &b : bands) {
         uint32_t pieces = (uint32_t)(b.words / cap + (b.words % cap != 0));
         if (pieces < 1) pieces = 1;
@@ -320,7 +372,8 @@ fmx_status get_plan(fmx_ctx *ctx, fmx_spm *s, uint32_t hs, bool ordered, Plan **
                 dirs[i] = t;
                 carries[i] = kNone;
             } else {
-                carries[i] = carry_at(s, b.br, p, dirs[i]);
+                split_point(s, b.br, p, dirs[i]);
+                carries[i] = kNone;  // resolved below
             }
             cut[i] = p;
         }
@@ -337,9 +390,25 @@ fmx_status get_plan(fmx_ctx *ctx, fmx_spm *s, uint32_t hs, bool ordered, Plan **
             t.dir_begin = dirs[i];
             t.carry = carries[i];
             t.slot = nslots++;
+            if (i > 0 && !s->esz) {
+                carry_item.push_back(plan->items.size());
+                carry_base.push_back((uint64_t)s->offs[b.br] / 2);
+                carry_word.push_back(cut[i]);
+            }
             plan->items.push_back(t);
         }
         splits.push_back(sp);
+    }
+    if (!carry_item.empty()) {
+        std::vector<uint32_t> c;
+        if (s->host_bytes()) {
+            c.resize(carry_item.size());
+            for (size_t j = 0; j < c.size(); j++) c[j] = carry_at(s, carry_base[j], carry_word[j]);
+        } else {
+            fmx_status st = plan_carries(ctx, s, carry_base, carry_word, c);
+            if (st) return st;
+        }
+        for (size_t j = 0; j < c.size(); j++) plan->items[carry_item[j]].carry = c[j];
     }
     // Longest items first, so the hardware dispatcher approximates LPT.
     std::stable_sort(plan->items.begin(), plan->items.end(),
@@ -434,23 +503,41 @@ fmx_status upload_dirs(fmx_ctx *ctx, fmx_spm *s)
     return FMX_OK;
 }
 
-fmx_status finish_device_image(fmx_ctx *ctx, fmx_spm *s, bool internal)
-{
-    s->h_image.resize(s->bytes);
-    FMX_HIP(ctx, hipMemcpy(s->h_image.data(), s->d_image, s->bytes, hipMemcpyDeviceToHost));
-    fmx_status st = analyse_image(ctx, s);
-    if (st) return st;
-    st = upload_dirs(ctx, s);
-    if (st || internal) return st;
-    st = prepare_work(ctx, s);
-    if (st) return st;
-    return prepare_fast(ctx, s);
-}
-
 fmx_status prepare_work(fmx_ctx *ctx, fmx_spm *s)
 {
     const uint32_t h2 = work_rows_for(s);
     if (!h2) return FMX_OK;
+    if (!s->host_bytes() && s->d_image) {
+        // built on the device, without a host copy: the non-zeros listed in
+        // stream order and built again in h2-row tiles (every entry as it
+        // is, so the result is the re-cut of the same image)
+        struct Lists {
+            uint32_t *rows = nullptr, *cols = nullptr;
+            void *vals = nullptr;
+            ~Lists()
+            {
+                if (rows) (void)hipFree(rows);
+                if (cols) (void)hipFree(cols);
+                if (vals) (void)hipFree(vals);
+            }
+        } l;
+        const uint64_t nz = s->nnz ? s->nnz : 1;
+        if (hipMalloc(&l.rows, nz * 4) != hipSuccess || hipMalloc(&l.cols, nz * 4) != hipSuccess ||
+            (s->esz && hipMalloc(&l.vals, nz * s->esz) != hipSuccess))
+            return fail(ctx, FMX_ENOMEM, "re-cut work image");
+        fmx_status st = decode_lists(ctx, s, l.rows, l.cols, l.vals, nullptr);
+        if (st) return st;
+        int lh2 = 0;
+        while ((1u << lh2) < h2) lh2++;
+        fmx_spm *w = nullptr;
+        st = build_impl(ctx, l.rows, l.cols, s->nnz, s->nrows, (uint32_t)lh2, (uint32_t)s->log2w, FMX_BUILD_KEEP_SELF,
+                        s->esz ? l.vals : nullptr, (uint32_t)s->esz, s->prim, &w, s->ncols, true);
+        if (st) return st;
+        std::unique_ptr<fmx_spm> wp(w);
+        if (wp->nnz != s->nnz) return fail(ctx, FMX_EINVAL, "internal error: the re-cut work image lost non-zeros");
+        s->work = std::move(wp);
+        return FMX_OK;
+    }
     std::unique_ptr<fmx_spm> w(new fmx_spm());
     w->ctx = ctx;
     retile_image(s, h2, w->h_image, w->offs);

@@ -147,6 +147,8 @@ int elem_size(fmx_dtype dt);
 
 // Validates a host image and fills the matrix fields + directory.
 fmx_status analyse_image(fmx_ctx *ctx, fmx_spm *spm);
+// Consistency of the directory of an image built on the device.
+fmx_status check_device_dir(fmx_ctx *ctx, const fmx_spm *spm);
 // Device construction (build_tiles.hip); ncols != 0: an n x ncols matrix;
 // internal: the result gets no work images of its own.
 fmx_status build_impl(fmx_ctx *ctx, const uint32_t *src, const uint32_t *dst, size_t m, uint64_t n,
@@ -157,6 +159,24 @@ fmx_status build_impl(fmx_ctx *ctx, const uint32_t *src, const uint32_t *dst, si
 // stream_chunk != 0: the image is streamed from pinned host memory in
 // chunks of that many bytes, as the matrix it belongs to.
 fmx_status prepare_fast(fmx_ctx *ctx, fmx_spm *spm, uint64_t stream_chunk = 0);
+// The non-zeros of the image in HBM of src (image_dev.hip) in stream order
+// into device lists rows, cols (relabelled into gather order) and vals (esz
+// bytes each, if weighted); *hot (device, allocated here) lists the nhot
+// columns that cover `permille` of the non-zeros, in their new order.
+// *nhot = 0 (and nothing allocated) when there are none.
+fmx_status gather_order_lists(fmx_ctx *ctx, const fmx_spm *src, uint64_t ncols, uint64_t permille,
+                              uint32_t *rows, uint32_t *cols, void *vals, uint32_t **hot, uint64_t *nhot);
+// The non-zeros of the image in HBM of src in stream order into device lists
+// rows, cols and vals (esz bytes each, if weighted); uses (ncols counters,
+// zeroed by the caller, or NULL) counts each column's non-zeros
+// (image_dev.hip).
+fmx_status decode_lists(fmx_ctx *ctx, const fmx_spm *src, uint32_t *rows, uint32_t *cols, void *vals,
+                        uint32_t *uses);
+// carry[i]: the row in force at word p[i] of the block row whose first word
+// is base[i] (an absolute word index into the image in HBM), kNone at a
+// header word; p[i] lies past its tile's header (image_dev.hip).
+fmx_status plan_carries(fmx_ctx *ctx, const fmx_spm *s, const std::vector<uint64_t> &base,
+                        const std::vector<uint32_t> &p, std::vector<uint32_t> &carry);
 fmx_status finish_streamed(fmx_ctx *ctx, fmx_spm *s, uint64_t chunk_bytes, bool whole_file);
 // The chunks, staging buffers and directory of an analysed streamed image
 // (or of its re-cut work image), without a gather-order image.

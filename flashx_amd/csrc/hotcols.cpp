@@ -19,9 +19,10 @@
 // order.
 //
 // The image is built once, when the matrix is created: the column degrees
-// and the relabelled (row, column[, value]) list come from the host copy of
-// the image, and the device builder (build_tiles.hip) writes the new image
-// in tiles of at most 64 rows by 32768 columns.
+// and the relabelled (row, column[, value]) list come from the image in HBM
+// (image_dev.hip) or, for a matrix streamed from host memory, from its
+// host copy (below); the device builder (build_tiles.hip) then writes the
+// new image in tiles of at most 64 rows by 32768 columns.
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
@@ -78,16 +79,18 @@ void for_nonzeros(const fmx_spm *s, uint64_t br, F fn)
 
 unsigned host_threads() { return std::max(1u, std::min(16u, std::thread::hardware_concurrency())); }
 
-}  // namespace
-
-fmx_status prepare_fast(fmx_ctx *ctx, fmx_spm *s, uint64_t stream_chunk)
+// The host form of gather_order_lists (image_dev.hip) for an image that
+// is not in HBM: the same lists, from the host copy, uploaded into rows,
+// cols and vals; *hot_out (device) and *nhot_out as there.
+fmx_status host_lists(fmx_ctx *ctx, const fmx_spm *src, uint64_t ncols, uint64_t permille, uint32_t *d_rows,
+                      uint32_t *d_cols, void *d_vals, uint32_t **hot_out, uint64_t *nhot_out)
 {
-    const char *e = getenv("FMX_HOT_COLS");  // "0" turns the gather-order image off
-    if ((e && !strcmp(e, "0")) || s->nnz == 0 || s->nnz >= (1ull << 31)) return FMX_OK;
-    const fmx_spm *src = s->work ? s->work.get() : s;  // either walks the same non-zeros
+    *hot_out = nullptr;
+    *nhot_out = 0;
+    const uint64_t nnz = src->nnz, esz = src->esz;
     const unsigned nth = host_threads();
     // column degrees: per-thread counts over block-row stripes, then summed
-    std::vector<std::vector<uint32_t>> part(nth, std::vector<uint32_t>(s->ncols, 0));
+    std::vector<std::vector<uint32_t>> part(nth, std::vector<uint32_t>(ncols, 0));
     {
         std::vector<std::thread> th;
         for (unsigned t = 0; t < nth; t++)
@@ -98,31 +101,28 @@ fmx_status prepare_fast(fmx_ctx *ctx, fmx_spm *s, uint64_t stream_chunk)
             });
         for (auto &x : th) x.join();
     }
-    std::vector<uint32_t> deg(s->ncols, 0);
+    std::vector<uint32_t> deg(ncols, 0);
     for (unsigned t = 0; t < nth; t++) {
-        for (uint64_t c = 0; c < s->ncols; c++) deg[c] += part[t][c];
+        for (uint64_t c = 0; c < ncols; c++) deg[c] += part[t][c];
         std::vector<uint32_t>().swap(part[t]);
     }
     // the most used columns, descending, ties by column: the fewest whose
-    // non-zeros make up 95% of the total (FMX_HOT_PERMILLE: another share)
-    std::vector<uint32_t> order(s->ncols);
+    // non-zeros make up the wanted share of the total
+    std::vector<uint32_t> order(ncols);
     {
         // counting sort by descending degree, ascending column among equals
         const uint32_t maxd = *std::max_element(deg.begin(), deg.end());
         std::vector<uint64_t> pos((size_t)maxd + 2, 0);
-        for (uint64_t c = 0; c < s->ncols; c++) pos[maxd - deg[c] + 1]++;
+        for (uint64_t c = 0; c < ncols; c++) pos[maxd - deg[c] + 1]++;
         for (size_t i = 1; i < pos.size(); i++) pos[i] += pos[i - 1];
-        for (uint64_t c = 0; c < s->ncols; c++) order[pos[maxd - deg[c]]++] = (uint32_t)c;
+        for (uint64_t c = 0; c < ncols; c++) order[pos[maxd - deg[c]]++] = (uint32_t)c;
     }
-    const char *pm = getenv("FMX_HOT_PERMILLE");
-    const uint64_t permille = std::min<uint64_t>(1000, pm ? strtoull(pm, nullptr, 10) : 950);
     uint64_t nhot = 0, acc = 0;
-    while (nhot < s->ncols && acc * 1000 < s->nnz * permille && deg[order[nhot]] > 0) acc += deg[order[nhot++]];
-    if (nhot == 0) return FMX_OK;
-    std::vector<uint32_t> newid(s->ncols);
-    for (uint64_t c = 0; c < s->ncols; c++) newid[c] = (uint32_t)(nhot + c);
+    while (nhot < ncols && acc * 1000 < nnz * permille && deg[order[nhot]] > 0) acc += deg[order[nhot++]];
+    if (nhot == 0 || nhot + ncols > (1ull << 32)) return FMX_OK;  // no hot columns, or ids past 32 bits
+    std::vector<uint32_t> newid(ncols);
+    for (uint64_t c = 0; c < ncols; c++) newid[c] = (uint32_t)(nhot + c);
     for (uint64_t i = 0; i < nhot; i++) newid[order[i]] = (uint32_t)i;
-    if (nhot + s->ncols > (1ull << 32)) return FMX_OK;  // ids would not fit in 32 bits
     // the relabelled non-zeros in stream order, each block row's at its
     // prefix-sum position
     std::vector<uint64_t> nzoff(src->nbr + 1, 0);
@@ -137,9 +137,9 @@ fmx_status prepare_fast(fmx_ctx *ctx, fmx_spm *s, uint64_t stream_chunk)
         void *p = nullptr;
         ~Pinned() { if (p) (void)hipHostFree(p); }
     } prow, pcol, pval;
-    FMX_HIP(ctx, hipHostMalloc(&prow.p, s->nnz * 4, hipHostMallocDefault));
-    FMX_HIP(ctx, hipHostMalloc(&pcol.p, s->nnz * 4, hipHostMallocDefault));
-    if (s->esz) FMX_HIP(ctx, hipHostMalloc(&pval.p, s->nnz * s->esz, hipHostMallocDefault));
+    FMX_HIP(ctx, hipHostMalloc(&prow.p, nnz * 4, hipHostMallocDefault));
+    FMX_HIP(ctx, hipHostMalloc(&pcol.p, nnz * 4, hipHostMallocDefault));
+    if (esz) FMX_HIP(ctx, hipHostMalloc(&pval.p, nnz * esz, hipHostMallocDefault));
     uint32_t *rows = (uint32_t *)prow.p, *cols = (uint32_t *)pcol.p;
     uint8_t *vals = (uint8_t *)pval.p;
     {
@@ -151,39 +151,65 @@ fmx_status prepare_fast(fmx_ctx *ctx, fmx_spm *s, uint64_t stream_chunk)
                     for_nonzeros(src, br, [&](uint64_t r, uint64_t c, const uint8_t *v) {
                         rows[i] = (uint32_t)r;
                         cols[i] = newid[c];
-                        if (s->esz) memcpy(vals + s->esz * i, v, s->esz);
+                        if (esz) memcpy(vals + esz * i, v, esz);
                         i++;
                     });
                 }
             });
         for (auto &x : th) x.join();
     }
-    uint32_t *d_rows = nullptr, *d_cols = nullptr;
-    void *d_vals = nullptr;
-    auto release = [&]() {
-        if (d_rows) (void)hipFree(d_rows);
-        if (d_cols) (void)hipFree(d_cols);
-        if (d_vals) (void)hipFree(d_vals);
-    };
-    FMX_HIP(ctx, hipMalloc(&d_rows, s->nnz * 4));
-    if (hipMalloc(&d_cols, s->nnz * 4) != hipSuccess || (s->esz && hipMalloc(&d_vals, s->nnz * s->esz) != hipSuccess)) {
-        release();
-        return fail(ctx, FMX_ENOMEM, "gather-order image");
+    if (hipMemcpy(d_rows, rows, nnz * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_cols, cols, nnz * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        (esz && hipMemcpy(d_vals, vals, nnz * esz, hipMemcpyHostToDevice) != hipSuccess))
+        return fail(ctx, FMX_EHIP, "gather-order image upload");
+    uint32_t *hot = nullptr;
+    FMX_HIP(ctx, hipMalloc(&hot, nhot * 4));
+    if (hipMemcpy(hot, order.data(), nhot * 4, hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipFree(hot);
+        return fail(ctx, FMX_EHIP, "gather-order image upload");
     }
-    fmx_status st = FMX_OK;
-    if (hipMemcpy(d_rows, rows, s->nnz * 4, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(d_cols, cols, s->nnz * 4, hipMemcpyHostToDevice) != hipSuccess ||
-        (s->esz && hipMemcpy(d_vals, vals, s->nnz * s->esz, hipMemcpyHostToDevice) != hipSuccess))
-        st = fail(ctx, FMX_EHIP, "gather-order image upload");
+    *hot_out = hot;
+    *nhot_out = nhot;
+    return FMX_OK;
+}
+
+}  // namespace
+
+fmx_status prepare_fast(fmx_ctx *ctx, fmx_spm *s, uint64_t stream_chunk)
+{
+    const char *e = getenv("FMX_HOT_COLS");  // "0" turns the gather-order image off
+    if ((e && !strcmp(e, "0")) || s->nnz == 0 || s->nnz >= (1ull << 31)) return FMX_OK;
+    const fmx_spm *src = s->work ? s->work.get() : s;  // either walks the same non-zeros
+    // the share of the non-zeros the hot columns cover (FMX_HOT_PERMILLE)
+    const char *pm = getenv("FMX_HOT_PERMILLE");
+    const uint64_t permille = std::min<uint64_t>(1000, pm ? strtoull(pm, nullptr, 10) : 950);
+    struct Lists {
+        uint32_t *rows = nullptr, *cols = nullptr, *hot = nullptr;
+        void *vals = nullptr;
+        ~Lists()
+        {
+            if (rows) (void)hipFree(rows);
+            if (cols) (void)hipFree(cols);
+            if (vals) (void)hipFree(vals);
+            if (hot) (void)hipFree(hot);
+        }
+    } l;
+    if (hipMalloc(&l.rows, s->nnz * 4) != hipSuccess || hipMalloc(&l.cols, s->nnz * 4) != hipSuccess ||
+        (s->esz && hipMalloc(&l.vals, s->nnz * s->esz) != hipSuccess))
+        return fail(ctx, FMX_ENOMEM, "gather-order image");
+    uint64_t nhot = 0;
+    fmx_status st = src->d_image
+                        ? gather_order_lists(ctx, src, s->ncols, permille, l.rows, l.cols, l.vals, &l.hot, &nhot)
+                        : host_lists(ctx, src, s->ncols, permille, l.rows, l.cols, l.vals, &l.hot, &nhot);
+    if (st || nhot == 0) return st;
     fmx_spm *f = nullptr;
-    if (!st) {
+    {
         const uint32_t lh = (uint32_t)std::min(s->log2h, 6), lw = 15;
         // every non-zero as it is: self edges kept, duplicates kept (in
         // stream order, so a weighted row keeps its value order)
-        st = build_impl(ctx, d_rows, d_cols, s->nnz, s->nrows, lh, lw, FMX_BUILD_KEEP_SELF,
-                        s->esz ? d_vals : nullptr, (uint32_t)s->esz, s->prim, &f, nhot + s->ncols, true);
+        st = build_impl(ctx, l.rows, l.cols, s->nnz, s->nrows, lh, lw, FMX_BUILD_KEEP_SELF,
+                        s->esz ? l.vals : nullptr, (uint32_t)s->esz, s->prim, &f, nhot + s->ncols, true);
     }
-    release();
     if (st) return st;
     std::unique_ptr<fmx_spm> fast(f);
     if (fast->nnz != s->nnz) return fail(ctx, FMX_EINVAL, "internal error: the gather-order image lost non-zeros");
@@ -195,7 +221,7 @@ fmx_status prepare_fast(fmx_ctx *ctx, fmx_spm *s, uint64_t stream_chunk)
         h->ctx = ctx;
         fmx_status st2 = alloc_pinned(ctx, h.get(), d->bytes);
         if (st2) return st2;
-        memcpy(h->h_pinned, d->h_image.data(), d->bytes);
+        FMX_HIP(ctx, hipMemcpy(h->h_pinned, d->d_image, d->bytes, hipMemcpyDeviceToHost));
         h->offs = d->offs;
         fast.reset();
         st2 = analyse_image(ctx, h.get());
@@ -203,8 +229,8 @@ fmx_status prepare_fast(fmx_ctx *ctx, fmx_spm *s, uint64_t stream_chunk)
         if (st2) return st2;
         fast = std::move(h);
     }
-    FMX_HIP(ctx, hipMalloc(&s->d_hot, nhot * 4));
-    FMX_HIP(ctx, hipMemcpy(s->d_hot, order.data(), nhot * 4, hipMemcpyHostToDevice));
+    s->d_hot = l.hot;
+    l.hot = nullptr;
     s->nhot = nhot;
     s->fast = std::move(fast);
     return FMX_OK;

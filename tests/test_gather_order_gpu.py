@@ -1,4 +1,5 @@
-"""GPU parity of the gather-order image (flashx_amd/csrc/hotcols.cpp): the
+"""GPU parity of the gather-order image (flashx_amd/csrc/hotcols.cpp,
+image_dev.hip): the
 default-mode SpMM of a matrix whose X is large runs on a copy of the matrix
 with its most used columns relabelled into a dense front block of X that is
 restaged every call.  FMX_HOT_MIN_BYTES=0 makes small matrices take that
@@ -67,6 +68,9 @@ def test_gather_order_weighted_streamed_layouts(fmx, orc, rmat16, forced):
     img = orc.build_2d_w(src, dst, vals, n=1 << 16, log2h=7, log2w=15)
     res = fmx.SparseMatrix.upload(img.data, img.offs)
     st = fmx.SparseMatrix.upload_streamed(img.data, img.offs, chunk_bytes=1 << 16)
+    # the resident image is relabelled on the device, the streamed one from
+    # its host copy: the same hot columns
+    assert res.info.hot_columns == st.info.hot_columns > 0
     X = np.random.default_rng(5).random((img.n, 8)).astype(np.float32)
     Yr = res.multiply(dev(X)).cpu().numpy()
     assert rel(Yr, orc.spmm(img, X.astype(np.float64))) <= 2.0 ** -24 + 1e-15
