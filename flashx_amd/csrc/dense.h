@@ -17,6 +17,16 @@ fmx_status gram(fmx_ctx *ctx, fmx_dtype dt, int mode, uint64_t n, int ka, int kb
 fmx_status gemm_tall(fmx_ctx *ctx, fmx_dtype dt, uint64_t n, int ka, int kc, double alpha,
                      const void *A, int64_t lda, const double *C, double beta, void *Y, int64_t ldy,
                      hipStream_t s);
+// G = A^T B with A already BF16 (n x ka, leading dimension lda, ka and lda
+// multiples of 8, ka <= 128, 16-byte aligned) and B FP32 (kb <= 32, a
+// multiple of 4): BF16 MFMA, FP32 accumulation, partials summed in FP64.
+fmx_status gram_bf16_shadow(fmx_ctx *ctx, uint64_t n, int ka, int kb, const unsigned short *Ab, int64_t lda,
+                            const float *B, int64_t ldb, double *G, hipStream_t s);
+// gemm_tall on FP32 (kc <= 32, ka <= 128) that also writes each stored
+// value of Y rounded to BF16 into Yb (leading dimension ldyb).
+fmx_status gemm_tall_bf16copy(fmx_ctx *ctx, uint64_t n, int ka, int kc, double alpha, const float *A, int64_t lda,
+                              const double *C, double beta, float *Y, int64_t ldy, unsigned short *Yb, int64_t ldyb,
+                              hipStream_t s);
 fmx_status gemm_tall_f32_f64(fmx_ctx *ctx, uint64_t n, int ka, int kc, double alpha, const float *A,
                              int64_t lda, const double *C, double beta, double *Y, int64_t ldy,
                              hipStream_t s);

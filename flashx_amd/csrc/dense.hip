@@ -90,6 +90,41 @@ __global__ void __launch_bounds__(kT) reduce_partials(const TP *partial, int nbl
     }
 }
 
+// out[o] = the sum over 16 interleaved segments of the partials (segment q
+// holds blocks q, q + 16, ...), each summed in block order, then the
+// segments in order; FP64 throughout (deterministic).  16 outputs x 16
+// segments per block: the many-partial reduce of the BF16 Gram.
+template <typename TP>
+__global__ void __launch_bounds__(kT) reduce_partials_seg(const TP *partial, int nblk, int nout, double *out)
+{
+    __shared__ double red[kT];
+    const int ol = threadIdx.x & 15, sg = threadIdx.x >> 4;
+    const int o = blockIdx.x * 16 + ol;
+    double acc = 0.0;
+    if (o < nout)
+        for (int b = sg; b < nblk; b += 16) acc += (double)partial[(uint64_t)b * nout + o];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    if (sg == 0 && o < nout) {
+        double t = 0.0;
+        for (int q = 0; q < 16; q++) t += red[q * 16 + ol];
+        out[o] = t;
+    }
+}
+
+// ---- BF16 operands ---------------------------------------------------------------
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ unsigned short f2bf(float f)
+{
+    // round to nearest even
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7F800000u) == 0x7F800000u) return (unsigned short)(u >> 16);  // inf / nan
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+
 // ---- FP64 MFMA Gram and tall GEMM ------------------------------------------------
 // v_mfma_f64_16x16x4f64 with operands A: 16 x 4 (lane l holds A[l%16][l/16]),
 // B: 4 x 16 (lane l holds B[l/16][l%16]); products of FP32 inputs are exact
@@ -265,7 +300,7 @@ template <typename T, typename TY, int KAP, int TJ>
 __global__ void __launch_bounds__(kT) gemm_stage_kernel(const T *A, int64_t lda, uint64_t n, int ka, int kc,
                                                         const double *C, double alpha, double beta, TY *Y,
                                                         int64_t ldy, uint64_t rows_per_wave, uint64_t nwaves,
-                                                        int veca)
+                                                        int veca, unsigned short *Yb, int64_t ldyb)
 {
     constexpr int LA = KAP + 1;       // odd stride: the 16 rows of a k column hit distinct banks
     constexpr int KS = KAP / 4;       // MFMA K steps
@@ -318,7 +353,11 @@ __global__ void __launch_bounds__(kT) gemm_stage_kernel(const T *A, int64_t lda,
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 const uint64_t gr = r + lk + 4 * q;
-                if (gr < r1 && j < kc) Y[gr * ldy + j] = (TY)(prev[tj][q] + alpha * acc[tj][q]);
+                if (gr < r1 && j < kc) {
+                    const TY y = (TY)(prev[tj][q] + alpha * acc[tj][q]);
+                    Y[gr * ldy + j] = y;
+                    if (Yb) Yb[gr * ldyb + j] = f2bf((float)y);  // the BF16 copy of what was stored
+                }
             }
         }
     }
@@ -340,18 +379,6 @@ __global__ void __launch_bounds__(kT) reduce_partials4(const double *partial, in
 }
 
 // ---- BF16 MFMA Gram ------------------------------------------------------------
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef float floatx4 __attribute__((ext_vector_type(4)));
-
-__device__ __forceinline__ unsigned short f2bf(float f)
-{
-    // round to nearest even
-    uint32_t u = __float_as_uint(f);
-    if ((u & 0x7F800000u) == 0x7F800000u) return (unsigned short)(u >> 16);  // inf / nan
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    return (unsigned short)(u >> 16);
-}
-
 constexpr int kBK = 64;          // rows per LDS stage (two MFMA k-steps)
 constexpr int kBPad = kBK + 8;   // padded row length (bf16 elements)
 
@@ -420,11 +447,17 @@ __global__ void __launch_bounds__(kT) gram_bf16_kernel(const float *A, int64_t l
 
 // Fast path of gram_bf16_kernel for 16-byte aligned rows whose real widths
 // are multiples of 4 (ka <= 128, kb <= 32): each stage's rows are read as
-// float4 pieces, and the next stage's pieces are requested before the
-// current stage's MFMAs, so the loads overlap the products.
-constexpr int kBVA = kBK * 128 / 4 / kT;  // float4 pieces of A per thread and stage (8)
+// 16-byte pieces, and the next stage's pieces are requested before the
+// current stage's MFMAs, so the loads overlap the products.  AB16: A is
+// already BF16 (widths a multiple of 8), so its pieces carry 8 values and
+// the stage moves half the bytes.
+constexpr int kBVA = kBK * 128 / 4 / kT;  // 16-byte pieces of FP32 A per thread and stage (8)
 constexpr int kBVB = kBK * 32 / 4 / kT;   // and of B (2)
-__global__ void __launch_bounds__(kT) gram_bf16_vec_kernel(const float *A, int64_t lda, const float *B,
+struct alignas(16) Bf8 {
+    unsigned short v[8];
+};
+template <bool AB16>
+__global__ void __launch_bounds__(kT) gram_bf16_vec_kernel(const void *Av, int64_t lda, const float *B,
                                                            int64_t ldb, uint64_t n, int ka, int kb,
                                                            int ka_real, int kb_real,
                                                            uint64_t rows_per_blk, float *partial)
@@ -435,24 +468,32 @@ __global__ void __launch_bounds__(kT) gram_bf16_vec_kernel(const float *A, int64
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int tiles_i = ka / 16, tiles_j = kb / 16, ntiles = tiles_i * tiles_j;
     constexpr int kMaxTiles = 4;                 // ka <= 128, kb <= 32: at most 16 tiles
+    constexpr int VA = AB16 ? 8 : 4;             // A values per piece
+    constexpr int NA = AB16 ? kBVA / 2 : kBVA;   // A pieces per thread and stage
     floatx4 acc[kMaxTiles];
 #pragma unroll
     for (int t = 0; t < kMaxTiles; t++) acc[t] = floatx4{0.f, 0.f, 0.f, 0.f};
     const uint64_t r0 = blockIdx.x * rows_per_blk;
     const uint64_t r1 = min(n, r0 + rows_per_blk);
-    const int qa = ka / 4, qb = kb / 4;          // pieces per (padded) row
+    const int qa = ka / VA, qb = kb / 4;         // pieces per (padded) row
     const int pa = kBK * qa, pb = kBK * qb;      // pieces per stage
-    float4 ra[kBVA], rbv[kBVB];
+    float4 ra[AB16 ? 1 : NA], rbv[kBVB];
+    Bf8 rh[AB16 ? NA : 1];
     auto load = [&](uint64_t rb) {
 #pragma unroll
-        for (int t = 0; t < kBVA; t++) {
+        for (int t = 0; t < NA; t++) {
             const int e = threadIdx.x + t * kT;
-            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (e < pa) {
-                const int r = e / qa, c4 = e - r * qa;
-                if (rb + r < r1 && c4 * 4 < ka_real) v = *(const float4 *)(A + (rb + r) * lda + c4 * 4);
+            const int r = e / qa, ca = e - r * qa;
+            const bool in = e < pa && rb + r < r1 && ca * VA < ka_real;
+            if constexpr (AB16) {
+                Bf8 v = {};
+                if (in) v = *(const Bf8 *)((const unsigned short *)Av + (rb + r) * lda + ca * 8);
+                rh[t] = v;
+            } else {
+                float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (in) v = *(const float4 *)((const float *)Av + (rb + r) * lda + ca * 4);
+                ra[t] = v;
             }
-            ra[t] = v;
         }
 #pragma unroll
         for (int t = 0; t < kBVB; t++) {
@@ -467,14 +508,19 @@ __global__ void __launch_bounds__(kT) gram_bf16_vec_kernel(const float *A, int64
     };
     auto store = [&]() {
 #pragma unroll
-        for (int t = 0; t < kBVA; t++) {
+        for (int t = 0; t < NA; t++) {
             const int e = threadIdx.x + t * kT;
             if (e < pa) {
-                const int r = e / qa, c = (e - r * qa) * 4;
-                At[(c + 0) * kBPad + r] = f2bf(ra[t].x);
-                At[(c + 1) * kBPad + r] = f2bf(ra[t].y);
-                At[(c + 2) * kBPad + r] = f2bf(ra[t].z);
-                At[(c + 3) * kBPad + r] = f2bf(ra[t].w);
+                const int r = e / qa, c = (e - r * qa) * VA;
+                if constexpr (AB16) {
+#pragma unroll
+                    for (int u = 0; u < 8; u++) At[(c + u) * kBPad + r] = rh[t].v[u];
+                } else {
+                    At[(c + 0) * kBPad + r] = f2bf(ra[t].x);
+                    At[(c + 1) * kBPad + r] = f2bf(ra[t].y);
+                    At[(c + 2) * kBPad + r] = f2bf(ra[t].z);
+                    At[(c + 3) * kBPad + r] = f2bf(ra[t].w);
+                }
             }
         }
 #pragma unroll
@@ -653,6 +699,24 @@ __global__ void __launch_bounds__(kT) row_scale_kernel(uint64_t n, int k, const 
     }
 }
 
+// Y = diag(d) X for FP32 rows of k = 4 * k4 values (k4 dividing 256): one
+// float4 per thread, kT / k4 rows per block and step.
+__global__ void __launch_bounds__(kT) row_scale_vec_kernel(uint64_t n, int k4, const double *d, const float *X,
+                                                           int64_t ldx, float *Y, int64_t ldy)
+{
+    const int rpi = kT / k4, rr = threadIdx.x / k4, c = (threadIdx.x - rr * k4) * 4;
+    for (uint64_t r = blockIdx.x * (uint64_t)rpi + rr; r < n; r += (uint64_t)gridDim.x * rpi) {
+        const float4 x = *(const float4 *)(X + r * ldx + c);
+        const double dr = d[r];
+        float4 y;
+        y.x = (float)((double)x.x * dr);
+        y.y = (float)((double)x.y * dr);
+        y.z = (float)((double)x.z * dr);
+        y.w = (float)((double)x.w * dr);
+        *(float4 *)(Y + r * ldy + c) = y;
+    }
+}
+
 uint32_t blocks_for(fmx_ctx *ctx, uint64_t n, uint64_t rows_min)
 {
     uint64_t b = (n + rows_min - 1) / rows_min;
@@ -729,21 +793,21 @@ fmx_status gram(fmx_ctx *ctx, fmx_dtype dt, int mode, uint64_t n, int ka, int kb
         if (!attr) {
             FMX_HIP(ctx, hipFuncSetAttribute((const void *)gram_bf16_kernel,
                                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-            FMX_HIP(ctx, hipFuncSetAttribute((const void *)gram_bf16_vec_kernel,
+            FMX_HIP(ctx, hipFuncSetAttribute((const void *)gram_bf16_vec_kernel<false>,
                                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
             attr = true;
         }
         const bool vec = ka % 4 == 0 && kb % 4 == 0 && kap <= 128 && kbp <= 32 && (uintptr_t)A % 16 == 0 &&
                          (uintptr_t)B % 16 == 0 && lda % 4 == 0 && ldb % 4 == 0;
         if (vec)
-            hipLaunchKernelGGL(gram_bf16_vec_kernel, dim3(nblk), dim3(kT), lds, s, (const float *)A, lda,
+            hipLaunchKernelGGL(gram_bf16_vec_kernel<false>, dim3(nblk), dim3(kT), lds, s, A, lda,
                                (const float *)B, ldb, n, kap, kbp, ka, kb, rpb, (float *)ws);
         else
             hipLaunchKernelGGL(gram_bf16_kernel, dim3(nblk), dim3(kT), lds, s, (const float *)A, lda,
                                (const float *)B, ldb, n, kap, kbp, ka, kb, rpb, (float *)ws);
         FMX_HIP(ctx, hipGetLastError());
-        hipLaunchKernelGGL((reduce_partials<float, double>), dim3((nout + kT - 1) / kT), dim3(kT), 0, s,
-                           (const float *)ws, (int)nblk, nout, G, 0);
+        hipLaunchKernelGGL(reduce_partials_seg<float>, dim3((nout + 15) / 16), dim3(kT), 0, s, (const float *)ws,
+                           (int)nblk, nout, G);
         FMX_HIP(ctx, hipGetLastError());
         return FMX_OK;
     }
@@ -880,7 +944,8 @@ fmx_status gram(fmx_ctx *ctx, fmx_dtype dt, int mode, uint64_t n, int ka, int kb
 // C in registers; A may be Y.
 template <typename T, typename TY>
 fmx_status gemm_staged(fmx_ctx *ctx, uint64_t n, int ka, int kc, double alpha, const T *A, int64_t lda,
-                       const double *C, double beta, TY *Y, int64_t ldy, hipStream_t s)
+                       const double *C, double beta, TY *Y, int64_t ldy, hipStream_t s,
+                       unsigned short *Yb = nullptr, int64_t ldyb = 0)
 {
     if (n == 0) return FMX_OK;
     const int kap = ka <= 16 ? 16 : ka <= 32 ? 32 : ka <= 64 ? 64 : ka <= 96 ? 96 : 128;
@@ -909,7 +974,7 @@ fmx_status gemm_staged(fmx_ctx *ctx, uint64_t n, int ka, int kc, double alpha, c
     }
 #define FMX_GEMMS(P, J)                                                                                       \
     hipLaunchKernelGGL((gemm_stage_kernel<T, TY, P, J>), dim3(nblk), dim3(kT), lds, s, A, lda, n, ka, kc, C,     \
-                       alpha, beta, Y, ldy, rpw, waves, veca)
+                       alpha, beta, Y, ldy, rpw, waves, veca, Yb, ldyb)
 #define FMX_GEMMS_P(J)                                                                                        \
     do {                                                                                                      \
         if (kap == 16) FMX_GEMMS(16, J);                                                                      \
@@ -924,6 +989,44 @@ fmx_status gemm_staged(fmx_ctx *ctx, uint64_t n, int ka, int kc, double alpha, c
 #undef FMX_GEMMS
     FMX_HIP(ctx, hipGetLastError());
     return FMX_OK;
+}
+
+fmx_status gram_bf16_shadow(fmx_ctx *ctx, uint64_t n, int ka, int kb, const unsigned short *Ab, int64_t lda,
+                            const float *B, int64_t ldb, double *G, hipStream_t s)
+{
+    const int kap = (ka + 15) & ~15, kbp = (kb + 15) & ~15;
+    if (ka <= 0 || kb <= 0 || ka % 8 || kb % 4 || kap > 128 || kbp > 32 || lda < ka || ldb < kb || lda % 8 ||
+        ldb % 4 || (uintptr_t)Ab % 16 || (uintptr_t)B % 16)
+        return fail(ctx, FMX_EDIM, "BF16-operand Gram: widths, strides or alignment unsupported");
+    const int nout = ka * kb;
+    uint32_t nblk = blocks_for(ctx, n, 1024);
+    void *ws;
+    fmx_status st = ctx_scratch(ctx, (size_t)nblk * nout * sizeof(float), &ws);
+    if (st) return st;
+    const uint64_t rpb = (n + nblk - 1) / nblk;
+    const size_t lds = (size_t)(kap + kbp) * kBPad * sizeof(unsigned short);
+    static bool attr = false;
+    if (!attr) {
+        FMX_HIP(ctx, hipFuncSetAttribute((const void *)gram_bf16_vec_kernel<true>,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+        attr = true;
+    }
+    hipLaunchKernelGGL(gram_bf16_vec_kernel<true>, dim3(nblk), dim3(kT), lds, s, (const void *)Ab, lda, B, ldb, n,
+                       kap, kbp, ka, kb, rpb, (float *)ws);
+    FMX_HIP(ctx, hipGetLastError());
+    hipLaunchKernelGGL(reduce_partials_seg<float>, dim3((nout + 15) / 16), dim3(kT), 0, s, (const float *)ws,
+                       (int)nblk, nout, G);
+    FMX_HIP(ctx, hipGetLastError());
+    return FMX_OK;
+}
+
+fmx_status gemm_tall_bf16copy(fmx_ctx *ctx, uint64_t n, int ka, int kc, double alpha, const float *A, int64_t lda,
+                              const double *C, double beta, float *Y, int64_t ldy, unsigned short *Yb, int64_t ldyb,
+                              hipStream_t s)
+{
+    if (ka <= 0 || kc <= 0 || ka > 128 || kc > 32 || lda < ka || ldy < kc || ldyb < kc)
+        return fail(ctx, FMX_EDIM, "gemm_tall with a BF16 copy needs kc <= 32, ka <= 128");
+    return gemm_staged<float, float>(ctx, n, ka, kc, alpha, A, lda, C, beta, Y, ldy, s, Yb, ldyb);
 }
 
 fmx_status gemm_tall_f32_f64(fmx_ctx *ctx, uint64_t n, int ka, int kc, double alpha, const float *A,
@@ -1055,7 +1158,13 @@ fmx_status row_scale(fmx_ctx *ctx, fmx_dtype dt, uint64_t n, int k, const double
     uint64_t blocks = (tot + kT - 1) / kT;
     if (blocks > (uint64_t)ctx->num_cus * 16) blocks = (uint64_t)ctx->num_cus * 16;
     if (!blocks) return FMX_OK;
-    if (dt == FMX_F32)
+    if (dt == FMX_F32 && k % 4 == 0 && kT % (k / 4) == 0 && ldx % 4 == 0 && ldy % 4 == 0 &&
+        (uintptr_t)X % 16 == 0 && (uintptr_t)Y % 16 == 0) {
+        const uint64_t rpi = kT / (k / 4);
+        const uint64_t vb = std::min<uint64_t>((n + rpi - 1) / rpi, (uint64_t)ctx->num_cus * 16);
+        hipLaunchKernelGGL(row_scale_vec_kernel, dim3((uint32_t)vb), dim3(kT), 0, s, n, k / 4, d, (const float *)X,
+                           ldx, (float *)Y, ldy);
+    } else if (dt == FMX_F32)
         hipLaunchKernelGGL((row_scale_kernel<float, double>), dim3((uint32_t)blocks), dim3(kT), 0, s, n, k,
                            d, (const float *)X, ldx, (float *)Y, ldy);
     else if (dt == FMX_F64)

@@ -20,8 +20,9 @@
 //                                           SpMM with the outer scaling
 //                                           applied to its sums; SVD: two
 //                                           SpMMs through an m_rows x b block)
-//   C1 = V^T W, W -= V C1                  (first pass; BF16 MFMA Gram in the
-//                                           mixed mode, FP64 otherwise)
+//   C1 = V^T W, W -= V C1                  (first pass; in the mixed mode a
+//                                           BF16 MFMA Gram over a BF16 copy of
+//                                           V kept beside it, FP64 otherwise)
 //   C2 = V^T W, W -= V C2                  (FP64 reorthogonalisation)
 //   H[:, j] = C1 + C2
 //   V_{j+1} = W R^-1, R from Cholesky-QR twice (FP64 Gram + Cholesky)
@@ -79,7 +80,7 @@ private:
     fmx_status apply(const T *X, int64_t ldx, T *Y, int64_t ldy);
     fmx_status apply64(const double *X, double *Y, int k);
     fmx_status project(T *W, int cols, double *Cacc);
-    fmx_status cholqr(T *W, int64_t ldw, T *dst, int64_t ldd, bool &ok);
+    fmx_status cholqr(T *W, int64_t ldw, T *dst, int64_t ldd, bool &ok, unsigned short *dstb = nullptr);
     fmx_status orth_block(T *W, int cols, int dst_col);
     fmx_status sync() { FMX_HIP(ctx_, hipStreamSynchronize(s_)); return FMX_OK; }
 
@@ -99,6 +100,10 @@ private:
     uint64_t n_, nmid_;  // basis rows; rows of A (the SVD's middle product)
     int b_, m_;
     Dev<T> V_, W_, T1_, T2_;
+    // mixed mode: a BF16 copy of the basis, written with it, which the
+    // first-pass Grams read (half the bytes of the FP32 basis)
+    Dev<unsigned short> Vb_;
+    bool shadow_ = false;
     Dev<double> dinv_, H_, S_, ev_, C1_, C2_, G_, Rinv_, info_, X64_, Y64_, Z64_, M64_, Ssel_, Scols_, theta_, rn_,
         part_;
     Dev<float> deg_;
@@ -135,6 +140,10 @@ fmx_status Solver<T>::alloc()
     FMX_HIP(ctx_, hipMalloc(&theta_.p, (size_t)m_ * sizeof(double)));
     FMX_HIP(ctx_, hipMalloc(&rn_.p, (size_t)m_ * sizeof(double)));
     FMX_HIP(ctx_, hipMalloc(&part_.p, (size_t)1024 * 256 * sizeof(double)));
+    const int keep = std::min(m_ - b_, ((o_.nev + b_ - 1) / b_) * b_ + b_);
+    shadow_ = std::is_same<T, float>::value && o_.precision == FMX_EIG_MIXED && b_ % 8 == 0 && m_ % 8 == 0 &&
+              m_ <= 128 && keep <= 32;
+    if (shadow_) FMX_HIP(ctx_, hipMalloc(&Vb_.p, n * m_ * sizeof(unsigned short)));
     if (op_ == FMX_EIG_OP_SVD) {
         const uint64_t nm = nmid_ ? nmid_ : 1;
         FMX_HIP(ctx_, hipMalloc(&T2_.p, nm * b_ * sizeof(T)));
@@ -191,7 +200,10 @@ fmx_status Solver<T>::project(T *W, int cols, double *Cacc)
 {
     const int first_mode = (std::is_same<T, float>::value && o_.precision == FMX_EIG_MIXED)
                                ? FMX_GRAM_BF16 : FMX_GRAM_F64;
-    TRY(gram(ctx_, dt_of<T>(), first_mode, n_, cols, b_, V_.p, m_, W, b_, Cacc, s_));
+    if (shadow_)
+        TRY(gram_bf16_shadow(ctx_, n_, cols, b_, Vb_.p, m_, (const float *)W, b_, Cacc, s_));
+    else
+        TRY(gram(ctx_, dt_of<T>(), first_mode, n_, cols, b_, V_.p, m_, W, b_, Cacc, s_));
     TRY(gemm_tall(ctx_, dt_of<T>(), n_, cols, b_, -1.0, V_.p, m_, Cacc, 1.0, W, b_, s_));
     TRY(gram(ctx_, dt_of<T>(), FMX_GRAM_F64, n_, cols, b_, V_.p, m_, W, b_, C2_.p, s_));
     TRY(gemm_tall(ctx_, dt_of<T>(), n_, cols, b_, -1.0, V_.p, m_, C2_.p, 1.0, W, b_, s_));
@@ -200,7 +212,7 @@ fmx_status Solver<T>::project(T *W, int cols, double *Cacc)
 
 // dst = orthonormal basis of W (Cholesky-QR applied twice, FP64)
 template <typename T>
-fmx_status Solver<T>::cholqr(T *W, int64_t ldw, T *dst, int64_t ldd, bool &ok)
+fmx_status Solver<T>::cholqr(T *W, int64_t ldw, T *dst, int64_t ldd, bool &ok, unsigned short *dstb)
 {
     ok = true;
     double info[2];
@@ -212,7 +224,11 @@ fmx_status Solver<T>::cholqr(T *W, int64_t ldw, T *dst, int64_t ldd, bool &ok)
         FMX_HIP(ctx_, hipMemcpyAsync(info, info_.p, sizeof(double), hipMemcpyDeviceToHost, s_));
         TRY(sync());
         if (info[0] != 0.0) { ok = false; return FMX_OK; }
-        TRY(gemm_tall(ctx_, dt_of<T>(), n_, b_, b_, 1.0, src, lds, Rinv_.p, 0.0, dst, ldd, s_));
+        if (pass == 1 && dstb)
+            TRY(gemm_tall_bf16copy(ctx_, n_, b_, b_, 1.0, (const float *)src, lds, Rinv_.p, 0.0, (float *)dst, ldd,
+                                   dstb, m_, s_));
+        else
+            TRY(gemm_tall(ctx_, dt_of<T>(), n_, b_, b_, 1.0, src, lds, Rinv_.p, 0.0, dst, ldd, s_));
     }
     return FMX_OK;
 }
@@ -228,7 +244,7 @@ fmx_status Solver<T>::orth_block(T *W, int cols, int dst_col)
             TRY(rand_uniform(ctx_, dt_of<T>(), W, b_, n_, b_, o_.seed + 7919ull * (attempt + dst_col), s_));
             if (cols) TRY(project(W, cols, C1_.p));
         }
-        TRY(cholqr(W, b_, V_.p + dst_col, m_, ok));
+        TRY(cholqr(W, b_, V_.p + dst_col, m_, ok, shadow_ ? Vb_.p + dst_col : nullptr));
     }
     if (!ok) return fail(ctx_, FMX_EINVAL, "Krylov basis breakdown: operator rank too small");
     return FMX_OK;
@@ -367,7 +383,11 @@ fmx_status Solver<T>::run(const void *X0, int64_t ldx0, fmx_dtype x0_dt, double 
             double *Sk = Scols_.p;  // m x keep, stream-ordered after the Ritz-vector product
             FMX_HIP(ctx_, hipMemcpy2DAsync(Sk, keep * sizeof(double), Ssel_.p, m * sizeof(double),
                                            keep * sizeof(double), m, hipMemcpyDeviceToDevice, s_));
-            TRY(gemm_tall(ctx_, dt_of<T>(), n_, m, keep, 1.0, V_.p, m, Sk, 0.0, V_.p, m, s_));
+            if (shadow_)
+                TRY(gemm_tall_bf16copy(ctx_, n_, m, keep, 1.0, (const float *)V_.p, m, Sk, 0.0, (float *)V_.p, m,
+                                       Vb_.p, m, s_));
+            else
+                TRY(gemm_tall(ctx_, dt_of<T>(), n_, m, keep, 1.0, V_.p, m, Sk, 0.0, V_.p, m, s_));
             std::vector<double> hH((size_t)m * m, 0.0);
             for (int i = 0; i < keep; i++) hH[(size_t)i * m + i] = th[i];
             FMX_HIP(ctx_, hipMemcpyAsync(H_.p, hH.data(), (size_t)m * m * sizeof(double),

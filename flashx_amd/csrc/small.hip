@@ -69,7 +69,10 @@ __global__ void __launch_bounds__(kJT) jacobi_kernel(const double *Ain, int m, d
         }
         tot = red[0];
         __syncthreads();
-        if (off <= 1e-30 * tot || off == 0.0) break;
+        // off-diagonal norm below 1e-12 of the whole: a Ritz pair's residual
+        // then moves by at most about that fraction of ||H|| (the product of
+        // rotations stays orthogonal to rounding whatever the threshold)
+        if (off <= 1e-24 * tot || off == 0.0) break;
         for (int r = 0; r < m - 1; r++) {
             if (tid < half) {
                 const int k = tid;
