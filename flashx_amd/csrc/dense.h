@@ -22,11 +22,12 @@ fmx_status gemm_tall(fmx_ctx *ctx, fmx_dtype dt, uint64_t n, int ka, int kc, dou
 // multiple of 4): BF16 MFMA, FP32 accumulation, partials summed in FP64.
 fmx_status gram_bf16_shadow(fmx_ctx *ctx, uint64_t n, int ka, int kb, const unsigned short *Ab, int64_t lda,
                             const float *B, int64_t ldb, double *G, hipStream_t s);
-// gemm_tall on FP32 (kc <= 32, ka <= 128) that also writes each stored
-// value of Y rounded to BF16 into Yb (leading dimension ldyb).
-fmx_status gemm_tall_bf16copy(fmx_ctx *ctx, uint64_t n, int ka, int kc, double alpha, const float *A, int64_t lda,
-                              const double *C, double beta, float *Y, int64_t ldy, unsigned short *Yb, int64_t ldyb,
-                              hipStream_t s);
+// gemm_tall (kc <= 32, ka <= 128) that also writes, for each stored value
+// y of row r, y rounded to BF16 into Yb (FP32 only; or NULL) and y * ds[r]
+// into Ys (or NULL), with leading dimensions ldyb and ldys.
+fmx_status gemm_tall_copies(fmx_ctx *ctx, fmx_dtype dt, uint64_t n, int ka, int kc, double alpha, const void *A,
+                            int64_t lda, const double *C, double beta, void *Y, int64_t ldy, unsigned short *Yb,
+                            int64_t ldyb, const double *ds, void *Ys, int64_t ldys, hipStream_t s);
 fmx_status gemm_tall_f32_f64(fmx_ctx *ctx, uint64_t n, int ka, int kc, double alpha, const float *A,
                              int64_t lda, const double *C, double beta, double *Y, int64_t ldy,
                              hipStream_t s);

@@ -300,7 +300,8 @@ template <typename T, typename TY, int KAP, int TJ>
 __global__ void __launch_bounds__(kT) gemm_stage_kernel(const T *A, int64_t lda, uint64_t n, int ka, int kc,
                                                         const double *C, double alpha, double beta, TY *Y,
                                                         int64_t ldy, uint64_t rows_per_wave, uint64_t nwaves,
-                                                        int veca, unsigned short *Yb, int64_t ldyb)
+                                                        int veca, unsigned short *Yb, int64_t ldyb,
+                                                        const double *ds, TY *Ys, int64_t ldys)
 {
     constexpr int LA = KAP + 1;       // odd stride: the 16 rows of a k column hit distinct banks
     constexpr int KS = KAP / 4;       // MFMA K steps
@@ -357,6 +358,7 @@ __global__ void __launch_bounds__(kT) gemm_stage_kernel(const T *A, int64_t lda,
                     const TY y = (TY)(prev[tj][q] + alpha * acc[tj][q]);
                     Y[gr * ldy + j] = y;
                     if (Yb) Yb[gr * ldyb + j] = f2bf((float)y);  // the BF16 copy of what was stored
+                    if (Ys) Ys[gr * ldys + j] = (TY)((double)y * ds[gr]);  // and its row-scaled copy
                 }
             }
         }
@@ -945,7 +947,8 @@ fmx_status gram(fmx_ctx *ctx, fmx_dtype dt, int mode, uint64_t n, int ka, int kb
 template <typename T, typename TY>
 fmx_status gemm_staged(fmx_ctx *ctx, uint64_t n, int ka, int kc, double alpha, const T *A, int64_t lda,
                        const double *C, double beta, TY *Y, int64_t ldy, hipStream_t s,
-                       unsigned short *Yb = nullptr, int64_t ldyb = 0)
+                       unsigned short *Yb = nullptr, int64_t ldyb = 0, const double *ds = nullptr,
+                       TY *Ys = nullptr, int64_t ldys = 0)
 {
     if (n == 0) return FMX_OK;
     const int kap = ka <= 16 ? 16 : ka <= 32 ? 32 : ka <= 64 ? 64 : ka <= 96 ? 96 : 128;
@@ -974,7 +977,7 @@ fmx_status gemm_staged(fmx_ctx *ctx, uint64_t n, int ka, int kc, double alpha, c
     }
 #define FMX_GEMMS(P, J)                                                                                       \
     hipLaunchKernelGGL((gemm_stage_kernel<T, TY, P, J>), dim3(nblk), dim3(kT), lds, s, A, lda, n, ka, kc, C,     \
-                       alpha, beta, Y, ldy, rpw, waves, veca, Yb, ldyb)
+                       alpha, beta, Y, ldy, rpw, waves, veca, Yb, ldyb, ds, Ys, ldys)
 #define FMX_GEMMS_P(J)                                                                                        \
     do {                                                                                                      \
         if (kap == 16) FMX_GEMMS(16, J);                                                                      \
@@ -1020,13 +1023,20 @@ fmx_status gram_bf16_shadow(fmx_ctx *ctx, uint64_t n, int ka, int kb, const unsi
     return FMX_OK;
 }
 
-fmx_status gemm_tall_bf16copy(fmx_ctx *ctx, uint64_t n, int ka, int kc, double alpha, const float *A, int64_t lda,
-                              const double *C, double beta, float *Y, int64_t ldy, unsigned short *Yb, int64_t ldyb,
-                              hipStream_t s)
+fmx_status gemm_tall_copies(fmx_ctx *ctx, fmx_dtype dt, uint64_t n, int ka, int kc, double alpha, const void *A,
+                            int64_t lda, const double *C, double beta, void *Y, int64_t ldy, unsigned short *Yb,
+                            int64_t ldyb, const double *ds, void *Ys, int64_t ldys, hipStream_t s)
 {
-    if (ka <= 0 || kc <= 0 || ka > 128 || kc > 32 || lda < ka || ldy < kc || ldyb < kc)
-        return fail(ctx, FMX_EDIM, "gemm_tall with a BF16 copy needs kc <= 32, ka <= 128");
-    return gemm_staged<float, float>(ctx, n, ka, kc, alpha, A, lda, C, beta, Y, ldy, s, Yb, ldyb);
+    if (ka <= 0 || kc <= 0 || ka > 128 || kc > 32 || lda < ka || ldy < kc || (Yb && ldyb < kc) ||
+        (Ys && (ldys < kc || !ds)))
+        return fail(ctx, FMX_EDIM, "gemm_tall with copies needs kc <= 32, ka <= 128");
+    if (dt == FMX_F32)
+        return gemm_staged<float, float>(ctx, n, ka, kc, alpha, (const float *)A, lda, C, beta, (float *)Y, ldy, s,
+                                         Yb, ldyb, ds, (float *)Ys, ldys);
+    if (dt == FMX_F64 && !Yb)
+        return gemm_staged<double, double>(ctx, n, ka, kc, alpha, (const double *)A, lda, C, beta, (double *)Y,
+                                           ldy, s, nullptr, 0, ds, (double *)Ys, ldys);
+    return fail(ctx, FMX_ETYPE, "gemm_tall with copies: FP32, or FP64 without a BF16 copy");
 }
 
 fmx_status gemm_tall_f32_f64(fmx_ctx *ctx, uint64_t n, int ka, int kc, double alpha, const float *A,

@@ -80,7 +80,8 @@ private:
     fmx_status apply(const T *X, int64_t ldx, T *Y, int64_t ldy);
     fmx_status apply64(const double *X, double *Y, int k);
     fmx_status project(T *W, int cols, double *Cacc);
-    fmx_status cholqr(T *W, int64_t ldw, T *dst, int64_t ldd, bool &ok, unsigned short *dstb = nullptr);
+    fmx_status cholqr(T *W, int64_t ldw, T *dst, int64_t ldd, bool &ok, unsigned short *dstb = nullptr,
+                      bool scaled = false);
     fmx_status orth_block(T *W, int cols, int dst_col);
     fmx_status sync() { FMX_HIP(ctx_, hipStreamSynchronize(s_)); return FMX_OK; }
 
@@ -104,6 +105,9 @@ private:
     // first-pass Grams read (half the bytes of the FP32 basis)
     Dev<unsigned short> Vb_;
     bool shadow_ = false;
+    // the block whose D^-1/2-scaled copy T1_ holds (the normalised operator's
+    // input, written with the block by Cholesky-QR), or null
+    const T *scaled_of_ = nullptr;
     Dev<double> dinv_, H_, S_, ev_, C1_, C2_, G_, Rinv_, info_, X64_, Y64_, Z64_, M64_, Ssel_, Scols_, theta_, rn_,
         part_;
     Dev<float> deg_;
@@ -160,7 +164,8 @@ fmx_status Solver<T>::apply(const T *X, int64_t ldx, T *Y, int64_t ldy)
     if (op_ == FMX_EIG_OP_NORMALIZED) {
         // D^-1/2 X, then A (D^-1/2 X) with the outer D^-1/2 applied to each
         // row's FP64 sum before its rounding
-        TRY(row_scale(ctx_, dt_of<T>(), n_, b_, dinv_.p, X, ldx, T1_.p, b_, s_));
+        if (X != scaled_of_) TRY(row_scale(ctx_, dt_of<T>(), n_, b_, dinv_.p, X, ldx, T1_.p, b_, s_));
+        scaled_of_ = X;
         TRY(spmm(spm_, dt_of<T>(), b_, T1_.p, b_, Y, ldy, dinv_.p));
     } else if (op_ == FMX_EIG_OP_PLAIN) {
         TRY(spmm(spm_, dt_of<T>(), b_, X, ldx, Y, ldy, nullptr));
@@ -212,7 +217,7 @@ fmx_status Solver<T>::project(T *W, int cols, double *Cacc)
 
 // dst = orthonormal basis of W (Cholesky-QR applied twice, FP64)
 template <typename T>
-fmx_status Solver<T>::cholqr(T *W, int64_t ldw, T *dst, int64_t ldd, bool &ok, unsigned short *dstb)
+fmx_status Solver<T>::cholqr(T *W, int64_t ldw, T *dst, int64_t ldd, bool &ok, unsigned short *dstb, bool scaled)
 {
     ok = true;
     double info[2];
@@ -224,9 +229,9 @@ fmx_status Solver<T>::cholqr(T *W, int64_t ldw, T *dst, int64_t ldd, bool &ok, u
         FMX_HIP(ctx_, hipMemcpyAsync(info, info_.p, sizeof(double), hipMemcpyDeviceToHost, s_));
         TRY(sync());
         if (info[0] != 0.0) { ok = false; return FMX_OK; }
-        if (pass == 1 && dstb)
-            TRY(gemm_tall_bf16copy(ctx_, n_, b_, b_, 1.0, (const float *)src, lds, Rinv_.p, 0.0, (float *)dst, ldd,
-                                   dstb, m_, s_));
+        if (pass == 1 && (dstb || scaled))
+            TRY(gemm_tall_copies(ctx_, dt_of<T>(), n_, b_, b_, 1.0, src, lds, Rinv_.p, 0.0, dst, ldd, dstb, m_,
+                                 scaled ? dinv_.p : nullptr, scaled ? T1_.p : nullptr, b_, s_));
         else
             TRY(gemm_tall(ctx_, dt_of<T>(), n_, b_, b_, 1.0, src, lds, Rinv_.p, 0.0, dst, ldd, s_));
     }
@@ -244,7 +249,9 @@ fmx_status Solver<T>::orth_block(T *W, int cols, int dst_col)
             TRY(rand_uniform(ctx_, dt_of<T>(), W, b_, n_, b_, o_.seed + 7919ull * (attempt + dst_col), s_));
             if (cols) TRY(project(W, cols, C1_.p));
         }
-        TRY(cholqr(W, b_, V_.p + dst_col, m_, ok, shadow_ ? Vb_.p + dst_col : nullptr));
+        const bool scaled = op_ == FMX_EIG_OP_NORMALIZED;
+        TRY(cholqr(W, b_, V_.p + dst_col, m_, ok, shadow_ ? Vb_.p + dst_col : nullptr, scaled));
+        scaled_of_ = (ok && scaled) ? V_.p + dst_col : nullptr;
     }
     if (!ok) return fail(ctx_, FMX_EINVAL, "Krylov basis breakdown: operator rank too small");
     return FMX_OK;
@@ -383,9 +390,10 @@ fmx_status Solver<T>::run(const void *X0, int64_t ldx0, fmx_dtype x0_dt, double 
             double *Sk = Scols_.p;  // m x keep, stream-ordered after the Ritz-vector product
             FMX_HIP(ctx_, hipMemcpy2DAsync(Sk, keep * sizeof(double), Ssel_.p, m * sizeof(double),
                                            keep * sizeof(double), m, hipMemcpyDeviceToDevice, s_));
+            scaled_of_ = nullptr;  // the kept blocks are rotated
             if (shadow_)
-                TRY(gemm_tall_bf16copy(ctx_, n_, m, keep, 1.0, (const float *)V_.p, m, Sk, 0.0, (float *)V_.p, m,
-                                       Vb_.p, m, s_));
+                TRY(gemm_tall_copies(ctx_, FMX_F32, n_, m, keep, 1.0, V_.p, m, Sk, 0.0, V_.p, m, Vb_.p, m, nullptr,
+                                     nullptr, 0, s_));
             else
                 TRY(gemm_tall(ctx_, dt_of<T>(), n_, m, keep, 1.0, V_.p, m, Sk, 0.0, V_.p, m, s_));
             std::vector<double> hH((size_t)m * m, 0.0);
