@@ -221,9 +221,11 @@ fmx_status Solver<T>::cholqr(T *W, int64_t ldw, T *dst, int64_t ldd, bool &ok, u
 {
     ok = true;
     double info[2];
+    // the first pass works in place in W (contiguous rows), the second
+    // writes the basis block (every b-th stretch of V's rows) once
     for (int pass = 0; pass < 2; pass++) {
-        T *src = pass == 0 ? W : dst;
-        const int64_t lds = pass == 0 ? ldw : ldd;
+        T *src = W;
+        const int64_t lds = ldw;
         TRY(gram(ctx_, dt_of<T>(), FMX_GRAM_F64, n_, b_, b_, src, lds, src, lds, G_.p, s_));
         TRY(small_chol_inv(ctx_, G_.p, b_, Rinv_.p, info_.p, s_));
         FMX_HIP(ctx_, hipMemcpyAsync(info, info_.p, sizeof(double), hipMemcpyDeviceToHost, s_));
@@ -232,8 +234,10 @@ fmx_status Solver<T>::cholqr(T *W, int64_t ldw, T *dst, int64_t ldd, bool &ok, u
         if (pass == 1 && (dstb || scaled))
             TRY(gemm_tall_copies(ctx_, dt_of<T>(), n_, b_, b_, 1.0, src, lds, Rinv_.p, 0.0, dst, ldd, dstb, m_,
                                  scaled ? dinv_.p : nullptr, scaled ? T1_.p : nullptr, b_, s_));
-        else
+        else if (pass == 1)
             TRY(gemm_tall(ctx_, dt_of<T>(), n_, b_, b_, 1.0, src, lds, Rinv_.p, 0.0, dst, ldd, s_));
+        else
+            TRY(gemm_tall(ctx_, dt_of<T>(), n_, b_, b_, 1.0, W, ldw, Rinv_.p, 0.0, W, ldw, s_));
     }
     return FMX_OK;
 }
