@@ -131,7 +131,7 @@ def main():
         del big
     del src, dst
     info = spm.info
-    log("built: n=%d nnz=%d tiles=%d image=%d B (%.2f s incl. host validation)"
+    log("built: n=%d nnz=%d tiles=%d image=%d B (%.2f s for the whole build call)"
         % (info.nrows, info.nnz, info.ntiles, info.image_bytes, t_build))
     g = torch.Generator(device="cuda")
     g.manual_seed(SEED)
