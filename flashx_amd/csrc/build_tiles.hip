@@ -513,6 +513,20 @@ fmx_status build_impl(fmx_ctx *ctx, const uint32_t *src, const uint32_t *dst, si
         K = other;
         nk = nu;
     }
+    // the key (and payload) buffer that does not hold the result is not read
+    // again: its memory goes back before the run and tile arrays are made
+    // (17 GB at 2^31 edges)
+    FMX_HIP(ctx, hipStreamSynchronize(s));
+    for (DevBuf<uint64_t> *b : {&keys, &tmp})
+        if (b->p && b->p != K) {
+            FMX_HIP(ctx, hipFree(b->p));
+            b->p = nullptr;
+        }
+    for (DevBuf<uint32_t> *b : {&eidx, &etmp})
+        if (b->p && b->p != V) {
+            FMX_HIP(ctx, hipFree(b->p));
+            b->p = nullptr;
+        }
 
     // 3. runs
     uint32_t R = 0;

@@ -34,9 +34,7 @@ def autotune_tile(src, dst, n, k, dtype, flags=F.BUILD_DEDUP, candidates=DEFAULT
     keep=True it also returns the fastest candidate's matrix and its build
     seconds, so the caller need not build it again."""
     torch = F._torch()
-    if X is None:
-        X = torch.rand((n, k), device=src.device, dtype=torch.float32).to(dtype)
-    Y = torch.empty((n, k), device=src.device, dtype=dtype)
+    Y = None
     times, best, best_spm, best_build = {}, None, None, None
     for lh, lw in candidates:
         torch.cuda.synchronize()
@@ -44,6 +42,10 @@ def autotune_tile(src, dst, n, k, dtype, flags=F.BUILD_DEDUP, candidates=DEFAULT
         spm = F.SparseMatrix.build(src, dst, n=n, log2_h=lh, log2_w=lw, flags=flags)
         torch.cuda.synchronize()
         build_s = time.perf_counter() - t0
+        if X is None:  # the operands after the first construction, which needs the most memory
+            X = torch.rand((n, k), device=src.device, dtype=torch.float32).to(dtype)
+        if Y is None:
+            Y = torch.empty((n, k), device=src.device, dtype=dtype)
         times[(lh, lw)] = _time_spmm(spm, X, Y, reps)
         if best is None or times[(lh, lw)] < times[best]:
             best, best_build = (lh, lw), build_s
