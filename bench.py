@@ -298,8 +298,9 @@ def construction_leg(F, torch, spm, t_build, check=True):
            "device_GBps": round(alg / dev_s / 1e9, 1) if dev_s else None,
            "pct_hbm_peak": round(100 * alg / dev_s / 1e9 / PEAK_HBM_GBS, 2) if dev_s else None,
            "call_s": round(t_build, 3),
-           "call_note": "the whole call: construction kernels, the tile directory's download and check, "
-                        "and the gather-order image the SpMM runs on (decoded, ranked and built on the device)"}
+           "call_note": "the whole call: construction kernels (with the tile directory and its check), the "
+                        "block-row offsets' download, and the gather-order image the SpMM runs on (decoded, "
+                        "ranked and built on the device)"}
     if check:
         from oracle import oracle as O
         import numpy as np

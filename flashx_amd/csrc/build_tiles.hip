@@ -279,6 +279,29 @@ __global__ void __launch_bounds__(kB) fill_dir(const uint64_t *k, KeyFmt f, cons
         }
 }
 
+// The directory's invariants, per block row (the SpMM plans and kernels
+// trust them): it has tiles, the first at word 0, word offsets strictly
+// increasing and leaving room for a tile header inside the block row, block
+// columns increasing and in range, value words (weighted) past their tile's
+// header (one past the block row for the empty tile of an empty block row).
+__global__ void __launch_bounds__(kB) check_dir(const uint2 *dir, const uint32_t *vdir, const uint32_t *bdir,
+                                                const uint64_t *offs_ex, uint64_t nbr, uint64_t nbc, uint64_t ndir,
+                                                int weighted, unsigned int *bad)
+{
+    for (uint64_t b = gtid(); b < nbr; b += gstride()) {
+        const uint32_t lo = bdir[b], hi = bdir[b + 1];
+        const int64_t words = (int64_t)(offs_ex[b + 1] - offs_ex[b]) / 2;
+        bool ok = hi > lo && hi <= ndir && words >= 8 && dir[lo].x == 0;
+        for (uint32_t i = lo; ok && i < hi; i++) {
+            const uint2 d = dir[i];
+            ok = (int64_t)d.x + 8 <= words && (uint64_t)d.y < (nbc ? nbc : 1) &&
+                 (i == lo || (d.x > dir[i - 1].x && d.y > dir[i - 1].y)) &&
+                 (!weighted || (vdir[i] >= d.x + 8 && (int64_t)vdir[i] <= words + 1));
+        }
+        if (!ok) atomicOr(bad, 1u);
+    }
+}
+
 __device__ __forceinline__ void put16(uint8_t *img, uint64_t off, uint32_t v)
 {
     *(uint16_t *)(img + off) = (uint16_t)v;
@@ -651,18 +674,23 @@ fmx_status build_impl(fmx_ctx *ctx, const uint32_t *src, const uint32_t *dst, si
                        run_start.p, tile_first_run.p, T, offs_ex.p, br_first.p, tile_off.p, bdir.p, had.p, nbr,
                        vals ? 1 : 0, spm->d_dir, spm->d_vdir);
     FMX_HIP(ctx, hipGetLastError());
+    DevBuf<unsigned int> bad;
+    FMX_HIP(ctx, bad.alloc(1));
+    FMX_HIP(ctx, hipMemsetAsync(bad.p, 0, sizeof(unsigned int), s));
+    hipLaunchKernelGGL(check_dir, dim3(grid_for(ctx, nbr)), dim3(kB), 0, s, spm->d_dir, spm->d_vdir, bdir.p,
+                       offs_ex.p, nbr, nbc, (uint64_t)ndir, vals ? 1 : 0, bad.p);
+    FMX_HIP(ctx, hipGetLastError());
     FMX_HIP(ctx, hipEventRecord(ev.b, s));
+    // the host keeps the block-row offsets and first directory indices; the
+    // directory itself stays in HBM (plans copy the few block rows they split)
     std::vector<uint64_t> oex(nbr + 1);
+    unsigned int hbad = 0;
     spm->brow_dir.resize(nbr + 1);
-    spm->dir.resize(ndir);
-    if (vals) spm->vdir.resize(ndir);
     FMX_HIP(ctx, hipMemcpyAsync(oex.data(), offs_ex.p, (nbr + 1) * 8, hipMemcpyDeviceToHost, s));
     FMX_HIP(ctx, hipMemcpyAsync(spm->brow_dir.data(), bdir.p, (nbr + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-    FMX_HIP(ctx, hipMemcpyAsync(spm->dir.data(), spm->d_dir, (size_t)ndir * sizeof(uint2), hipMemcpyDeviceToHost, s));
-    if (vals)
-        FMX_HIP(ctx, hipMemcpyAsync(spm->vdir.data(), spm->d_vdir, (size_t)ndir * sizeof(uint32_t),
-                                    hipMemcpyDeviceToHost, s));
+    FMX_HIP(ctx, hipMemcpyAsync(&hbad, bad.p, sizeof(unsigned int), hipMemcpyDeviceToHost, s));
     FMX_HIP(ctx, hipStreamSynchronize(s));
+    if (hbad) return fail(ctx, FMX_EINVAL, "internal error: inconsistent device-built directory");
     float build_ms = 0.f;
     FMX_HIP(ctx, hipEventElapsedTime(&build_ms, ev.a, ev.b));
     spm->build_ns = (uint64_t)((double)build_ms * 1e6);
@@ -682,8 +710,6 @@ fmx_status build_impl(fmx_ctx *ctx, const uint32_t *src, const uint32_t *dst, si
     spm->esz = vals ? esz : 0;
     spm->prim = vals ? prim_type : 0;
     spm->ntiles = ndir;
-    st = check_device_dir(ctx, spm.get());
-    if (st) return st;
     if (!internal) {
         st = prepare_work(ctx, spm.get());
         if (!st) st = prepare_fast(ctx, spm.get());

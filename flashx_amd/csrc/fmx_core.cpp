@@ -223,45 +223,6 @@ fmx_status analyse_image(fmx_ctx *ctx, fmx_spm *s)
     return FMX_OK;
 }
 
-// The directory of an image built on the device (no host copy to analyse):
-// every block row has tiles, its first at word 0, word offsets strictly
-// increasing and leaving room for a tile header inside the block row, block
-// columns in range, value words (weighted) past their tile's header (one
-// past the block row for the empty tile of an empty block row).  The
-// SpMM plans and kernels trust it.
-fmx_status check_device_dir(fmx_ctx *ctx, const fmx_spm *s)
-{
-    const uint64_t nbr = s->nbr;
-    if (s->brow_dir.size() != nbr + 1 || s->offs.size() != nbr + 1 || s->brow_dir[0] != 0 ||
-        s->brow_dir[nbr] != s->dir.size() || (s->esz && s->vdir.size() != s->dir.size()))
-        return fail(ctx, FMX_EINVAL, "internal error: device-built directory has the wrong size");
-    const unsigned nth = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>({16, nbr / 4096 + 1,
-                                                                               std::thread::hardware_concurrency()}));
-    std::vector<int> bad(nth, 0);
-    std::vector<std::thread> th;
-    for (unsigned t = 0; t < nth; t++)
-        th.emplace_back([&, t] {
-            for (uint64_t br = t; br < nbr && !bad[t]; br += nth) {
-                const uint32_t lo = s->brow_dir[br], hi = s->brow_dir[br + 1];
-                const int64_t words = (s->offs[br + 1] - s->offs[br]) / 2;
-                if (hi <= lo || s->dir[lo].x != 0 || words < 8) { bad[t] = 1; break; }
-                for (uint32_t i = lo; i < hi; i++) {
-                    const uint2 d = s->dir[i];
-                    if ((int64_t)d.x + 8 > words || d.y >= std::max<uint64_t>(1, s->nbc) ||
-                        (i > lo && (d.x <= s->dir[i - 1].x || d.y <= s->dir[i - 1].y)) ||
-                        (s->esz && (s->vdir[i] < d.x + 8 || (int64_t)s->vdir[i] > words + 1))) {
-                        bad[t] = 1;
-                        break;
-                    }
-                }
-            }
-        });
-    for (auto &x : th) x.join();
-    for (int b : bad)
-        if (b) return fail(ctx, FMX_EINVAL, "internal error: inconsistent device-built directory");
-    return FMX_OK;
-}
-
 uint32_t band_rows(const fmx_spm *spm, int k, int s)
 {
     const char *e = getenv("FMX_BAND_BYTES");
@@ -273,17 +234,44 @@ uint32_t band_rows(const fmx_spm *spm, int k, int s)
     return (uint32_t)std::min<uint64_t>(hs, spm->H);
 }
 
-// Moves a cut at word `p` of block row `br` out of a tile header and names
-// the tile it falls in.
-static void split_point(const fmx_spm *s, uint64_t br, uint32_t &p, uint32_t &dir_idx)
+// The directory entries of one block row at a time: from the host copy, or
+// (an image built on the device keeps its directory in HBM only) copied
+// from HBM when a plan first needs them, which is only for the block rows
+// whose bands are split.
+class DirRows {
+public:
+    DirRows(fmx_ctx *ctx, const fmx_spm *s) : ctx_(ctx), s_(s) {}
+    // entries brow_dir[br] .. brow_dir[br + 1] - 1, or null after a failed copy
+    const uint2 *row(uint64_t br)
+    {
+        const uint32_t lo = s_->brow_dir[br], hi = s_->brow_dir[br + 1];
+        if (!s_->dir.empty()) return s_->dir.data() + lo;
+        auto it = cache_.find(br);
+        if (it != cache_.end()) return it->second.data();
+        std::vector<uint2> &v = cache_[br];
+        v.resize(hi - lo);
+        if (hipMemcpy(v.data(), s_->d_dir + lo, (size_t)(hi - lo) * sizeof(uint2), hipMemcpyDeviceToHost) !=
+            hipSuccess) {
+            (void)hip_fail(ctx_, hipGetLastError(), "directory copy");
+            return nullptr;
+        }
+        return v.data();
+    }
+
+private:
+    fmx_ctx *ctx_;
+    const fmx_spm *s_;
+    std::map<uint64_t, std::vector<uint2>> cache_;
+};
+
+// The last tile of block row br (entries d, local indices 0..n-1) starting
+// at or before word p, as a directory index.
+static uint32_t tile_at(const fmx_spm *s, uint64_t br, const uint2 *d, uint32_t p)
 {
-    uint32_t lo = s->brow_dir[br], hi = s->brow_dir[br + 1];
-    // last tile whose start <= p
-    uint32_t t = (uint32_t)(std::upper_bound(s->dir.begin() + lo, s->dir.begin() + hi, p,
-                                            [](uint32_t v, const uint2 &d) { return v < d.x; })
-                            - s->dir.begin()) - 1;
-    if (p < s->dir[t].x + 8) p = s->dir[t].x + 8;
-    dir_idx = t;
+    const uint32_t n = s->brow_dir[br + 1] - s->brow_dir[br];
+    const uint32_t i = (uint32_t)(std::upper_bound(d, d + n, p, [](uint32_t v, const uint2 &e) { return v < e.x; }) -
+                                  d) - 1;
+    return s->brow_dir[br] + i;
 }
 
 // Tile-relative row in force at word `p` of the block row whose first word
@@ -334,6 +322,7 @@ fmx_status get_plan(fmx_ctx *ctx, fmx_spm *s, uint32_t hs, bool ordered, Plan **
     std::vector<size_t> carry_item;
     std::vector<uint64_t> carry_base;
     std::vector<uint32_t> carry_word;
+    DirRows dir_rows(ctx, s);
     for (const Band &b : bands) {
         uint32_t pieces = (uint32_t)(b.words / cap + (b.words % cap != 0));
         if (pieces < 1) pieces = 1;
@@ -358,23 +347,22 @@ fmx_status get_plan(fmx_ctx *ctx, fmx_spm *s, uint32_t hs, bool ordered, Plan **
         cut[0] = 0;
         dirs[0] = s->brow_dir[b.br];
         carries[0] = kNone;
+        const uint2 *d = dir_rows.row(b.br);
+        if (!d) return FMX_EHIP;
         for (uint32_t i = 1; i < pieces; i++) {
             uint32_t p = (uint32_t)((uint64_t)b.words * i / pieces);
             if (p <= cut[i - 1]) p = cut[i - 1] + 1;
+            const uint32_t t = tile_at(s, b.br, d, p);
+            const uint32_t tx = d[t - s->brow_dir[b.br]].x;
             if (s->esz) {
                 // weighted: cut at the tile start at or before p, so no row
                 // and no value position is carried into a piece
-                const uint32_t lo = s->brow_dir[b.br], hi = s->brow_dir[b.br + 1];
-                uint32_t t = (uint32_t)(std::upper_bound(s->dir.begin() + lo, s->dir.begin() + hi, p,
-                                                         [](uint32_t v, const uint2 &d) { return v < d.x; }) -
-                                        s->dir.begin()) - 1;
-                p = std::max(s->dir[t].x, cut[i - 1]);
-                dirs[i] = t;
-                carries[i] = kNone;
-            } else {
-                split_point(s, b.br, p, dirs[i]);
-                carries[i] = kNone;  // resolved below
+                p = std::max(tx, cut[i - 1]);
+            } else if (p < tx + 8) {
+                p = tx + 8;  // out of the tile header
             }
+            dirs[i] = t;
+            carries[i] = kNone;  // binary: resolved below
             cut[i] = p;
         }
         cut[pieces] = b.words;
@@ -422,7 +410,7 @@ fmx_status get_plan(fmx_ctx *ctx, fmx_spm *s, uint32_t hs, bool ordered, Plan **
     // cache footprint (time per non-zero is the same with 1/8 of X), and
     // every rotation rate tried was 5-10% slower than none.
     const char *re = getenv("FMX_ROT_SLOTS");
-    const uint64_t slots = (re && !ordered) ? strtoull(re, nullptr, 10) : 0;
+    const uint64_t slots = (re && !ordered && !s->dir.empty()) ? strtoull(re, nullptr, 10) : 0;
     for (size_t i = 0; i < plan->items.size(); i++) {
         Item &t = plan->items[i];
         t.rot_begin = t.begin;
@@ -443,8 +431,20 @@ fmx_status get_plan(fmx_ctx *ctx, fmx_spm *s, uint32_t hs, bool ordered, Plan **
     }
     // The kernel trusts these invariants; check them before anything runs.
     for (const Item &t : plan->items) {
-        const bool ok = t.nwords >= 1 && t.dir_begin < t.dir_end && t.dir_end <= s->dir.size() &&
-                        s->dir[t.dir_begin].x <= t.begin && t.rot_begin >= t.begin &&
+        // the first tile of the item starts at or before the item: the block
+        // row's first tile at word 0, or the tile a cut fell in
+        const uint64_t br = (((uint64_t)t.row0_hi << 32) | t.row0_lo) / s->H;
+        bool first_ok = br < s->nbr;
+        if (first_ok && t.begin == 0) {
+            first_ok = t.dir_begin == s->brow_dir[br];
+        } else if (first_ok) {
+            const uint2 *d = dir_rows.row(br);
+            if (!d) return FMX_EHIP;
+            first_ok = t.dir_begin >= s->brow_dir[br] && t.dir_begin < s->brow_dir[br + 1] &&
+                       d[t.dir_begin - s->brow_dir[br]].x <= t.begin;
+        }
+        const bool ok = first_ok && t.nwords >= 1 && t.dir_begin < t.dir_end && t.dir_end <= s->ntiles &&
+                        t.rot_begin >= t.begin &&
                         t.rot_begin < t.begin + t.nwords && t.rot_dir >= t.dir_begin && t.rot_dir < t.dir_end &&
                         (t.rot_begin == t.begin ? t.rot_dir == t.dir_begin : s->dir[t.rot_dir].x == t.rot_begin) &&
                         t.rows >= 1 && t.rows <= hs;

@@ -101,7 +101,7 @@ struct fmx_spm {
     uint32_t *d_vdir = nullptr;         // weighted images: values-region word of each tile
     std::vector<uint8_t> h_image;       // host copy used to build plans (resident matrices)
     std::vector<int64_t> offs;          // block-row byte offsets
-    std::vector<uint2> dir;             // host directory
+    std::vector<uint2> dir;             // host directory (empty for an image built on the device)
     std::vector<uint32_t> vdir;         // weighted images: values-region word (block-row relative)
     std::vector<uint32_t> brow_dir;     // first directory index per block row (+1)
     std::map<uint32_t, std::unique_ptr<fmx::Plan>> plans;
@@ -147,8 +147,6 @@ int elem_size(fmx_dtype dt);
 
 // Validates a host image and fills the matrix fields + directory.
 fmx_status analyse_image(fmx_ctx *ctx, fmx_spm *spm);
-// Consistency of the directory of an image built on the device.
-fmx_status check_device_dir(fmx_ctx *ctx, const fmx_spm *spm);
 // Device construction (build_tiles.hip); ncols != 0: an n x ncols matrix;
 // internal: the result gets no work images of its own.
 fmx_status build_impl(fmx_ctx *ctx, const uint32_t *src, const uint32_t *dst, size_t m, uint64_t n,
