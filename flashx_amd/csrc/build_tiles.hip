@@ -204,31 +204,50 @@ __global__ void __launch_bounds__(kB) fill_tiles(const uint64_t *k, const uint32
     if (gtid() == 0) tile_first_run[T] = R;
 }
 
-__global__ void __launch_bounds__(kB) tile_props(const uint64_t *k, KeyFmt f, const uint32_t *run_start,
-                                                 const uint32_t *tile_first_run, uint32_t T,
-                                                 const uint64_t *scsr_ex, const uint32_t *coo_ex,
-                                                 uint64_t *tsize, unsigned long long *brsize, uint32_t esz)
+__global__ void __launch_bounds__(kB) tile_props(const uint32_t *run_start, const uint32_t *tile_first_run,
+                                                 uint32_t T, uint64_t *tsize, uint32_t esz)
 {
     for (uint64_t t = gtid(); t < T; t += gstride()) {
         const uint32_t fr = tile_first_run[t], lr = tile_first_run[t + 1];
         const uint64_t nnz = run_start[lr] - run_start[fr];
         const uint64_t nrow = lr - fr;
-        const uint64_t size = 16 + 2 * (nrow + nnz + 1) + (uint64_t)esz * nnz;
-        tsize[t] = size;
-        atomicAdd(&brsize[f.br(k[run_start[fr]])], (unsigned long long)size);
+        tsize[t] = 16 + 2 * (nrow + nnz + 1) + (uint64_t)esz * nnz;
     }
     if (gtid() == 0) tsize[T] = 0;
 }
 
-// empty block rows take a 16-byte empty tile; keep a copy of "had tiles"
+// Per block row with tiles (tiles are in block-row order): its bytes, from
+// the scanned tile sizes, and its directory entries, written by the row's
+// last tile.
+__global__ void __launch_bounds__(kB) row_totals(const uint64_t *k, KeyFmt f, const uint32_t *run_start,
+                                                 const uint32_t *tile_first_run, uint32_t T,
+                                                 const uint64_t *size_ex, const uint32_t *br_first,
+                                                 unsigned long long *brsize, uint32_t *ndir)
+{
+    for (uint64_t t = gtid(); t < T; t += gstride()) {
+        const uint32_t b = f.br(k[run_start[tile_first_run[t]]]);
+        if (t + 1 < T && f.br(k[run_start[tile_first_run[t + 1]]]) == b) continue;
+        brsize[b] = size_ex[t + 1] - size_ex[br_first[b]];
+        ndir[b] = (uint32_t)(t + 1 - br_first[b]);
+    }
+}
+
+// empty block rows take a 16-byte empty tile (one directory entry); keep a
+// copy of "had tiles"
 __global__ void __launch_bounds__(kB) fix_empty_rows(unsigned long long *brsize, uint64_t nbr,
-                                                     uint8_t *had)
+                                                     uint8_t *had, uint32_t *ndir)
 {
     for (uint64_t b = gtid(); b < nbr; b += gstride()) {
         had[b] = brsize[b] != 0;
-        if (!brsize[b]) brsize[b] = 16;
+        if (!brsize[b]) {
+            brsize[b] = 16;
+            ndir[b] = 1;
+        }
     }
-    if (gtid() == 0) brsize[nbr] = 0;
+    if (gtid() == 0) {
+        brsize[nbr] = 0;
+        ndir[nbr] = 0;
+    }
 }
 
 __global__ void __launch_bounds__(kB) first_tile_of_row(const uint64_t *k, KeyFmt f,
@@ -248,15 +267,6 @@ __global__ void __launch_bounds__(kB) first_tile_of_row(const uint64_t *k, KeyFm
 // tiles, or 1 for the empty tile of an empty block row), per directory entry
 // the tile's word offset inside its block row and its block column, and for
 // weighted images the word where the tile's values start.
-__global__ void __launch_bounds__(kB) dir_counts(const uint64_t *k, KeyFmt f, const uint32_t *run_start,
-                                                 const uint32_t *tile_first_run, uint32_t T, const uint8_t *had,
-                                                 uint64_t nbr, uint32_t *cnt)
-{
-    for (uint64_t t = gtid(); t < T; t += gstride()) atomicAdd(&cnt[f.br(k[run_start[tile_first_run[t]]])], 1u);
-    for (uint64_t b = gtid(); b < nbr; b += gstride())
-        if (!had[b]) cnt[b] = 1;
-}
-
 __global__ void __launch_bounds__(kB) fill_dir(const uint64_t *k, KeyFmt f, const uint32_t *run_start,
                                                const uint32_t *tile_first_run, uint32_t T, const uint64_t *offs_ex,
                                                const uint32_t *br_first, const uint64_t *tile_off,
@@ -595,27 +605,30 @@ fmx_status build_impl(fmx_ctx *ctx, const uint32_t *src, const uint32_t *dst, si
     FMX_HIP(ctx, offs_ex.alloc(nbr + 1));
     FMX_HIP(ctx, br_first.alloc(nbr + 1));
     FMX_HIP(ctx, had.alloc(nbr + 1));
+    DevBuf<uint32_t> bdir;  // directory entries per block row, then their exclusive scan
+    FMX_HIP(ctx, bdir.alloc(nbr + 1));
     FMX_HIP(ctx, hipMemsetAsync(brsize.p, 0, (nbr + 1) * sizeof(unsigned long long), s));
     if (T) {
         hipLaunchKernelGGL(fill_tiles, dim3(grid_for(ctx, R)), dim3(kB), 0, s, K, run_start.p, R, tshift,
                            tflag.p, tile_first_run.p, tile_of_run.p, T);
         FMX_HIP(ctx, hipGetLastError());
-        hipLaunchKernelGGL(tile_props, dim3(grid_for(ctx, T)), dim3(kB), 0, s, K, f, run_start.p,
-                           tile_first_run.p, T, scsr_ex.p, coo_ex.p, tsize.p, brsize.p, vals ? esz : 0u);
+        hipLaunchKernelGGL(tile_props, dim3(grid_for(ctx, T)), dim3(kB), 0, s, run_start.p, tile_first_run.p, T,
+                           tsize.p, vals ? esz : 0u);
         FMX_HIP(ctx, hipGetLastError());
+        st = scan_exclusive<uint64_t>(ctx, tsize.p, tsize.p, (uint64_t)T + 1, nullptr, s);
+        if (st) return st;
         hipLaunchKernelGGL(first_tile_of_row, dim3(grid_for(ctx, T)), dim3(kB), 0, s, K, f, run_start.p,
                            tile_first_run.p, T, br_first.p);
         FMX_HIP(ctx, hipGetLastError());
+        hipLaunchKernelGGL(row_totals, dim3(grid_for(ctx, T)), dim3(kB), 0, s, K, f, run_start.p,
+                           tile_first_run.p, T, tsize.p, br_first.p, brsize.p, bdir.p);
+        FMX_HIP(ctx, hipGetLastError());
     }
-    hipLaunchKernelGGL(fix_empty_rows, dim3(grid_for(ctx, nbr)), dim3(kB), 0, s, brsize.p, nbr, had.p);
+    hipLaunchKernelGGL(fix_empty_rows, dim3(grid_for(ctx, nbr)), dim3(kB), 0, s, brsize.p, nbr, had.p, bdir.p);
     FMX_HIP(ctx, hipGetLastError());
     uint64_t tiles_total = 0;
     st = scan_exclusive<uint64_t>(ctx, (const uint64_t *)brsize.p, offs_ex.p, nbr + 1, &tiles_total, s);
     if (st) return st;
-    if (T) {
-        st = scan_exclusive<uint64_t>(ctx, tsize.p, tsize.p, (uint64_t)T + 1, nullptr, s);
-        if (st) return st;
-    }
 
     // 5. serialise
     std::unique_ptr<fmx_spm> spm(new fmx_spm());
@@ -657,14 +670,8 @@ fmx_status build_impl(fmx_ctx *ctx, const uint32_t *src, const uint32_t *dst, si
                        spm->d_image);
     FMX_HIP(ctx, hipGetLastError());
 
-    // 6. the directory, in HBM where the SpMM reads it and on the host for
-    // the plans
-    DevBuf<uint32_t> bdir;
-    FMX_HIP(ctx, bdir.alloc(nbr + 1));
-    FMX_HIP(ctx, hipMemsetAsync(bdir.p, 0, (nbr + 1) * sizeof(uint32_t), s));
-    hipLaunchKernelGGL(dir_counts, dim3(grid_for(ctx, std::max<uint64_t>(T, nbr))), dim3(kB), 0, s, K, f,
-                       run_start.p, tile_first_run.p, T, had.p, nbr, bdir.p);
-    FMX_HIP(ctx, hipGetLastError());
+    // 6. the directory, in HBM where the SpMM reads it (the host keeps each
+    // block row's first directory index)
     uint32_t ndir = 0;
     st = scan_exclusive<uint32_t>(ctx, bdir.p, bdir.p, nbr + 1, &ndir, s);
     if (st) return st;

@@ -25,7 +25,8 @@
 //
 // plan_carries: for a word position inside a block row, the row of the SCSR
 // row part in force there (the word is a column index) or none (it is a
-// header), which a work item that starts there carries in.
+// header), which a work item that starts there carries in (one wave per
+// position, searching back 64 words at a time).
 #include "devprim.h"
 
 namespace fmx {
@@ -34,10 +35,8 @@ namespace {
 constexpr int kT = 256;
 constexpr uint64_t kMaxUses = 1ull << 31;  // non-zeros of a gather-order image stay below 2^31
 
-__device__ __forceinline__ uint32_t ld16(const uint8_t *p)
-{
-    return (uint32_t)p[0] | ((uint32_t)p[1] << 8);
-}
+// image words are 2-byte aligned (tiles start at even byte offsets)
+__device__ __forceinline__ uint32_t ld16(const uint8_t *p) { return *(const uint16_t *)p; }
 __device__ __forceinline__ uint32_t ld32(const uint8_t *p) { return ld16(p) | (ld16(p + 2) << 16); }
 
 __global__ void __launch_bounds__(kT) tile_origin(const uint32_t *brow_dir, uint64_t nbr, uint32_t *tbr)
@@ -100,14 +99,14 @@ __global__ void __launch_bounds__(kT) ranked_uses(const uint64_t *keys, uint64_t
 
 // Rank i is hot while the uses before it fall short of the share and it has
 // uses itself; both conditions hold on a prefix of the ranks, so nhot is the
-// first rank that fails (ncols if none does).
+// first rank that fails (ncols if none does): the one rank that fails while
+// its predecessor passes writes it.
 __global__ void __launch_bounds__(kT) cut(const uint64_t *keys, const uint64_t *before, uint64_t ncols,
                                           uint64_t target_permille, uint64_t *nhot)
 {
-    for (uint64_t i = blockIdx.x * (uint64_t)kT + threadIdx.x; i < ncols; i += (uint64_t)gridDim.x * kT) {
-        const uint64_t u = kMaxUses - (keys[i] >> 32);
-        if (!(before[i] * 1000 < target_permille && u > 0)) atomicMin((unsigned long long *)nhot, i);
-    }
+    auto hot = [&](uint64_t i) { return before[i] * 1000 < target_permille && (keys[i] >> 32) < kMaxUses; };
+    for (uint64_t i = blockIdx.x * (uint64_t)kT + threadIdx.x; i < ncols; i += (uint64_t)gridDim.x * kT)
+        if (!hot(i) && (i == 0 || hot(i - 1))) *nhot = i;
 }
 
 __global__ void __launch_bounds__(kT) new_ids(uint64_t ncols, uint64_t nhot, uint32_t *newid)
@@ -131,22 +130,36 @@ __global__ void __launch_bounds__(kT) relabel(uint32_t *cols, uint64_t nnz, cons
         cols[e] = newid[cols[e]];
 }
 
-// carry[i] for word p[i] of the block row starting at word base[i], whose
-// tile starts at word tstart[i] (p[i] is past that tile's 8-word header):
-// kNone if the word is a header (row part or COO entry), else the row of
-// the nearest row-part header before it.
+// carry[i] for word p[i] of the block row starting at word base[i] (p[i]
+// lies past its tile's 8-word header): kNone if the word is a header (row
+// part or COO entry), else the row of the nearest row-part header before it.
+// One wave per cut: the lanes read 64 words back at a time and a ballot
+// finds the nearest header (a hub row's part can be thousands of words long;
+// the word after the tile header always has the header bit, so the search
+// ends inside the tile).
 __global__ void __launch_bounds__(kT) carries(const uint16_t *img, const uint64_t *base, const uint32_t *p,
                                               uint64_t n, uint32_t *carry)
 {
-    for (uint64_t i = blockIdx.x * (uint64_t)kT + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kT) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t nw = ((uint64_t)gridDim.x * kT) >> 6;
+    for (uint64_t i = (blockIdx.x * (uint64_t)kT + threadIdx.x) >> 6; i < n; i += nw) {
         const uint16_t *w = img + base[i];
-        uint32_t q = p[i];
+        const int64_t q = p[i];
         if (w[q] & 0x8000u) {
-            carry[i] = kNone;
+            if (lane == 0) carry[i] = kNone;
             continue;
         }
-        while (!(w[q - 1] & 0x8000u)) q--;
-        carry[i] = w[q - 1] & 0x7FFFu;
+        for (int64_t hi = q - 1;; hi -= 64) {
+            const int64_t at = hi - lane;
+            const uint32_t v = at >= 0 ? w[at] : 0x8000u;
+            const uint64_t m = __ballot((v & 0x8000u) != 0);
+            if (m) {
+                const int l0 = __ffsll((unsigned long long)m) - 1;  // nearest to q
+                const uint32_t head = __shfl(v, l0, 64);
+                if (lane == 0) carry[i] = head & 0x7FFFu;
+                break;
+            }
+        }
     }
 }
 
@@ -249,7 +262,7 @@ fmx_status plan_carries(fmx_ctx *ctx, const fmx_spm *s, const std::vector<uint64
     FMX_HIP(ctx, dc.alloc(n));
     FMX_HIP(ctx, hipMemcpyAsync(db.p, base.data(), n * sizeof(uint64_t), hipMemcpyHostToDevice, st));
     FMX_HIP(ctx, hipMemcpyAsync(dp.p, p.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(carries, dim3(grid_for(ctx, n)), dim3(kT), 0, st, (const uint16_t *)s->d_image, db.p, dp.p, n,
+    hipLaunchKernelGGL(carries, dim3(grid_for(ctx, n * 64)), dim3(kT), 0, st, (const uint16_t *)s->d_image, db.p, dp.p, n,
                        dc.p);
     FMX_HIP(ctx, hipGetLastError());
     FMX_HIP(ctx, hipMemcpyAsync(carry.data(), dc.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
