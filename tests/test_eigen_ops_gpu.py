@@ -9,7 +9,7 @@ solver's own report:
     for the SVD operator (np.linalg.svd), the largest algebraic eigenvalues
     of A for the plain one (np.linalg.eigvalsh), and for the SVD operator of
     a symmetric A (no transpose given) the largest squares of A's
-    eigenvalues;
+    eigenvalues; at R-MAT scale 16, scipy's ARPACK singular values;
   * each returned pair's residual ||N x - theta x|| / |theta| is recomputed in
     FP64 from the returned vectors, with N applied by the oracle's CPU SpMM of
     the downloaded images (A, then A's transpose for the SVD operator);
@@ -111,3 +111,33 @@ def test_operator_errors(fmx, directed, symmetric):
         fmx.compute_eigen(S, nev=4, block_size=8, op="plain", At=S)  # only the SVD takes a transpose
     with pytest.raises(fmx.FmxError):
         fmx.compute_eigen(S, nev=4, block_size=8, op="bogus")
+
+
+def test_svd_operator_rmat16_vs_arpack(fmx, orc):
+    """A larger directed case: R-MAT scale 16 (65,536 vertices, about 1 M
+    non-zeros) against scipy's ARPACK singular values of the oracle-decoded
+    matrix, with every pair's residual recomputed through the oracle's SpMM of
+    both images."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as sla
+    n = 1 << 16
+    s, d = fmx.rmat_edges(16, 16 << 16, 9)
+    A = fmx.SparseMatrix.build(s, d, n=n, log2_h=6, log2_w=15, flags=fmx.BUILD_DEDUP)
+    At = fmx.SparseMatrix.build(d, s, n=n, log2_h=6, log2_w=15, flags=fmx.BUILD_DEDUP)
+    del s, d
+    data, offs = A.download()
+    img = orc.Image(data, offs, n)
+    tdata, toffs = At.download()
+    timg = orc.Image(tdata, toffs, n)
+    r, c = orc.nonzeros(img)
+    As = sp.csr_matrix((np.ones(r.size), (r.astype(np.int64), c.astype(np.int64))), shape=(n, n))
+    tol = 1e-8
+    evals, vecs, resid, st = fmx.compute_eigen(A, nev=NEV, block_size=8, num_blocks=6, tol=tol,
+                                               precision=fmx.EIG_FP64, op="svd", At=At, max_restarts=400)
+    assert st["converged"] == 1
+    check_pairs(evals, vecs, lambda V: orc.spmm(timg, np.ascontiguousarray(orc.spmm(img, V))), tol)
+    sv = np.sort(sla.svds(As, k=NEV, tol=1e-12, return_singular_vectors=False))[::-1]
+    want = sv ** 2
+    print("scale 16: squared singular values %s (ARPACK %s), %d block SpMMs"
+          % (np.array2string(evals, precision=8), np.array2string(want, precision=8), st["spmm_blocks"]))
+    assert np.all(np.abs(evals - want) <= 1e-7 * want)
