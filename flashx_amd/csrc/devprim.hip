@@ -140,14 +140,24 @@ __global__ void __launch_bounds__(kT) rs_count(const uint64_t *keys, uint64_t n,
     hist[(uint64_t)threadIdx.x * nb + blockIdx.x] = cnt[threadIdx.x];
 }
 
-// Scatter of one pass; the optional u32 payload (vin -> vout) moves with its key.
+// Scatter of one pass; the optional u32 payload (vin -> vout) moves with its
+// key.  The tile's keys are first ranked into digit order in LDS (equal
+// digits ranked inside each wave with ballots, waves and 64-key chunks in
+// order, so equal keys keep their order: the sort is stable), then written
+// out in that order, so consecutive threads write consecutive slots of each
+// digit's run in the output (coalesced).
 template <bool KV>
 __global__ void __launch_bounds__(kT) rs_scatter_t(const uint64_t *in, uint64_t *out, const uint32_t *vin,
                                                    uint32_t *vout, uint64_t n, int shift, const uint64_t *offs,
                                                    uint32_t nb)
 {
-    __shared__ uint32_t wcnt[kWaves][256];
-    __shared__ uint64_t wbase[kWaves][256];
+    __shared__ uint32_t wcnt[kWaves][256];   // keys of each digit per wave
+    __shared__ uint32_t lbase[kWaves][256];  // next tile slot of each digit per wave
+    __shared__ uint32_t dstart[256];         // first tile slot of each digit
+    __shared__ uint64_t gstart[256];         // first output slot of this tile's keys of each digit
+    __shared__ uint32_t wl[kWaves];
+    __shared__ uint64_t sk[kTile];           // the tile's keys in digit order
+    __shared__ uint32_t sv[KV ? kTile : 1];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     for (int i = threadIdx.x; i < kWaves * 256; i += kT) (&wcnt[0][0])[i] = 0;
     __syncthreads();
@@ -165,12 +175,18 @@ __global__ void __launch_bounds__(kT) rs_scatter_t(const uint64_t *in, uint64_t 
     __syncthreads();
     {
         const int d = threadIdx.x;  // one digit per thread
-        uint64_t run = offs[(uint64_t)d * nb + blockIdx.x];
+        uint32_t tot = 0;
+#pragma unroll
+        for (int ww = 0; ww < kWaves; ww++) tot += wcnt[ww][d];
+        uint32_t all;
+        uint32_t run = block_exclusive(tot, wl, all);
+        dstart[d] = run;
 #pragma unroll
         for (int ww = 0; ww < kWaves; ww++) {
-            wbase[ww][d] = run;
+            lbase[ww][d] = run;
             run += wcnt[ww][d];
         }
+        gstart[d] = offs[(uint64_t)d * nb + blockIdx.x];
     }
     __syncthreads();
     const uint64_t lt = (1ull << lane) - 1ull;
@@ -186,16 +202,26 @@ __global__ void __launch_bounds__(kT) rs_scatter_t(const uint64_t *in, uint64_t 
             const uint64_t bm = __ballot(valid && bit);
             peers &= bit ? bm : ~bm;
         }
-        const uint64_t pos = wbase[w][d] + __popcll(peers & lt);
+        const uint32_t lpos = lbase[w][d] + __popcll(peers & lt);
         if (valid) {
-            out[pos] = k[c];
-            if constexpr (KV) vout[pos] = pv[c];
+            sk[lpos] = k[c];
+            if constexpr (KV) sv[lpos] = pv[c];
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
-        if (valid && (peers & lt) == 0) wbase[w][d] += __popcll(peers);
+        if (valid && (peers & lt) == 0) lbase[w][d] += __popcll(peers);
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+    const uint64_t t0 = (uint64_t)blockIdx.x * kTile;
+    const uint32_t nt = (uint32_t)min((uint64_t)kTile, n - t0);
+    for (uint32_t j = threadIdx.x; j < nt; j += kT) {
+        const uint64_t key = sk[j];
+        const uint32_t d = (uint32_t)(key >> shift) & 0xFFu;
+        const uint64_t pos = gstart[d] + (j - dstart[d]);
+        out[pos] = key;
+        if constexpr (KV) vout[pos] = sv[j];
     }
 }
 
