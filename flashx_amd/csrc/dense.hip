@@ -176,15 +176,20 @@ struct Stage {
             reg[p] = v;
         }
     }
-    // into LDS rows of stride LS elements
+    // into LDS rows of stride LS elements (16-byte stores when the rows are
+    // 16-byte aligned; the caller's slice is)
     template <int LS>
     __device__ __forceinline__ void store(T *lds, int lane) const
     {
 #pragma unroll
         for (int p = 0; p < NP; p++) {
             const int e = (p * 64 + lane) * V, row = e / KP, col = e - row * KP;
+            if constexpr ((LS * sizeof(T)) % 16 == 0) {
+                *(Vec16<T> *)(lds + row * LS + col) = reg[p];
+            } else {
 #pragma unroll
-            for (int u = 0; u < V; u++) lds[row * LS + col + u] = reg[p].v[u];
+                for (int u = 0; u < V; u++) lds[row * LS + col + u] = reg[p].v[u];
+            }
         }
     }
 };
@@ -303,7 +308,10 @@ __global__ void __launch_bounds__(kT) gemm_stage_kernel(const T *A, int64_t lda,
                                                         int veca, unsigned short *Yb, int64_t ldyb,
                                                         const double *ds, TY *Ys, int64_t ldys)
 {
-    constexpr int LA = KAP + 1;       // odd stride: the 16 rows of a k column hit distinct banks
+    // LDS row stride: FP32 rows KAP + 4 floats put the 16 rows of a k column
+    // on 16 distinct 4-bank groups (and keep rows 16-byte aligned for vector
+    // stores); FP64 rows an odd count of doubles
+    constexpr int LA = sizeof(T) == 4 ? KAP + 4 : KAP + 1;
     constexpr int KS = KAP / 4;       // MFMA K steps
     extern __shared__ __align__(16) unsigned char smg[];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -952,7 +960,7 @@ fmx_status gemm_staged(fmx_ctx *ctx, uint64_t n, int ka, int kc, double alpha, c
 {
     if (n == 0) return FMX_OK;
     const int kap = ka <= 16 ? 16 : ka <= 32 ? 32 : ka <= 64 ? 64 : ka <= 96 ? 96 : 128;
-    const size_t lds = (size_t)4 * kStageRows * (kap + 1) * sizeof(T);
+    const size_t lds = (size_t)4 * kStageRows * (kap + (sizeof(T) == 4 ? 4 : 1)) * sizeof(T);
     uint64_t waves = (n + 255) / 256;
     // up to 32 waves per CU (LDS permitting): each wave has one 16-row stage
     // in flight, so the grid must be wide to keep enough rows in flight
