@@ -197,8 +197,8 @@ struct Stage {
 // G = A^T B, ka <= 16*TI, kb <= 16*TJ: per 16 staged rows, four K = 4 steps.
 // SYM (B is A: same pointer, stride and width, TI == TJ) stages the rows once
 // and reads both MFMA operands from them.
-template <typename T, int TI, int TJ, bool SYM = false>
-__global__ void __launch_bounds__(kT) gram_stage_kernel(const T *A, int64_t lda, const T *B, int64_t ldb,
+template <typename T, int TI, int TJ, bool SYM>
+__device__ __forceinline__ void gram_stage_body(const T *A, int64_t lda, const T *B, int64_t ldb,
                                                         uint64_t n, int ka, int kb, uint64_t rows_per_wave,
                                                         uint64_t nwaves, int veca, int vecb, double *partial)
 {
@@ -296,6 +296,25 @@ __global__ void __launch_bounds__(kT) gram_stage_kernel(const T *A, int64_t lda,
                     if (i < ka && j < kb) partial[(uint64_t)blockIdx.x * nout + i * kb + j] = acc[ti * TJ + tj][q];
                 }
     }
+}
+
+// The Gram kernel, and the same kernel asked to fit three waves per SIMD (168
+// registers) for the FP32 shapes whose body fits that budget without
+// spilling (at most 8 output tiles); the launcher runs the latter with up to
+// 12 waves per CU.
+template <typename T, int TI, int TJ, bool SYM = false>
+__global__ void __launch_bounds__(kT) gram_stage_kernel(const T *A, int64_t lda, const T *B, int64_t ldb,
+                                                        uint64_t n, int ka, int kb, uint64_t rows_per_wave,
+                                                        uint64_t nwaves, int veca, int vecb, double *partial)
+{
+    gram_stage_body<T, TI, TJ, SYM>(A, lda, B, ldb, n, ka, kb, rows_per_wave, nwaves, veca, vecb, partial);
+}
+template <typename T, int TI, int TJ, bool SYM = false>
+__global__ void __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(3))) gram_stage_kernel3(const T *A, int64_t lda, const T *B, int64_t ldb,
+                                                        uint64_t n, int ka, int kb, uint64_t rows_per_wave,
+                                                        uint64_t nwaves, int veca, int vecb, double *partial)
+{
+    gram_stage_body<T, TI, TJ, SYM>(A, lda, B, ldb, n, ka, kb, rows_per_wave, nwaves, veca, vecb, partial);
 }
 
 // Y = alpha A C + beta Y for kc <= 16*TJ, ka <= KAP: per 16 staged rows of A,
@@ -862,9 +881,16 @@ fmx_status gram(fmx_ctx *ctx, fmx_dtype dt, int mode, uint64_t n, int ka, int kb
     }
     if (tjn <= 4 && tip * tjp <= kGramTiles && std::max(stage_lds, (size_t)2 * nout * sizeof(double)) <= 96 * 1024 &&
         (dt == FMX_F32 || dt == FMX_F64)) {
-        // waves of at least 1024 rows (a multiple of 16), at most 8 per CU
+        // A^T A with one staged operand (the Cholesky-QR Grams)
+        const bool sym = A == B && lda == ldb && ka == kb && tip == tjp && tip <= 2;
+        // FP32 A^T A: the three-waves-per-SIMD kernel, up to 12 waves per CU
+        // (1.2x on the eigensolver's 16- and 32-column Grams); otherwise at
+        // most 8 per CU (the 48- to 96-row products ran 5-8% slower with three
+        // waves, and more waves only add partials to reduce)
+        const bool occ3 = dt == FMX_F32 && sym;
+        // waves of at least 1024 rows (a multiple of 16)
         uint64_t waves = (n + 1023) / 1024;
-        uint64_t cap = (uint64_t)ctx->num_cus * 8;  // more waves: more partials, no faster
+        uint64_t cap = (uint64_t)ctx->num_cus * (occ3 ? 12 : 8);
         if (fixed_ws) {
             const uint64_t fit = fixed_bytes / ((size_t)nout * sizeof(double));  // blocks whose partials fit
             if (fit == 0) return fail(ctx, FMX_EDIM, "Gram output larger than the fixed scratch");
@@ -900,7 +926,8 @@ fmx_status gram(fmx_ctx *ctx, fmx_dtype dt, int mode, uint64_t n, int ka, int kb
                 (const void *)gram_stage_kernel<double, 16, 1>, (const void *)gram_stage_kernel<double, 1, 2>,
                 (const void *)gram_stage_kernel<double, 2, 2>, (const void *)gram_stage_kernel<double, 4, 2>,
                 (const void *)gram_stage_kernel<double, 8, 2>, (const void *)gram_stage_kernel<double, 1, 4>,
-                (const void *)gram_stage_kernel<double, 2, 4>, (const void *)gram_stage_kernel<double, 4, 4>};
+                (const void *)gram_stage_kernel<double, 2, 4>, (const void *)gram_stage_kernel<double, 4, 4>,
+                (const void *)gram_stage_kernel3<float, 1, 1, true>, (const void *)gram_stage_kernel3<float, 2, 2, true>};
             for (const void *k : ks)
                 FMX_HIP(ctx, hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024));
             attr_g = true;
@@ -919,13 +946,11 @@ fmx_status gram(fmx_ctx *ctx, fmx_dtype dt, int mode, uint64_t n, int ka, int kb
         else if (tip == 8 || J == 2) FMX_GRAMS(TT, 8, (J == 2 ? 2 : 1));                                      \
         else FMX_GRAMS(TT, 16, 1);                                                                            \
     } while (0)
-        // A^T A with one staged operand (the Cholesky-QR Grams)
-        const bool sym = A == B && lda == ldb && ka == kb && tip == tjp && tip <= 2;
         if (sym && dt == FMX_F32) {
-            if (tip == 1) hipLaunchKernelGGL((gram_stage_kernel<float, 1, 1, true>), dim3(nblk), dim3(kT), glds, s,
+            if (tip == 1) hipLaunchKernelGGL((gram_stage_kernel3<float, 1, 1, true>), dim3(nblk), dim3(kT), glds, s,
                                              (const float *)A, lda, (const float *)B, ldb, n, ka, kb, rpw, waves,
                                              veca, vecb, (double *)ws);
-            else hipLaunchKernelGGL((gram_stage_kernel<float, 2, 2, true>), dim3(nblk), dim3(kT), glds, s,
+            else hipLaunchKernelGGL((gram_stage_kernel3<float, 2, 2, true>), dim3(nblk), dim3(kT), glds, s,
                                     (const float *)A, lda, (const float *)B, ldb, n, ka, kb, rpw, waves, veca,
                                     vecb, (double *)ws);
         } else if (sym) {
@@ -947,8 +972,12 @@ fmx_status gram(fmx_ctx *ctx, fmx_dtype dt, int mode, uint64_t n, int ka, int kb
 #undef FMX_GRAMS_I
 #undef FMX_GRAMS
         FMX_HIP(ctx, hipGetLastError());
-        hipLaunchKernelGGL(reduce_partials4, dim3((nout + 63) / 64), dim3(kT), 0, s, (const double *)ws,
-                           (int)nblk, nout, G);
+        if (nblk > 256)  // many partials: 16 segments per output instead of 4
+            hipLaunchKernelGGL(reduce_partials_seg<double>, dim3((nout + 15) / 16), dim3(kT), 0, s,
+                               (const double *)ws, (int)nblk, nout, G);
+        else
+            hipLaunchKernelGGL(reduce_partials4, dim3((nout + 63) / 64), dim3(kT), 0, s, (const double *)ws,
+                               (int)nblk, nout, G);
         FMX_HIP(ctx, hipGetLastError());
         return FMX_OK;
     }
