@@ -111,6 +111,7 @@ private:
     Dev<double> dinv_, H_, S_, ev_, C1_, C2_, G_, Rinv_, info_, X64_, Y64_, Z64_, M64_, Ssel_, Scols_, theta_, rn_,
         part_;
     Dev<float> deg_;
+    Dev<unsigned char> rot_;  // the Rayleigh-Ritz rotation log
     int spmm_blocks_ = 0, spmm64_blocks_ = 0;
     int history_len_ = 0;
     void record(double r)
@@ -138,6 +139,7 @@ fmx_status Solver<T>::alloc()
     FMX_HIP(ctx_, hipMalloc(&G_.p, (size_t)b_ * b_ * sizeof(double)));
     FMX_HIP(ctx_, hipMalloc(&Rinv_.p, (size_t)b_ * b_ * sizeof(double)));
     FMX_HIP(ctx_, hipMalloc(&info_.p, 4 * sizeof(double)));
+    FMX_HIP(ctx_, hipMalloc(&rot_.p, small_sym_eig_ws(m_)));
     FMX_HIP(ctx_, hipMalloc(&X64_.p, n * o_.nev * sizeof(double)));
     FMX_HIP(ctx_, hipMalloc(&Y64_.p, n * o_.nev * sizeof(double)));
     FMX_HIP(ctx_, hipMalloc(&Z64_.p, n * o_.nev * sizeof(double)));
@@ -313,7 +315,7 @@ fmx_status Solver<T>::run(const void *X0, int64_t ldx0, fmx_dtype x0_dt, double 
         }
         // Rayleigh-Ritz; with it the Gram of the last residual block W, for
         // the residual estimates of the Lanczos relation N V = V H + W E^T
-        TRY(small_sym_eig(ctx_, H_.p, m, ev_.p, S_.p, info_.p, s_));
+        TRY(small_sym_eig(ctx_, H_.p, m, ev_.p, S_.p, info_.p, rot_.p, s_));
         TRY(gram(ctx_, dt_of<T>(), FMX_GRAM_F64, n_, b, b, W_.p, b, W_.p, b, G_.p, s_));
         FMX_HIP(ctx_, hipMemcpyAsync(hev.data(), ev_.p, m * sizeof(double), hipMemcpyDeviceToHost, s_));
         FMX_HIP(ctx_, hipMemcpyAsync(hS.data(), S_.p, (size_t)m * m * sizeof(double), hipMemcpyDeviceToHost, s_));

@@ -7,11 +7,15 @@
 
 namespace fmx {
 
-// Eigen-decomposition of the symmetric m x m matrix H (device, row-major):
-// evals[m] unsorted, V[m x m] eigenvectors in columns; info[0] sweeps,
-// info[1] final relative off-diagonal norm.
+// Eigen-decomposition of the symmetric m x m matrix H (device, row-major,
+// m even, m <= 128): evals[m] unsorted, V[m x m] eigenvectors in columns;
+// info[0] sweeps, info[1] final relative off-diagonal norm.  ws holds
+// small_sym_eig_ws(m) bytes (the logged rotations of at most kJacobiSweeps
+// sweeps).
+constexpr int kJacobiSweeps = 40;
+size_t small_sym_eig_ws(int m);
 fmx_status small_sym_eig(fmx_ctx *ctx, const double *H, int m, double *evals, double *V,
-                         double *info, hipStream_t s);
+                         double *info, void *ws, hipStream_t s);
 // G (b x b) = R^T R, Rinv = R^-1 upper; info[0] = first bad pivot (1-based) or 0.
 fmx_status small_chol_inv(fmx_ctx *ctx, const double *G, int b, double *Rinv, double *info,
                           hipStream_t s);

@@ -13,33 +13,31 @@ namespace {
 constexpr int kJT = 1024;  // threads of the Jacobi workgroup (16 waves)
 
 // Cyclic Jacobi for a symmetric m x m matrix (m even, m <= 128), A in LDS.
-// Each round rotates m/2 disjoint pairs (round-robin ordering), rows first
-// then columns, so A' = P^T A P exactly as the sequential method would.
-// LDS rows are m + 1 doubles apart, so the column updates (one row per
-// thread) spread over the banks.  With VL the eigenvector matrix V is kept in
-// LDS after A (same padded stride) and copied out at the end; otherwise V is
-// updated in place in global memory (row-major, stride m).
-template <bool VL>
-__global__ void __launch_bounds__(kJT) jacobi_kernel(const double *Ain, int m, double *evals,
-                                                     double *Vout, int max_sweeps, double *info)
+// Each round rotates m/2 disjoint pairs (round-robin ordering: index 0 stays,
+// the others move one place per round), rows first then columns, so
+// A' = P^T A P exactly as the sequential method would.  LDS rows are m + 1
+// doubles apart, so the column updates (one row per thread) spread over the
+// banks.  Round t's rotations go to rot[t * m/2 + pair] as (c, s): the
+// eigenvector matrix is their product, which jacobi_vectors_kernel forms
+// row by row on other CUs afterwards (rows of V are independent), so this
+// single workgroup, the serial part, carries only A.  info[0] = sweeps that
+// rotated (the rounds logged are info[0] * (m - 1)), info[1] the final
+// relative off-diagonal norm.
+__global__ void __launch_bounds__(kJT) jacobi_kernel(const double *Ain, int m, double *evals, double2 *rot,
+                                                     int max_sweeps, double *info)
 {
-    extern __shared__ double A[];        // m x (m + 1), then V with VL
+    extern __shared__ double A[];        // m x (m + 1)
     __shared__ double cs[64], sn[64];
     __shared__ int pp[64], qq[64];
-    __shared__ double red[kJT];
+    __shared__ double red[2][kJT / 64];
     const int tid = threadIdx.x;
     const int S = m + 1;                 // LDS row stride
-    double *V = VL ? A + m * S : Vout;
-    const int SV = VL ? S : m;           // V row stride
     for (int e = tid; e < m * m; e += kJT) {
         const int r = e / m, c = e - r * m;
         A[r * S + c] = 0.5 * (Ain[e] + Ain[c * m + r]);  // symmetrise
-        V[r * SV + c] = (r == c) ? 1.0 : 0.0;
     }
     __syncthreads();
     const int half = m / 2;
-    const int k0 = tid / m, c0 = tid - k0 * m;  // this thread's first (pair, row)
-    const int dk = kJT / m, dc = kJT - dk * m;  // and its step
     const int kb0 = tid / half, lb0 = tid - kb0 * half;  // first (pair, pair) block
     const int dkb = kJT / half, dlb = kJT - dkb * half;
     int sweep = 0;
@@ -53,31 +51,34 @@ __global__ void __launch_bounds__(kJT) jacobi_kernel(const double *Ain, int m, d
             st += v;
             if (r != c) so += v;
         }
-        red[tid] = so;
-        __syncthreads();
-        for (int s = kJT / 2; s; s >>= 1) {
-            if (tid < s) red[tid] += red[tid + s];
-            __syncthreads();
+        for (int o = 32; o; o >>= 1) {
+            so += __shfl_xor(so, o);
+            st += __shfl_xor(st, o);
         }
-        off = red[0];
-        __syncthreads();
-        red[tid] = st;
-        __syncthreads();
-        for (int s = kJT / 2; s; s >>= 1) {
-            if (tid < s) red[tid] += red[tid + s];
-            __syncthreads();
+        if ((tid & 63) == 0) {
+            red[0][tid >> 6] = so;
+            red[1][tid >> 6] = st;
         }
-        tot = red[0];
+        __syncthreads();
+        off = 0.0;
+        tot = 0.0;
+        for (int w = 0; w < kJT / 64; w++) {  // every thread sums in the same order
+            off += red[0][w];
+            tot += red[1][w];
+        }
         __syncthreads();
         // off-diagonal norm below 1e-12 of the whole: a Ritz pair's residual
         // then moves by at most about that fraction of ||H|| (the product of
         // rotations stays orthogonal to rounding whatever the threshold)
         if (off <= 1e-24 * tot || off == 0.0) break;
+        // pair k of round r: (0 or 1 + (r + k - 1) mod (m - 1), 1 + (r + m - 2 - k) mod (m - 1)),
+        // both residues stepped by one per round
+        int ia = tid - 1, ib = m - 2 - tid;
         for (int r = 0; r < m - 1; r++) {
             if (tid < half) {
                 const int k = tid;
-                int a = (k == 0) ? 0 : ((r + k - 1) % (m - 1)) + 1;
-                int b = ((r + (m - 1 - k) - 1) % (m - 1)) + 1;
+                int a = (k == 0) ? 0 : ia + 1;
+                int b = ib + 1;
                 if (a > b) { int t = a; a = b; b = t; }
                 pp[k] = a;
                 qq[k] = b;
@@ -91,6 +92,9 @@ __global__ void __launch_bounds__(kJT) jacobi_kernel(const double *Ain, int m, d
                 }
                 cs[k] = c;
                 sn[k] = s;
+                rot[((size_t)sweep * (m - 1) + r) * half + k] = make_double2(c, s);
+                ia = ia + 1 == m - 1 ? 0 : ia + 1;
+                ib = ib + 1 == m - 1 ? 0 : ib + 1;
             }
             __syncthreads();
             // A <- G A G^T, one 2x2 block (rows of pair k, columns of pair l)
@@ -112,28 +116,53 @@ __global__ void __launch_bounds__(kJT) jacobi_kernel(const double *Ain, int m, d
                 l += dlb;
                 if (l >= half) { l -= half; k++; }
             }
-            // V <- V G^T: element e = k*m + row of the half*m (pair, row) grid,
-            // stepped by kJT without divisions
-            for (int k = k0, row = c0; k < half;) {
-                const int p = pp[k], q = qq[k];
-                const double c = cs[k], s = sn[k];
-                const double vp = V[row * SV + p], vq = V[row * SV + q];
-                V[row * SV + p] = c * vp - s * vq;
-                V[row * SV + q] = s * vp + c * vq;
-                k += dk;
-                row += dc;
-                if (row >= m) { row -= m; k++; }
-            }
             __syncthreads();
         }
     }
     for (int i = tid; i < m; i += kJT) evals[i] = A[i * S + i];
-    if (VL)
-        for (int e = tid; e < m * m; e += kJT) Vout[e] = V[(e / m) * S + e % m];
-    if (tid == 0 && info) {
+    if (tid == 0) {
         info[0] = sweep;
         info[1] = sqrt(off / (tot > 0 ? tot : 1.0));
     }
+}
+
+// Row blockIdx.x of V = G_1^T G_2^T ... (the logged rotations in order, from
+// the identity): V[row][p] <- c V[row][p] - s V[row][q], V[row][q] <-
+// s V[row][p] + c V[row][q] for each pair (p, q) of each round, the
+// sequential method's eigenvector update.  One wave per row; the next
+// round's (c, s) is loaded while this one is applied.
+__global__ void __launch_bounds__(64) jacobi_vectors_kernel(const double2 *rot, const double *info, int m,
+                                                            double *V)
+{
+    __shared__ double row[128];
+    const int r = blockIdx.x, lane = threadIdx.x;
+    const int half = m / 2;
+    for (int c = lane; c < m; c += 64) row[c] = (c == r) ? 1.0 : 0.0;
+    __syncthreads();
+    const long rounds = (long)info[0] * (m - 1);
+    int ia = lane - 1, ib = m - 2 - lane, rr = 0;
+    double2 nxt = (lane < half && rounds > 0) ? rot[lane] : make_double2(1.0, 0.0);
+    for (long t = 0; t < rounds; t++) {
+        const double2 g = nxt;
+        if (lane < half && t + 1 < rounds) nxt = rot[(t + 1) * half + lane];
+        if (lane < half) {
+            int a = (lane == 0) ? 0 : ia + 1;
+            int b = ib + 1;
+            if (a > b) { int x = a; a = b; b = x; }
+            const double vp = row[a], vq = row[b];
+            row[a] = g.x * vp - g.y * vq;
+            row[b] = g.y * vp + g.x * vq;
+            ia = ia + 1 == m - 1 ? 0 : ia + 1;
+            ib = ib + 1 == m - 1 ? 0 : ib + 1;
+        }
+        if (++rr == m - 1) {  // a new sweep starts the pairing again
+            rr = 0;
+            ia = lane - 1;
+            ib = m - 2 - lane;
+        }
+        __syncthreads();
+    }
+    for (int c = lane; c < m; c += 64) V[(size_t)r * m + c] = row[c];
 }
 
 // G = R^T R (R upper), Rinv = R^-1; info[0] = 0 on success, else 1-based
@@ -304,23 +333,27 @@ fmx_status rand_uniform(fmx_ctx *ctx, fmx_dtype dt, void *X, int64_t ldx, uint64
     return FMX_OK;
 }
 
+size_t small_sym_eig_ws(int m)
+{
+    return (size_t)kJacobiSweeps * std::max(m - 1, 1) * std::max(m / 2, 1) * sizeof(double2);
+}
+
 fmx_status small_sym_eig(fmx_ctx *ctx, const double *H, int m, double *evals, double *V, double *info,
-                         hipStream_t s)
+                         void *ws, hipStream_t s)
 {
     if (m <= 0 || m > 128 || (m & 1)) return fail(ctx, FMX_EDIM, "Rayleigh-Ritz size must be even and <= 128");
+    if (!info || !ws) return fail(ctx, FMX_EINVAL, "Rayleigh-Ritz needs its info and rotation buffers");
     static bool attr = false;
     if (!attr) {
-        FMX_HIP(ctx, hipFuncSetAttribute((const void *)jacobi_kernel<true>,
-                                         hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
-        FMX_HIP(ctx, hipFuncSetAttribute((const void *)jacobi_kernel<false>,
-                                         hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
+        FMX_HIP(ctx, hipFuncSetAttribute((const void *)jacobi_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         150 * 1024));
         attr = true;
     }
-    const size_t mm = (size_t)m * (m + 1) * sizeof(double);  // padded rows
-    if (2 * mm <= 150 * 1024)
-        hipLaunchKernelGGL(jacobi_kernel<true>, dim3(1), dim3(kJT), 2 * mm, s, H, m, evals, V, 40, info);
-    else
-        hipLaunchKernelGGL(jacobi_kernel<false>, dim3(1), dim3(kJT), mm, s, H, m, evals, V, 40, info);
+    const size_t mm = (size_t)m * (m + 1) * sizeof(double);  // padded rows (at most 132,096 bytes)
+    hipLaunchKernelGGL(jacobi_kernel, dim3(1), dim3(kJT), mm, s, H, m, evals, (double2 *)ws, kJacobiSweeps, info);
+    FMX_HIP(ctx, hipGetLastError());
+    hipLaunchKernelGGL(jacobi_vectors_kernel, dim3(m), dim3(64), 0, s, (const double2 *)ws, (const double *)info,
+                       m, V);
     FMX_HIP(ctx, hipGetLastError());
     return FMX_OK;
 }
