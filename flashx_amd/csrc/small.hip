@@ -38,8 +38,15 @@ __global__ void __launch_bounds__(kJT) jacobi_kernel(const double *Ain, int m, d
     }
     __syncthreads();
     const int half = m / 2;
-    const int kb0 = tid / half, lb0 = tid - kb0 * half;  // first (pair, pair) block
-    const int dkb = kJT / half, dlb = kJT - dkb * half;
+    // this thread's (pair, pair) blocks, the same every round: e = tid + j *
+    // kJT of the half x half grid (at most 4 with m <= 128)
+    int bk[4], bl[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const int e = tid + j * kJT;
+        bk[j] = e < half * half ? e / half : -1;
+        bl[j] = e < half * half ? e - bk[j] * half : 0;
+    }
     int sweep = 0;
     double off = 0.0, tot = 0.0;
     for (; sweep < max_sweeps; sweep++) {
@@ -101,20 +108,34 @@ __global__ void __launch_bounds__(kJT) jacobi_kernel(const double *Ain, int m, d
             // per element of the half x half grid: the blocks are disjoint,
             // so the row and the column rotation of a block need no barrier
             // between them (the same operations as rotating all rows, then
-            // all columns)
-            for (int k = kb0, l = lb0; k < half;) {
-                const int p = pp[k], q = qq[k], u = pp[l], v = qq[l];
-                const double ck = cs[k], sk = sn[k], cl = cs[l], sl = sn[l];
-                const double a = A[p * S + u], b = A[p * S + v], c = A[q * S + u], d = A[q * S + v];
+            // all columns), and every load of the round may precede every
+            // store
+            int o00[4], o01[4], o10[4], o11[4];
+            double x00[4], x01[4], x10[4], x11[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                if (bk[j] < 0) continue;
+                const int p = pp[bk[j]], q = qq[bk[j]], u = pp[bl[j]], v = qq[bl[j]];
+                o00[j] = p * S + u;
+                o01[j] = p * S + v;
+                o10[j] = q * S + u;
+                o11[j] = q * S + v;
+                x00[j] = A[o00[j]];
+                x01[j] = A[o01[j]];
+                x10[j] = A[o10[j]];
+                x11[j] = A[o11[j]];
+            }
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                if (bk[j] < 0) continue;
+                const double ck = cs[bk[j]], sk = sn[bk[j]], cl = cs[bl[j]], sl = sn[bl[j]];
+                const double a = x00[j], b = x01[j], c = x10[j], d = x11[j];
                 const double a1 = ck * a - sk * c, b1 = ck * b - sk * d;
                 const double c1 = sk * a + ck * c, d1 = sk * b + ck * d;
-                A[p * S + u] = cl * a1 - sl * b1;
-                A[p * S + v] = sl * a1 + cl * b1;
-                A[q * S + u] = cl * c1 - sl * d1;
-                A[q * S + v] = sl * c1 + cl * d1;
-                k += dkb;
-                l += dlb;
-                if (l >= half) { l -= half; k++; }
+                A[o00[j]] = cl * a1 - sl * b1;
+                A[o01[j]] = sl * a1 + cl * b1;
+                A[o10[j]] = cl * c1 - sl * d1;
+                A[o11[j]] = sl * c1 + cl * d1;
             }
             __syncthreads();
         }
