@@ -50,6 +50,10 @@ template <typename T> constexpr fmx_dtype dt_of();
 template <> constexpr fmx_dtype dt_of<float>() { return FMX_F32; }
 template <> constexpr fmx_dtype dt_of<double>() { return FMX_F64; }
 
+// Cholesky-QR of an FP32 block stops after one pass when ||R||_F ||R^-1||_F
+// is at most this
+constexpr double kOnePassCond = 1e3;
+
 #define TRY(x)                             \
     do {                                   \
         fmx_status st_ = (x);              \
@@ -230,16 +234,23 @@ fmx_status Solver<T>::cholqr(T *W, int64_t ldw, T *dst, int64_t ldd, bool &ok, u
         const int64_t lds = ldw;
         TRY(gram(ctx_, dt_of<T>(), FMX_GRAM_F64, n_, b_, b_, src, lds, src, lds, G_.p, s_));
         TRY(small_chol_inv(ctx_, G_.p, b_, Rinv_.p, info_.p, s_));
-        FMX_HIP(ctx_, hipMemcpyAsync(info, info_.p, sizeof(double), hipMemcpyDeviceToHost, s_));
+        FMX_HIP(ctx_, hipMemcpyAsync(info, info_.p, 2 * sizeof(double), hipMemcpyDeviceToHost, s_));
         TRY(sync());
         if (info[0] != 0.0) { ok = false; return FMX_OK; }
-        if (pass == 1 && (dstb || scaled))
+        // One pass suffices for a well-conditioned FP32 block: with the Gram
+        // summed in FP64, W R^-1 is orthonormal to about kappa^2 * 1e-16,
+        // below FP32 rounding while kappa <= info[1] <= 1e3 (the second pass
+        // only repairs what the first lost to conditioning).  An FP64 basis
+        // always takes both.
+        const bool last = pass == 1 || (sizeof(T) == 4 && info[1] <= kOnePassCond);
+        if (last && (dstb || scaled))
             TRY(gemm_tall_copies(ctx_, dt_of<T>(), n_, b_, b_, 1.0, src, lds, Rinv_.p, 0.0, dst, ldd, dstb, m_,
                                  scaled ? dinv_.p : nullptr, scaled ? T1_.p : nullptr, b_, s_));
-        else if (pass == 1)
+        else if (last)
             TRY(gemm_tall(ctx_, dt_of<T>(), n_, b_, b_, 1.0, src, lds, Rinv_.p, 0.0, dst, ldd, s_));
         else
             TRY(gemm_tall(ctx_, dt_of<T>(), n_, b_, b_, 1.0, W, ldw, Rinv_.p, 0.0, W, ldw, s_));
+        if (last) break;
     }
     return FMX_OK;
 }

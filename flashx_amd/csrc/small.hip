@@ -187,7 +187,9 @@ __global__ void __launch_bounds__(64) jacobi_vectors_kernel(const double2 *rot, 
 }
 
 // G = R^T R (R upper), Rinv = R^-1; info[0] = 0 on success, else 1-based
-// index of the first non-positive pivot (relative threshold eps).
+// index of the first non-positive pivot (relative threshold eps); info[1] =
+// ||R||_F ||R^-1||_F = sqrt(trace G) ||R^-1||_F, an upper bound on the
+// 2-norm condition number of R (and of any W with W^T W = G).
 __global__ void __launch_bounds__(64) chol_inv_kernel(const double *G, int b, double *Rinv, double eps,
                                                       double *info)
 {
@@ -201,8 +203,11 @@ __global__ void __launch_bounds__(64) chol_inv_kernel(const double *G, int b, do
         I[e] = 0.0;
     }
     __syncthreads();
-    double dmax = 0.0;
-    for (int i = 0; i < b; i++) dmax = fmax(dmax, R[i * b + i]);
+    double dmax = 0.0, tr = 0.0;
+    for (int i = 0; i < b; i++) {
+        dmax = fmax(dmax, R[i * b + i]);
+        tr += R[i * b + i];
+    }
     // right-looking Cholesky on the upper triangle
     for (int k = 0; k < b; k++) {
         if (tid == 0) {
@@ -229,11 +234,18 @@ __global__ void __launch_bounds__(64) chol_inv_kernel(const double *G, int b, do
         }
     }
     __syncthreads();
+    double f2 = 0.0;
     for (int e = tid; e < b * b; e += 64) {
         const int i = e / b, j = e - i * b;
-        Rinv[e] = (j >= i) ? I[e] : 0.0;
+        const double v = (j >= i) ? I[e] : 0.0;
+        Rinv[e] = v;
+        f2 += v * v;
     }
-    if (tid == 0) info[0] = bad;
+    for (int o = 32; o; o >>= 1) f2 += __shfl_xor(f2, o);
+    if (tid == 0) {
+        info[0] = bad;
+        info[1] = bad ? INFINITY : sqrt(tr * f2);
+    }
 }
 
 // H[0:rows, c0:c0+b] = C1 + C2 and the mirrored block, H is ld x ld.  In
