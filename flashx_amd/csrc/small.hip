@@ -12,21 +12,21 @@ namespace {
 
 constexpr int kJT = 1024;  // threads of the Jacobi workgroup (16 waves)
 
-// Cyclic Jacobi for a symmetric m x m matrix (m even, m <= 128), A in LDS.
-// Each round rotates m/2 disjoint pairs (round-robin ordering: index 0 stays,
-// the others move one place per round), rows first then columns, so
-// A' = P^T A P exactly as the sequential method would.  LDS rows are m + 1
-// doubles apart, so the column updates (one row per thread) spread over the
-// banks.  Round t's rotations go to rot[t * m/2 + pair] as (c, s): the
-// eigenvector matrix is their product, which jacobi_vectors_kernel forms
-// row by row on other CUs afterwards (rows of V are independent), so this
-// single workgroup, the serial part, carries only A.  info[0] = sweeps that
-// rotated (the rounds logged are info[0] * (m - 1)), info[1] the final
-// relative off-diagonal norm.
+// Cyclic Jacobi for a symmetric m x m matrix (m even, m <= 128), the upper
+// triangle of A in LDS (entry (i, j), i <= j, at i * (m + 1) + j).  Each
+// round rotates m/2 disjoint pairs (round-robin ordering: index 0 stays, the
+// others move one place per round), rows first then columns, so
+// A' = P^T A P exactly as the sequential method would.  Round t's rotations
+// go to rot[t * m/2 + pair] as (c, s): the eigenvector matrix is their
+// product, which jacobi_vectors_kernel forms row by row on other CUs
+// afterwards (rows of V are independent), so this single workgroup, the
+// serial part, carries only A.  info[0] = sweeps that rotated (the rounds
+// logged are info[0] * (m - 1)), info[1] the final relative off-diagonal
+// norm.
 __global__ void __launch_bounds__(kJT) jacobi_kernel(const double *Ain, int m, double *evals, double2 *rot,
                                                      int max_sweeps, double *info)
 {
-    extern __shared__ double A[];        // m x (m + 1)
+    extern __shared__ double A[];        // m x (m + 1), upper triangle used
     __shared__ double cs[64], sn[64];
     __shared__ int pp[64], qq[64];
     __shared__ double red[2][kJT / 64];
@@ -34,18 +34,23 @@ __global__ void __launch_bounds__(kJT) jacobi_kernel(const double *Ain, int m, d
     const int S = m + 1;                 // LDS row stride
     for (int e = tid; e < m * m; e += kJT) {
         const int r = e / m, c = e - r * m;
-        A[r * S + c] = 0.5 * (Ain[e] + Ain[c * m + r]);  // symmetrise
+        if (c >= r) A[r * S + c] = 0.5 * (Ain[e] + Ain[c * m + r]);  // symmetrise
     }
     __syncthreads();
     const int half = m / 2;
-    // this thread's (pair, pair) blocks, the same every round: e = tid + j *
-    // kJT of the half x half grid (at most 4 with m <= 128)
-    int bk[4], bl[4];
+    // this thread's (pair k, pair l) blocks, k <= l, the same every round:
+    // element tid + j * kJT of the half (half + 1) / 2 triangle (at most 3
+    // with m <= 128); block (l, k) holds the mirrored entries of (k, l)
+    int bk[3], bl[3];
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
-        const int e = tid + j * kJT;
-        bk[j] = e < half * half ? e / half : -1;
-        bl[j] = e < half * half ? e - bk[j] * half : 0;
+    for (int j = 0; j < 3; j++) {
+        int e = tid + j * kJT, k = 0;
+        while (k < half && e >= half - k) {
+            e -= half - k;
+            k++;
+        }
+        bk[j] = k < half ? k : -1;
+        bl[j] = k + e;
     }
     int sweep = 0;
     double off = 0.0, tot = 0.0;
@@ -54,9 +59,10 @@ __global__ void __launch_bounds__(kJT) jacobi_kernel(const double *Ain, int m, d
         double so = 0.0, st = 0.0;
         for (int e = tid; e < m * m; e += kJT) {
             const int r = e / m, c = e - r * m;
+            if (c < r) continue;
             const double v = A[r * S + c] * A[r * S + c];
-            st += v;
-            if (r != c) so += v;
+            st += r == c ? v : 2.0 * v;
+            if (r != c) so += 2.0 * v;
         }
         for (int o = 32; o; o >>= 1) {
             so += __shfl_xor(so, o);
@@ -105,28 +111,29 @@ __global__ void __launch_bounds__(kJT) jacobi_kernel(const double *Ain, int m, d
             }
             __syncthreads();
             // A <- G A G^T, one 2x2 block (rows of pair k, columns of pair l)
-            // per element of the half x half grid: the blocks are disjoint,
-            // so the row and the column rotation of a block need no barrier
-            // between them (the same operations as rotating all rows, then
-            // all columns), and every load of the round may precede every
-            // store
-            int o00[4], o01[4], o10[4], o11[4];
-            double x00[4], x01[4], x10[4], x11[4];
+            // per element of the triangle: the blocks are disjoint, so the
+            // row and the column rotation of a block need no barrier between
+            // them (the same operations as rotating all rows, then all
+            // columns), and every load of the round may precede every store.
+            // In a diagonal block (k = l) entries (p, q) and (q, p) share one
+            // slot, which takes the (q, p) result.
+            int o00[3], o01[3], o10[3], o11[3];
+            double x00[3], x01[3], x10[3], x11[3];
 #pragma unroll
-            for (int j = 0; j < 4; j++) {
+            for (int j = 0; j < 3; j++) {
                 if (bk[j] < 0) continue;
                 const int p = pp[bk[j]], q = qq[bk[j]], u = pp[bl[j]], v = qq[bl[j]];
-                o00[j] = p * S + u;
-                o01[j] = p * S + v;
-                o10[j] = q * S + u;
-                o11[j] = q * S + v;
+                o00[j] = p < u ? p * S + u : u * S + p;
+                o01[j] = p < v ? p * S + v : v * S + p;
+                o10[j] = q < u ? q * S + u : u * S + q;
+                o11[j] = q < v ? q * S + v : v * S + q;
                 x00[j] = A[o00[j]];
                 x01[j] = A[o01[j]];
                 x10[j] = A[o10[j]];
                 x11[j] = A[o11[j]];
             }
 #pragma unroll
-            for (int j = 0; j < 4; j++) {
+            for (int j = 0; j < 3; j++) {
                 if (bk[j] < 0) continue;
                 const double ck = cs[bk[j]], sk = sn[bk[j]], cl = cs[bl[j]], sl = sn[bl[j]];
                 const double a = x00[j], b = x01[j], c = x10[j], d = x11[j];
