@@ -176,6 +176,24 @@ struct Stage {
             reg[p] = v;
         }
     }
+    // Every piece loaded with no branch, so that the count of loads in
+    // flight is the same on every path (a wait for a later load then need
+    // not wait for these): a piece past row r1 holds row r's, a piece past
+    // the width the row's first.  Only for a product whose rows past r1 are
+    // not stored and whose factor is zero past the width (the finite values
+    // there then add nothing).  M 16-byte aligned, ld and k multiples of V,
+    // r < r1.
+    __device__ __forceinline__ void load_all(const T *M, int64_t ld, int k, uint64_t r, uint64_t r1, int lane)
+    {
+#pragma unroll
+        for (int p = 0; p < NP; p++) {
+            const int e = (p * 64 + lane) * V, row = e / KP, col = e - row * KP;
+            const uint64_t gr = r + row;
+            const Vec16<T> t = *(const Vec16<T> *)(M + (gr < r1 ? gr : r) * ld + (col < k ? col : 0));
+#pragma unroll
+            for (int u = 0; u < V; u++) reg[p].v[u] = t.v[u];  // (a whole-struct copy stays in scratch)
+        }
+    }
     // into LDS rows of stride LS elements (16-byte stores when the rows are
     // 16-byte aligned; the caller's slice is)
     template <int LS>
@@ -320,7 +338,7 @@ __global__ void __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(3))) gr
 // Y = alpha A C + beta Y for kc <= 16*TJ, ka <= KAP: per 16 staged rows of A,
 // ka/4 MFMA steps against C held in registers (lane l: C[4s + l/16][l%16]).
 // A row is staged whole before any output of it is written, so Y may be A.
-template <typename T, typename TY, int KAP, int TJ>
+template <typename T, typename TY, int KAP, int TJ, bool VEC, int SC = -1, int OLD = -1>
 __device__ __forceinline__ void gemm_stage_body(const T *A, int64_t lda, uint64_t n, int ka, int kc,
                                                         const double *C, double alpha, double beta, TY *Y,
                                                         int64_t ldy, uint64_t rows_per_wave, uint64_t nwaves,
@@ -349,21 +367,51 @@ __device__ __forceinline__ void gemm_stage_body(const T *A, int64_t lda, uint64_
     const uint64_t r0 = wave * rows_per_wave;
     const uint64_t r1 = min(n, r0 + rows_per_wave);
     Stage<T, KAP> sa;
-    if (r0 < r1) sa.load(A, lda, ka, r0, r1, lane, veca);
+    if (r0 < r1) {
+        if constexpr (VEC) sa.load_all(A, lda, ka, r0, r1, lane);
+        else sa.load(A, lda, ka, r0, r1, lane, veca);
+    }
     for (uint64_t r = r0; r < r1; r += kStageRows) {
         sa.template store<LA>(As, lane);
         wave_lds_sync();
-        if (r + kStageRows < r1) sa.load(A, lda, ka, r + kStageRows, r1, lane, veca);
-        // this stage's old Y values, requested before the MFMAs that hide them
-        double prev[TJ][4];
+        // This stage's old Y values (and row scales), loaded from in-bounds
+        // addresses whatever the rows and beta, and requested before the
+        // next stage's rows: the epilogue's wait for them then leaves those
+        // rows in flight through the MFMAs.
+        // (SC: whether the row-scaled copy Ys is written, 1 or 0 when known
+        // at compile time, else -1; a run-time branch here would wait for
+        // these loads at once)
+        const bool sc = SC < 0 ? Ys != nullptr : SC > 0;
+        double dsc[4] = {0.0, 0.0, 0.0, 0.0};
+        if (sc) {
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const uint64_t gr = r + lk + 4 * q;
+                dsc[q] = ds[gr < r1 ? gr : r];
+            }
+        }
+        // (OLD: whether Y's old values are read, that is beta != 0, known
+        // at compile time as SC is, or -1)
+        const bool old = OLD < 0 ? beta != 0.0 : OLD > 0;
+        TY yold[TJ][4];
 #pragma unroll
         for (int tj = 0; tj < TJ; tj++) {
             const int j = tj * 16 + li;
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 const uint64_t gr = r + lk + 4 * q;
-                prev[tj][q] = (beta != 0.0 && gr < r1 && j < kc) ? beta * (double)Y[gr * ldy + j] : 0.0;
+                yold[tj][q] = old ? Y[(gr < r1 ? gr : r) * ldy + (j < kc ? j : 0)] : TY(0);
             }
+        }
+        if constexpr (VEC) {
+            // the last stage reloads its own rows (unused), so that every
+            // stage has the same loads in flight; the scheduling barriers
+            // keep the loads in this order, ahead of the MFMAs
+            __builtin_amdgcn_sched_barrier(0);
+            sa.load_all(A, lda, ka, r + kStageRows < r1 ? r + kStageRows : r, r1, lane);
+            __builtin_amdgcn_sched_barrier(0);
+        } else {
+            if (r + kStageRows < r1) sa.load(A, lda, ka, r + kStageRows, r1, lane, veca);
         }
         doublex4 acc[TJ];
 #pragma unroll
@@ -374,6 +422,7 @@ __device__ __forceinline__ void gemm_stage_body(const T *A, int64_t lda, uint64_
 #pragma unroll
             for (int tj = 0; tj < TJ; tj++) acc[tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, c[tj][s], acc[tj], 0, 0, 0);
         }
+        if constexpr (VEC) __builtin_amdgcn_sched_barrier(0);  // the old values' uses stay after the MFMAs
         wave_lds_sync();
 #pragma unroll
         for (int tj = 0; tj < TJ; tj++) {
@@ -382,10 +431,11 @@ __device__ __forceinline__ void gemm_stage_body(const T *A, int64_t lda, uint64_
             for (int q = 0; q < 4; q++) {
                 const uint64_t gr = r + lk + 4 * q;
                 if (gr < r1 && j < kc) {
-                    const TY y = (TY)(prev[tj][q] + alpha * acc[tj][q]);
+                    const double prev = old ? beta * (double)yold[tj][q] : 0.0;
+                    const TY y = (TY)(prev + alpha * acc[tj][q]);
                     Y[gr * ldy + j] = y;
                     if (Yb) Yb[gr * ldyb + j] = f2bf((float)y);  // the BF16 copy of what was stored
-                    if (Ys) Ys[gr * ldys + j] = (TY)((double)y * ds[gr]);  // and its row-scaled copy
+                    if (sc) Ys[gr * ldys + j] = (TY)((double)y * dsc[q]);  // and its row-scaled copy
                 }
             }
         }
@@ -404,16 +454,16 @@ __global__ void __launch_bounds__(kT) gemm_stage_kernel(const T *A, int64_t lda,
                                                         int veca, unsigned short *Yb, int64_t ldyb,
                                                         const double *ds, TY *Ys, int64_t ldys)
 {
-    gemm_stage_body<T, TY, KAP, TJ>(A, lda, n, ka, kc, C, alpha, beta, Y, ldy, rows_per_wave, nwaves, veca, Yb, ldyb, ds, Ys, ldys);
+    gemm_stage_body<T, TY, KAP, TJ, false>(A, lda, n, ka, kc, C, alpha, beta, Y, ldy, rows_per_wave, nwaves, veca, Yb, ldyb, ds, Ys, ldys);
 }
-template <typename T, typename TY, int KAP, int TJ>
+template <typename T, typename TY, int KAP, int TJ, bool VEC, int SC, int OLD>
 __global__ void __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(3))) gemm_stage_kernel3(const T *A, int64_t lda, uint64_t n, int ka, int kc,
                                                         const double *C, double alpha, double beta, TY *Y,
                                                         int64_t ldy, uint64_t rows_per_wave, uint64_t nwaves,
                                                         int veca, unsigned short *Yb, int64_t ldyb,
                                                         const double *ds, TY *Ys, int64_t ldys)
 {
-    gemm_stage_body<T, TY, KAP, TJ>(A, lda, n, ka, kc, C, alpha, beta, Y, ldy, rows_per_wave, nwaves, veca, Yb, ldyb, ds, Ys, ldys);
+    gemm_stage_body<T, TY, KAP, TJ, VEC, SC, OLD>(A, lda, n, ka, kc, C, alpha, beta, Y, ldy, rows_per_wave, nwaves, veca, Yb, ldyb, ds, Ys, ldys);
 }
 
 // out[o] = sum over p of partial[p][o], p in order 0..np-1 within four
@@ -1035,10 +1085,21 @@ fmx_status gemm_staged(fmx_ctx *ctx, uint64_t n, int ka, int kc, double alpha, c
         for (const void *k : ks)
             FMX_HIP(ctx, hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024));
         if constexpr (sizeof(T) == 4) {
-            const void *k3[] = {(const void *)gemm_stage_kernel3<T, TY, 16, 1>,
-                                (const void *)gemm_stage_kernel3<T, TY, 32, 1>,
-                                (const void *)gemm_stage_kernel3<T, TY, 64, 1>,
-                                (const void *)gemm_stage_kernel3<T, TY, 96, 1>};
+            const void *k3[] = {(const void *)gemm_stage_kernel3<T, TY, 16, 1, false, -1, -1>,
+                                (const void *)gemm_stage_kernel3<T, TY, 32, 1, false, -1, -1>,
+                                (const void *)gemm_stage_kernel3<T, TY, 64, 1, false, -1, -1>,
+                                (const void *)gemm_stage_kernel3<T, TY, 16, 1, true, 0, 1>,
+                                (const void *)gemm_stage_kernel3<T, TY, 32, 1, true, 0, 1>,
+                                (const void *)gemm_stage_kernel3<T, TY, 64, 1, true, 0, 1>,
+                                (const void *)gemm_stage_kernel3<T, TY, 96, 1, true, 0, 1>,
+                                (const void *)gemm_stage_kernel3<T, TY, 16, 1, true, 0, 0>,
+                                (const void *)gemm_stage_kernel3<T, TY, 32, 1, true, 0, 0>,
+                                (const void *)gemm_stage_kernel3<T, TY, 64, 1, true, 0, 0>,
+                                (const void *)gemm_stage_kernel3<T, TY, 96, 1, true, 0, 0>,
+                                (const void *)gemm_stage_kernel3<T, TY, 16, 1, true, 1, 0>,
+                                (const void *)gemm_stage_kernel3<T, TY, 32, 1, true, 1, 0>,
+                                (const void *)gemm_stage_kernel3<T, TY, 64, 1, true, 1, 0>,
+                                (const void *)gemm_stage_kernel3<T, TY, 96, 1, true, 1, 0>};
             for (const void *k : k3)
                 FMX_HIP(ctx, hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024));
         }
@@ -1047,15 +1108,40 @@ fmx_status gemm_staged(fmx_ctx *ctx, uint64_t n, int ka, int kc, double alpha, c
 #define FMX_GEMMS(K, P, J)                                                                                    \
     hipLaunchKernelGGL((K<T, TY, P, J>), dim3(nblk), dim3(kT), lds, s, A, lda, n, ka, kc, C, alpha, beta, Y, ldy, \
                        rpw, waves, veca, Yb, ldyb, ds, Ys, ldys)
-    // FP32 rows and one column tile: the three-waves-per-SIMD form up to 96 columns
+#define FMX_GEMMS3(P, VEC, SC, OLD)                                                                           \
+    hipLaunchKernelGGL((gemm_stage_kernel3<T, TY, P, 1, VEC, SC, OLD>), dim3(nblk), dim3(kT), lds, s, A, lda, n, \
+                       ka, kc, C, alpha, beta, Y, ldy, rpw, waves, veca, Yb, ldyb, ds, Ys, ldys)
+#define FMX_GEMMS3W(SC, OLD)                                                                                  \
+    do {                                                                                                      \
+        if (kap == 16) FMX_GEMMS3(16, true, SC, OLD);                                                         \
+        else if (kap == 32) FMX_GEMMS3(32, true, SC, OLD);                                                    \
+        else if (kap == 64) FMX_GEMMS3(64, true, SC, OLD);                                                    \
+        else FMX_GEMMS3(96, true, SC, OLD);                                                                   \
+    } while (0)
+    // FP32 rows and one column tile: the three-waves-per-SIMD form up to 96
+    // columns, with whole-row vector loads when the rows allow them
     bool done = false;
     if constexpr (sizeof(T) == 4) {
         if (kc <= 16 && kap <= 96) {
-            if (kap == 16) FMX_GEMMS(gemm_stage_kernel3, 16, 1);
-            else if (kap == 32) FMX_GEMMS(gemm_stage_kernel3, 32, 1);
-            else if (kap == 64) FMX_GEMMS(gemm_stage_kernel3, 64, 1);
-            else FMX_GEMMS(gemm_stage_kernel3, 96, 1);
-            done = true;
+            // whole-row vector loads, with the row-scaled copy and the old
+            // values' loads fixed at compile time: Y = A C + beta Y, Y = A C,
+            // and Y = A C with its scaled copy
+            const bool vec = veca && ka % 4 == 0;
+            if (vec && !Ys && beta != 0.0) {
+                FMX_GEMMS3W(0, 1);
+                done = true;
+            } else if (vec && !Ys) {
+                FMX_GEMMS3W(0, 0);
+                done = true;
+            } else if (vec && beta == 0.0) {
+                FMX_GEMMS3W(1, 0);
+                done = true;
+            } else if (kap <= 64) {  // (the 96-column form of these loads would spill)
+                if (kap == 16) FMX_GEMMS3(16, false, -1, -1);
+                else if (kap == 32) FMX_GEMMS3(32, false, -1, -1);
+                else FMX_GEMMS3(64, false, -1, -1);
+                done = true;
+            }
         }
     }
     if (!done) {
@@ -1072,6 +1158,8 @@ fmx_status gemm_staged(fmx_ctx *ctx, uint64_t n, int ka, int kc, double alpha, c
 #undef FMX_GEMMS_P
     }
 #undef FMX_GEMMS
+#undef FMX_GEMMS3
+#undef FMX_GEMMS3W
     FMX_HIP(ctx, hipGetLastError());
     return FMX_OK;
 }

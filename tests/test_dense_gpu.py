@@ -95,7 +95,11 @@ def test_gram_f64_strided(fmx, orc):
 
 
 @pytest.mark.parametrize("n,ka,kc", [(100003, 128, 16), (3000, 16, 48), (10, 3, 5), (1000003, 96, 16),
-                                     (20011, 7, 40), (5003, 96, 80)])
+                                     (20011, 7, 40), (5003, 96, 80),
+                                     # FP32 whole-row vector loads at each staged width, a width
+                                     # short of its 16-column tile, and fewer rows than a stage
+                                     (65537, 16, 16), (4099, 32, 12), (30001, 64, 16), (777, 60, 16),
+                                     (13, 16, 16)])
 def test_gemm_tall(fmx, orc, n, ka, kc):
     rng = np.random.default_rng(ka * kc)
     C = rng.standard_normal((ka, kc))
