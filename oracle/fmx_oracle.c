@@ -1422,7 +1422,7 @@ int orc_pagerank(const uint8_t *a_img, uint64_t a_bytes, const int64_t *a_offs, 
                  uint64_t n, int max_iters, float damping, float *pr, int *iters)
 {
     float *deg = (float *)malloc(n * sizeof(float));
-    float *ones = (float *)malloc(n * sizeof(float));
+    float *ones = (float *)calloc(n ? n : 1, sizeof(float));
     float *in = (float *)malloc(n * sizeof(float));
     float *out = (float *)malloc(n * sizeof(float));
     float *pr1 = (float *)malloc(n * sizeof(float));
