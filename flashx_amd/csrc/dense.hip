@@ -447,14 +447,14 @@ __device__ __forceinline__ void gemm_stage_body(const T *A, int64_t lda, uint64_
 // for FP32 rows with one column tile and KAP <= 96, where the body fits that
 // budget without spilling (the 96-column product 8 percent faster, the
 // 64-column 11 percent); wider variants would spill.
-template <typename T, typename TY, int KAP, int TJ>
+template <typename T, typename TY, int KAP, int TJ, bool VEC = false, int SC = -1, int OLD = -1>
 __global__ void __launch_bounds__(kT) gemm_stage_kernel(const T *A, int64_t lda, uint64_t n, int ka, int kc,
                                                         const double *C, double alpha, double beta, TY *Y,
                                                         int64_t ldy, uint64_t rows_per_wave, uint64_t nwaves,
                                                         int veca, unsigned short *Yb, int64_t ldyb,
                                                         const double *ds, TY *Ys, int64_t ldys)
 {
-    gemm_stage_body<T, TY, KAP, TJ, false>(A, lda, n, ka, kc, C, alpha, beta, Y, ldy, rows_per_wave, nwaves, veca, Yb, ldyb, ds, Ys, ldys);
+    gemm_stage_body<T, TY, KAP, TJ, VEC, SC, OLD>(A, lda, n, ka, kc, C, alpha, beta, Y, ldy, rows_per_wave, nwaves, veca, Yb, ldyb, ds, Ys, ldys);
 }
 template <typename T, typename TY, int KAP, int TJ, bool VEC, int SC, int OLD>
 __global__ void __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(3))) gemm_stage_kernel3(const T *A, int64_t lda, uint64_t n, int ka, int kc,
@@ -1102,6 +1102,13 @@ fmx_status gemm_staged(fmx_ctx *ctx, uint64_t n, int ka, int kc, double alpha, c
                                 (const void *)gemm_stage_kernel3<T, TY, 96, 1, true, 1, 0>};
             for (const void *k : k3)
                 FMX_HIP(ctx, hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024));
+            const void *k2v[] = {(const void *)gemm_stage_kernel<T, TY, 16, 2, true, 0, 0>,
+                                 (const void *)gemm_stage_kernel<T, TY, 32, 2, true, 0, 0>,
+                                 (const void *)gemm_stage_kernel<T, TY, 64, 2, true, 0, 0>,
+                                 (const void *)gemm_stage_kernel<T, TY, 96, 2, true, 0, 0>,
+                                 (const void *)gemm_stage_kernel<T, TY, 128, 2, true, 0, 0>};
+            for (const void *k : k2v)
+                FMX_HIP(ctx, hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024));
         }
         attr = true;
     }
@@ -1142,6 +1149,22 @@ fmx_status gemm_staged(fmx_ctx *ctx, uint64_t n, int ka, int kc, double alpha, c
                 else FMX_GEMMS3(64, false, -1, -1);
                 done = true;
             }
+        }
+    }
+    // FP32 rows, two column tiles, beta = 0, no scaled copy (the thick
+    // restart's V = V S): whole-row vector loads as above
+    if constexpr (sizeof(T) == 4) {
+        if (!done && kc > 16 && veca && ka % 4 == 0 && beta == 0.0 && !Ys) {
+#define FMX_GEMMS2V(P)                                                                                        \
+    hipLaunchKernelGGL((gemm_stage_kernel<T, TY, P, 2, true, 0, 0>), dim3(nblk), dim3(kT), lds, s, A, lda, n, ka, \
+                       kc, C, alpha, beta, Y, ldy, rpw, waves, veca, Yb, ldyb, ds, Ys, ldys)
+            if (kap == 16) FMX_GEMMS2V(16);
+            else if (kap == 32) FMX_GEMMS2V(32);
+            else if (kap == 64) FMX_GEMMS2V(64);
+            else if (kap == 96) FMX_GEMMS2V(96);
+            else FMX_GEMMS2V(128);
+#undef FMX_GEMMS2V
+            done = true;
         }
     }
     if (!done) {

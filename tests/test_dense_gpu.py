@@ -113,6 +113,21 @@ def test_gemm_tall(fmx, orc, n, ka, kc):
         assert np.all(np.abs(Y - ref) <= tol * scale)
 
 
+@pytest.mark.parametrize("n,ka,kc", [(70001, 96, 32), (4099, 64, 20), (33, 16, 24), (20011, 128, 32)])
+def test_gemm_tall_new_output(fmx, orc, n, ka, kc):
+    """Y = A C into a fresh output (beta = 0, no old values read), two
+    16-column output tiles, against the oracle's FP64 product."""
+    rng = np.random.default_rng(n + ka * kc)
+    C = rng.standard_normal((ka, kc))
+    for dt in (np.float32, np.float64):
+        A = rng.standard_normal((n, ka)).astype(dt)
+        Y = fmx.gemm_tall(dev(A), dev(C)).cpu().numpy()
+        ref = orc.gemm_tall_f64(A.astype(np.float64), C, 1.0, 0.0, np.zeros((n, kc)))
+        tol = 1e-6 if dt == np.float32 else 1e-12
+        scale = np.abs(A.astype(np.float64)) @ np.abs(C)
+        assert np.all(np.abs(Y - ref) <= tol * scale + 1e-300)
+
+
 @pytest.mark.parametrize("m,ka,kc", [(96, 96, 16), (96, 80, 32), (96, 96, 48), (33, 33, 8), (16, 16, 16)])
 def test_gemm_tall_in_place(fmx, orc, m, ka, kc):
     """Y = A C written over A's own first kc columns (the thick restart's
