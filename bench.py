@@ -283,7 +283,7 @@ def cpu_baseline_extra(F, torch, data, offs, big, n, Xh, threads, alg_bytes, rep
 
 
 def construction_leg(F, torch, spm, t_build, check=True):
-    """Tile construction as the survey defines its bytes: 8*m (the edge list
+    """Tile construction, counted as algorithmic bytes: 8*m (the edge list
     read) + the tile bytes + 8*(block rows + 1) (written), over the device
     time of the construction kernels (fmx_spm_info.build_ns); beside it the
     whole fmx_spm_build call (host validation and the work images

@@ -14,7 +14,7 @@ random block (matrix/eigensolver/FM_MultiVector.h:164-168).
 
     python -m flashx_amd.learned train --graphs 200 --out flashx_amd/weights/warmstart.npy
     python -m flashx_amd.learned eval --weights flashx_amd/weights/warmstart.npy --graphs 20
-    python -m flashx_amd.learned analyse --graphs 3 --report gpurun_out/warmstart_analysis.json
+    python -m flashx_amd.learned analyse --graphs 3 --report warmstart_analysis.json
 """
 import argparse
 import ctypes
