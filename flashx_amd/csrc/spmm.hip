@@ -659,10 +659,10 @@ __device__ __forceinline__ ItemState item_state(const SpmmArgs &a, const Item &i
 
 // ---------------------------------------------------------------------------
 // The SpMM kernel: chunks of 64*R words, each decoded and then drained
-// batch by batch.  The next chunk's words and window are requested before
-// the drain, so their latency hides behind the gathers; the gathers of one
-// wave are not overlapped with its own decode, other resident waves fill that
-// gap (four per SIMD where LDS allows).
+// batch by batch.  The next chunk's words and window are requested behind
+// the first batch's gathers, so their latency overlaps the gathers'; the
+// gathers of one wave are not overlapped with its own decode, other resident
+// waves fill that gap (four per SIMD where LDS allows).
 // SEQ: the sequential mode (accumulate_batch_seq): the band holds T and
 // every element is summed in the reference's order; the plan of such a call
 // neither splits nor rotates items.
@@ -709,21 +709,34 @@ spmm_lane(SpmmArgs a)
         wave_sync();
         const uint32_t qn = decode_chunk<R, SEGP>(st.ic, c, wreg, win, st.carry, qcol, qrow, lane);
         const uint64_t cn = c + CW;
-        if (cn < abs_end) {
-            st.ct += window_advance<DPL>(dreg, live, (uint32_t)((int64_t)cn - (int64_t)st.ic.bw)) - 1;
-            load_words<R>(a.img, cn, lastw, lane, wreg);
-            load_window<DPL>(a.dir, st.ct, st.dlast, lane, dreg, live);
-        }
+        // the next chunk's words and window (this chunk's are in LDS now)
+        auto fetch_next = [&]() {
+            if (cn < abs_end) {
+                st.ct += window_advance<DPL>(dreg, live, (uint32_t)((int64_t)cn - (int64_t)st.ic.bw)) - 1;
+                load_words<R>(a.img, cn, lastw, lane, wreg);
+                load_window<DPL>(a.dir, st.ct, st.dlast, lane, dreg, live);
+            }
+        };
+        if constexpr (K == 0) fetch_next();
         wave_sync();
         if constexpr (K > 0) {
+            // requested behind the first batch's gathers: a wait for the
+            // gathers need not wait for these, and their latency overlaps
+            // the gathers' instead of preceding them
             using C = Lanes<T, K>;
+            bool fetched = false;
             for (uint32_t base = 0; base < qn; base += C::NZP * P) {
                 Vec<T, C::E> v[P];
                 uint32_t rr[P];
                 gather_batch<T, K, P, A, HOT>(a, qcol, qrow, base, qn, lane, v, rr);
+                if (!fetched) {
+                    fetch_next();
+                    fetched = true;
+                }
                 if constexpr (SEQ) accumulate_batch_seq<T, K, P>(band, S, lane, v, rr);
                 else accumulate_batch<T, K, P, A>(band, S, lane, v, rr);
             }
+            if (!fetched) fetch_next();
         } else if (qn) {
             if constexpr (SEQ) drain_generic_seq<T, T, P, false>(a, band, S, qcol, qrow, nullptr, nullptr, qn, lane);
             else drain_generic<T, P, A, HOT>(a, band, S, qcol, qrow, qn, lane);
