@@ -119,3 +119,31 @@ def test_warm_start_from_solution(fmx, orc):
     assert st1["converged"] == 1 and st1["spmm_blocks"] <= st0["spmm_blocks"]
     assert st1["restarts"] == 0
     assert np.allclose(ev, ev2, atol=1e-9)
+
+
+def test_ill_conditioned_start_block(fmx, orc):
+    """A start block whose 16 columns are one random vector plus 1e-4
+    relative perturbations (condition number about 1e4-1e5, above the bound
+    under which an FP32 block's Cholesky-QR stops after one pass) takes the
+    second pass and must still give a converged, orthonormal solve that
+    agrees with eigsh."""
+    import torch
+    s, d = fmx.rmat_edges(13, 16 << 13, 77)
+    flags = fmx.BUILD_DEDUP | fmx.BUILD_SYMMETRIZE
+    n = 1 << 13
+    spm = fmx.SparseMatrix.build(s, d, n=n, log2_h=8, log2_w=13, flags=flags)
+    rng = np.random.default_rng(5)
+    base = rng.random(n)
+    X0 = base[:, None] * (1.0 + 1e-4 * rng.standard_normal((n, 16)))
+    G = X0.T @ X0
+    assert np.linalg.cond(G) > 1e7  # condition of the block above 1e3.5
+    X0 = torch.from_numpy(X0.astype(np.float32)).cuda()
+    evals, vecs, resid, st = fmx.compute_eigen(spm, nev=16, tol=1e-6, precision=fmx.EIG_FP32, X0=X0,
+                                               max_restarts=300)
+    assert st["converged"] == 1 and np.all(resid <= 1e-6)
+    img = orc.build_2d(s.cpu().numpy().view(np.uint32), d.cpu().numpy().view(np.uint32),
+                       n=n, log2h=8, log2w=13, flags=orc.DEDUP | orc.SYMMETRIZE)
+    N = normalized_operator(orc, img)
+    ref = np.sort(sla.eigsh(N, k=16, which="LA", tol=1e-12, return_eigenvectors=False))[::-1]
+    assert np.allclose(evals, ref, atol=2e-6)
+    check_pairs(N, evals, vecs, 1e-6)
