@@ -147,3 +147,22 @@ def test_ill_conditioned_start_block(fmx, orc):
     ref = np.sort(sla.eigsh(N, k=16, which="LA", tol=1e-12, return_eigenvectors=False))[::-1]
     assert np.allclose(evals, ref, atol=2e-6)
     check_pairs(N, evals, vecs, 1e-6)
+
+
+@pytest.mark.parametrize("num_blocks", [7, 8])
+def test_largest_bases(fmx, orc, num_blocks):
+    """Bases of 112 and 128 vectors (the largest Rayleigh-Ritz problem the
+    one-workgroup Jacobi takes) against the torus's closed form in FP64."""
+    p, q = 40, 37
+    src, dst = torus_edges(p, q)
+    flags = fmx.BUILD_DEDUP | fmx.BUILD_SYMMETRIZE
+    spm = fmx.SparseMatrix.build(dev_u32(src), dev_u32(dst), n=p * q, log2_h=8, log2_w=10, flags=flags)
+    nev, tol = 8, 1e-10
+    evals, vecs, resid, st = fmx.compute_eigen(spm, nev=nev, block_size=16, num_blocks=num_blocks, tol=tol,
+                                               precision=fmx.EIG_FP64, max_restarts=300)
+    assert st["converged"] == 1
+    lam = np.sort(((np.cos(2 * np.pi * np.arange(p) / p)[:, None] +
+                    np.cos(2 * np.pi * np.arange(q) / q)[None, :]) / 2).ravel())[::-1]
+    assert np.allclose(evals, lam[:nev], atol=10 * tol)
+    img = orc.build_2d(src, dst, n=p * q, log2h=8, log2w=10, flags=orc.DEDUP | orc.SYMMETRIZE)
+    check_pairs(normalized_operator(orc, img), evals, vecs, tol)
