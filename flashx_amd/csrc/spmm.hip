@@ -667,7 +667,7 @@ __device__ __forceinline__ ItemState item_state(const SpmmArgs &a, const Item &i
 // every element is summed in the reference's order; the plan of such a call
 // neither splits nor rotates items.
 template <typename T, int K, int R, int P, bool SEQ, bool HOT>
-__global__ void __launch_bounds__(64 * kWPB) __attribute__((amdgpu_waves_per_eu(4, 4)))
+__global__ void __launch_bounds__(64 * kWPB) __attribute__((amdgpu_waves_per_eu(5, 5)))
 spmm_lane(SpmmArgs a)
 {
     using LC = LaneCfg<R>;
