@@ -374,10 +374,10 @@ __device__ __forceinline__ void gemm_stage_body(const T *A, int64_t lda, uint64_
     for (uint64_t r = r0; r < r1; r += kStageRows) {
         sa.template store<LA>(As, lane);
         wave_lds_sync();
-        // This stage's old Y values (and row scales), loaded from in-bounds
-        // addresses whatever the rows and beta, and requested before the
-        // next stage's rows: the epilogue's wait for them then leaves those
-        // rows in flight through the MFMAs.
+        // This stage's old Y values (read when beta != 0) and row scales,
+        // loaded from in-bounds addresses whatever the rows, and requested
+        // before the next stage's rows: the epilogue's wait for them then
+        // leaves those rows in flight through the MFMAs.
         // (SC: whether the row-scaled copy Ys is written, 1 or 0 when known
         // at compile time, else -1; a run-time branch here would wait for
         // these loads at once)
