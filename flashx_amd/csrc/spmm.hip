@@ -958,27 +958,36 @@ spmm_lane_w(SpmmArgs a)
         const uint32_t qn = decode_chunk_w<R, SEGP>(st.ic, c, wreg, win, wvb, wq, c0, a.vwords, st.carry,
                                                     qcol, qrow, qval, lane);
         const uint64_t cn = c + CW;
-        if (cn < abs_end) {
-            const uint32_t pn = (uint32_t)((int64_t)cn - (int64_t)st.ic.bw);
-            const uint32_t adv = window_advance<DPL>(dreg, live, pn);
-            // the next chunk's window tile 0 is window tile adv-1; one that
-            // starts exactly at the next chunk's first word has no columns in
-            // this chunk, and its wq slot was not written by this decode
-            c0 = win[adv - 1].x == pn ? 0 : (int)qn - wq[adv - 1];
-            st.ct += adv - 1;
-            load_words<R>(a.img, cn, lastw, lane, wreg);
-            load_window<DPL>(a.dir, st.ct, st.dlast, lane, dreg, live);
-#pragma unroll
-            for (int h = 0; h < DPL; h++) {
-                const uint32_t t = st.ct + h * 64 + lane;
-                vreg[h] = a.vdir[t < st.dlast ? t : st.dlast];
-            }
-        }
         wave_sync();
+        // the next chunk's words, window and value-directory entries (this
+        // chunk's are in LDS now)
+        auto fetch_next = [&]() {
+            if (cn < abs_end) {
+                const uint32_t pn = (uint32_t)((int64_t)cn - (int64_t)st.ic.bw);
+                const uint32_t adv = window_advance<DPL>(dreg, live, pn);
+                // the next chunk's window tile 0 is window tile adv-1; one
+                // that starts exactly at the next chunk's first word has no
+                // columns in this chunk, and its wq slot was not written by
+                // this decode
+                c0 = win[adv - 1].x == pn ? 0 : (int)qn - wq[adv - 1];
+                st.ct += adv - 1;
+                load_words<R>(a.img, cn, lastw, lane, wreg);
+                load_window<DPL>(a.dir, st.ct, st.dlast, lane, dreg, live);
+#pragma unroll
+                for (int h = 0; h < DPL; h++) {
+                    const uint32_t t = st.ct + h * 64 + lane;
+                    vreg[h] = a.vdir[t < st.dlast ? t : st.dlast];
+                }
+            }
+        };
+        if constexpr (K == 0) fetch_next();
         if constexpr (K > 0) {
+            // requested behind the first batch's gathers and value loads (as
+            // in spmm_lane)
             using C = Lanes<T, K>;
             const int q = lane % C::L;
             const int g = lane / C::L;
+            bool fetched = false;
             for (uint32_t base = 0; base < qn; base += C::NZP * P) {
                 Vec<T, C::E> v[P];
                 uint32_t rr[P], gcs[P];
@@ -997,6 +1006,10 @@ spmm_lane_w(SpmmArgs a)
 #pragma unroll
                 for (int pp = 0; pp < P; pp++)
                     v[pp] = *(const Vec<T, C::E> *)(xrow<T, HOT>(a, gcs[pp]) + q * C::E);
+                if (!fetched) {
+                    fetch_next();
+                    fetched = true;
+                }
                 if constexpr (SEQ) {
                     // products rounded in T, then added in order
 #pragma clang fp contract(off)
@@ -1033,6 +1046,7 @@ spmm_lane_w(SpmmArgs a)
                     for (int i = 0; i < C::E; i++) lds_add(dst + i * C::L, acc[i]);
                 }
             }
+            if (!fetched) fetch_next();
         } else if (SEQ) {
             if (qn) drain_generic_seq<T, V, P, true>(a, (T *)band, S, qcol, qrow, qval, vimg, qn, lane);
         } else {
